@@ -1,0 +1,132 @@
+"""The PP oracle pinned against the reference's golden vectors and against
+closed forms (CPU only)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_pp_case
+from cube2_amd.geometry import APM3, RATIO_SF, REFERENCE_DEFAULT, C0, ixyz3
+from cube2_amd.layout import build_layout
+
+
+def test_fra_matches_reference_table(oracle_lib):
+    z = np.load(GOLDEN / "fra_table.npz")
+    got = oracle_lib.F_ra(z["r"], z["apm"])
+    assert np.array_equal(got.view(np.uint32), z["f"].view(np.uint32))
+
+
+@pytest.mark.parametrize("a", [0.05, 0.06, 0.4375, 1.75])
+def test_fra_pieces_join_continuously(oracle_lib, a):
+    # ep = 1 and ep = 2 are the joins of the three pieces; a wrong
+    # coefficient shows up as a jump there.
+    for ep in (1.0, 2.0):
+        r0 = np.float32(ep * a / 2)
+        lo = oracle_lib.F_ra([np.nextafter(r0, np.float32(0))], a)[0]
+        hi = oracle_lib.F_ra([np.nextafter(r0, np.float32(9))], a)[0]
+        assert abs(lo - hi) <= 2e-4 * abs(hi)
+    r = np.float32(2 * a)
+    assert oracle_lib.F_ra([r], a)[0] == np.float32(1) / (r * r)
+
+
+def test_stable_sort_matches_reference_radix(oracle_lib):
+    z = np.load(GOLDEN / "sort_cases.npz")
+    for s in range(4):
+        k, bound = z[f"keys{s}"], int(z[f"bound{s}"])
+        ks, vs = oracle_lib.stable_sort(k, np.arange(k.size, dtype=np.uint32), bound)
+        assert np.array_equal(ks, z[f"skeys{s}"])
+        assert np.array_equal(vs, z[f"svals{s}"])
+        assert np.array_equal(vs, np.argsort(k, kind="stable"))
+
+
+@pytest.mark.parametrize("name", ["pp_case1", "pp_case2"])
+def test_pp_oracle_bit_exact_vs_reference(oracle_lib, name):
+    c = load_pp_case(name)
+    g = REFERENCE_DEFAULT
+    ires = np.full(g.ns3, 2, np.int32)
+    o = oracle_lib.pp_force(g, ires, ixyz3(g), APM3, RATIO_SF, c["rhoc"], c["idx_b_r"], c["xp"],
+                            c["vp_in"], c["mass"], c["a_mid"], c["dt"])
+    assert np.array_equal(o["vp"].view(np.uint32), c["vp_out"].view(np.uint32))
+    assert np.float32(o["f2max"]) == np.float32(c["f2max"])
+    assert np.array_equal(o["vmax"], c["vmax"])
+    # the reference C kernel counts per parent tile, numbered x-slowest
+    per_tile = o["ptotal"].reshape(g.nnt, g.nnt, g.nnt, -1).sum(-1)      # [tz,ty,tx]
+    assert np.array_equal(per_tile, c["ptotal_tile"].reshape(g.nnt, g.nnt, g.nnt).transpose(2, 1, 0))
+
+
+def _two_body(oracle_lib, sep, mass=1.3):
+    g = C0
+    pos = np.array([[30.3, 31.1, 29.7], [30.3 + sep[0], 31.1 + sep[1], 29.7 + sep[2]]])
+    lay = build_layout(pos, np.zeros((2, 3), np.float32), g)
+    ires = np.full(g.ns3, 2, np.int32)
+    o = oracle_lib.pp_force(g, ires, ixyz3(g), APM3, RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
+                            lay.vp, mass, 0.02, 1e-3)
+    return lay, o
+
+
+def test_two_body_closed_form(oracle_lib):
+    lay, o = _two_body(oracle_lib, (0.9, -0.4, 0.3))
+    p = lay.decoded_positions()
+    phys = lay.physical_mask()
+    i0 = np.flatnonzero(phys & (lay.src == 0))[0]
+    i1 = np.flatnonzero(phys & (lay.src == 1))[0]
+    # FP64 evaluation of the same two-body kernel
+    rng = float(APM3[1])
+    app = C0.app
+    acc = oracle_lib.direct_pp_f64(p[[i0, i1]], C0.ng, [0], [rng], app)[0]
+    f = acc * 1.3 ** 2 * 0.02 * 1e-3 / (6 * np.pi)
+    dv0 = o["vp"][i0].astype(np.float64)
+    assert np.allclose(dv0, f, rtol=2e-5, atol=1e-12)
+    assert np.allclose(o["vp"][i1], -o["vp"][i0], rtol=1e-6)
+    assert o["p_acc"] == 2
+
+
+def test_pair_sum_matches_fp64_direct_sum_and_conserves_momentum(oracle_lib):
+    g = C0
+    rng = np.random.default_rng(3)
+    pos = np.concatenate([rng.uniform(0, 64, (20000, 3)), 32 + rng.normal(0, 1.0, (3000, 3))])
+    lay = build_layout(pos, np.zeros((pos.shape[0], 3), np.float32), g)
+    ires = np.full(g.ns3, 2, np.int32)
+    mass, amid, dt = 1.0, 0.02, 1e-3
+    o = oracle_lib.pp_force(g, ires, ixyz3(g), APM3, RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
+                            lay.vp, mass, amid, dt)
+    phys = np.flatnonzero(lay.physical_mask())
+    dv = o["vp"][phys].astype(np.float64)
+    # every pair is counted twice with opposite signs when all are active
+    assert np.abs(dv.sum(0)).max() < 1e-6 * np.abs(dv).sum()
+    # FP64 direct sum over the decoded positions of the physical slots
+    p = lay.decoded_positions()[phys]
+    sub = rng.choice(phys.size, 400, replace=False).astype(np.int32)
+    acc = oracle_lib.direct_pp_f64(p, g.ng, sub, np.full(sub.size, float(APM3[1])), g.app)
+    ref = acc * mass * mass * amid * dt / (6 * np.pi)
+    err = np.sqrt(((dv[sub] - ref) ** 2).sum() / (ref ** 2).sum())
+    assert err < 2e-6
+
+
+def test_level_selection_thresholds_and_isort(oracle_lib):
+    g = REFERENCE_DEFAULT.with_(nnt=1, ng=128, nns=4)      # 64 subtiles, ntt=8
+    ix = ixyz3(g)
+    rhoc = np.zeros((1, g.nte, g.nte, g.nte), np.int32)
+    # subtile s gets max count m_s in one physical cell
+    maxima = {0: 1535, 1: 1536, 2: 7295, 3: 7296, 4: 20479, 5: 20480, 6: 86399, 7: 86400, 8: 0}
+    for s, m in maxima.items():
+        sx, sy, sz = ix[s, :3] - 1
+        rhoc[0, g.ncb + sz * g.ntt + 3, g.ncb + sy * g.ntt + 2, g.ncb + sx * g.ntt + 1] = m
+    ires, isort, napm, npc = oracle_lib.select_levels(g, rhoc, ix)
+    want = {0: 2, 1: 3, 2: 3, 3: 4, 4: 4, 5: 5, 6: 5, 7: 6, 8: 2}
+    for s, L in want.items():
+        assert ires[s] == L, (s, ires[s], L)
+    assert napm.sum() == g.ns3
+    # isort lists subtiles by non-increasing npc_max (heapsort of -npc_max)
+    assert np.all(np.diff(npc[isort - 1]) <= 0)
+    assert sorted(isort.tolist()) == list(range(1, g.ns3 + 1))
+
+
+def test_indexed_sort_orders_and_permutes(oracle_lib):
+    # Heapsort is not stable; ties only change the PM3 job order, never a
+    # result, so the pin is: a permutation that orders the keys.
+    rng = np.random.default_rng(4)
+    arr = rng.integers(-6, 0, 301).astype(np.float32)
+    idx = oracle_lib.indexed_sort(arr)
+    assert np.all(np.diff(arr[idx - 1]) >= 0)
+    assert sorted(idx.tolist()) == list(range(1, arr.size + 1))
