@@ -1,0 +1,135 @@
+"""ctypes binding of libcube2_short.so (the HIP library).
+
+The product path has no fallback: if the library is missing or fails to
+load, every call raises.  Device buffers are passed as raw pointers
+(e.g. ``torch.Tensor.data_ptr()`` of a ROCm tensor); no torch types cross
+the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libcube2_short.so"
+_LIB = None
+
+OK, ERR_ARG, ERR_GEOMETRY, ERR_CAPACITY, ERR_HIP, ERR_STATE = range(6)
+PP_EXACT, PP_FAST = 0, 1
+
+
+class Cube2Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cube2 error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("ng", C.c_int), ("nnt", C.c_int), ("nns", C.c_int), ("ratio_cs", C.c_int),
+                ("np_nc", C.c_int), ("ngb", C.c_int), ("ncore", C.c_int),
+                ("body_centered_cubic", C.c_int), ("image_buffer", C.c_double),
+                ("tile_buffer", C.c_double), ("app", C.c_double)]
+
+
+vp_ = C.c_void_p
+i32 = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); the exported surface of include/cube2_short.h
+SIGNATURES = {
+    "c_pp_force_kernel": (None, [vp_] * 9 + [C.c_float] * 3 + [vp_] * 4),
+    "c_pm3_force_kernel": (None, [vp_] * 13 + [C.c_float] * 3 + [vp_] * 3),
+    "c_pm3_fft_finalize": (None, []),
+    "c_gpu_velocity_finalize": (None, []),
+    "cube2_set_params": (C.c_int, [C.POINTER(Params)]),
+    "cube2_get_params": (C.c_int, [C.POINTER(Params)]),
+    "cube2_set_pp_mode": (C.c_int, [C.c_int]),
+    "cube2_pm3_force_dev": (C.c_int, [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3 + [vp_, vp_]),
+    "cube2_pp_force_dev": (C.c_int, [vp_] * 8 + [C.c_int64] + [C.c_float] * 3 + [vp_] * 5),
+    "cube2_velocity_apply_dev": (C.c_int, [vp_, vp_, vp_, C.c_int64, vp_]),
+    "cube2_green_3d_dev": (C.c_int, [vp_, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "cube2_select_levels": (C.c_int, [vp_] * 5),
+    "cube2_pp_debug_celllist": (C.c_int, [C.c_int] + [vp_] * 6),
+    "cube2_pm3_debug_job": (C.c_int, [C.c_int] + [vp_] * 6 + [C.c_int64, vp_, C.c_float] + [vp_] * 4),
+    "cube2_fft3d_dev": (C.c_int, [vp_, C.c_int, C.c_int, C.c_int]),
+    "cube2_profile_enable": (C.c_int, [C.c_int]),
+    "cube2_profile_reset": (C.c_int, []),
+    "cube2_profile_get": (C.c_int, [C.c_char_p, vp_, vp_, vp_, vp_]),
+    "cube2_profile_stages": (C.c_int, [C.c_char_p, C.c_int]),
+    "cube2_synchronize": (C.c_int, []),
+    "cube2_last_error": (C.c_char_p, []),
+}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() or python cube2_amd/build.py")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise Cube2Error(rc, lib().cube2_last_error().decode(errors="replace"))
+
+
+def ptr(x) -> int | None:
+    """Address of a numpy array or a torch tensor (host or device)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        assert x.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):
+        assert x.is_contiguous(), "tensor must be contiguous"
+        return x.data_ptr()
+    if isinstance(x, int):
+        return x
+    raise TypeError(type(x))
+
+
+def set_params(geom) -> None:
+    p = Params(geom.ng, geom.nnt, geom.nns, geom.ratio_cs, geom.np_nc, geom.ngb, geom.ncore,
+               int(geom.body_centered_cubic), geom.image_buffer, geom.tile_buffer, geom.app)
+    check(lib().cube2_set_params(C.byref(p)))
+
+
+def get_params() -> Params:
+    p = Params()
+    check(lib().cube2_get_params(C.byref(p)))
+    return p
+
+
+def set_pp_mode(mode: int) -> None:
+    check(lib().cube2_set_pp_mode(int(mode)))
+
+
+def profile(on: bool = True) -> None:
+    check(lib().cube2_profile_enable(int(on)))
+
+
+def profile_reset() -> None:
+    check(lib().cube2_profile_reset())
+
+
+def profile_stats() -> dict:
+    buf = C.create_string_buffer(4096)
+    check(lib().cube2_profile_stages(buf, 4096))
+    out = {}
+    for name in filter(None, buf.value.decode().split(",")):
+        ms, n, b, f = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
+        check(lib().cube2_profile_get(name.encode(), C.byref(ms), C.byref(n), C.byref(b), C.byref(f)))
+        out[name] = dict(ms=ms.value, launches=n.value, bytes=b.value, flops=f.value)
+    return out
+
+
+def synchronize() -> None:
+    check(lib().cube2_synchronize())

@@ -1,0 +1,281 @@
+// C-ABI entry points: the reference's four bind(C) symbols (host pointers,
+// fatal errors) and the cube2_* device-resident API (status codes).
+#include <cstdlib>
+#include <mutex>
+
+#include "common.h"
+#include "profile.h"
+
+namespace cube2 {
+int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+             const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
+             int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
+             double* ptotal, double* ttotal, uint64_t* pair_counts);
+int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf);
+int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t* slot,
+                      int32_t* cell_start);
+void pp_release();
+int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
+              const float* const* d_green, float mass, float amid, float dt, float* f2max,
+              double* ptotal);
+int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf, const int* d_rhoc,
+                  const int64_t* d_idx, const int16_t* d_xp, int64_t n_used, const float* d_green,
+                  float mass, float* rho, float* phi, float* force, float* f2max);
+int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t n_used,
+                   float* vmax);
+int green_3d(float* d_G, int n, float apm_soft, float apm_range, float gridsize);
+int fft3d(float* d, int n, int nb, int dir);
+void pm3_release();
+int green_cache_get(const float* host, size_t count, const float** dev);
+void green_cache_release();
+}  // namespace cube2
+
+using namespace cube2;
+
+static std::recursive_mutex g_mx;
+
+// Level selection of kick.f90:107-131; job order = heapsort of -npc_max
+// (kick.f90:570-614), restated.
+static void heap_index(int n, const float* arr, int* indx) {
+    for (int j = 1; j <= n; j++) indx[j - 1] = j;
+    if (n < 2) return;
+    auto A = [&](int q) { return arr[q - 1]; };
+    auto IX = [&](int q) -> int& { return indx[q - 1]; };
+    int l = n / 2 + 1, ir = n;
+    for (;;) {
+        int it;
+        float qv;
+        if (l > 1) {
+            it = IX(--l);
+            qv = A(it);
+        } else {
+            it = IX(ir);
+            qv = A(it);
+            IX(ir) = IX(1);
+            if (--ir == 1) {
+                IX(1) = it;
+                return;
+            }
+        }
+        int i = l, j = l + l;
+        while (j <= ir) {
+            if (j < ir && A(IX(j)) < A(IX(j + 1))) j++;
+            if (qv < A(IX(j))) {
+                IX(i) = IX(j);
+                i = j;
+                j += j;
+            } else {
+                j = ir + 1;
+            }
+        }
+        IX(i) = it;
+    }
+}
+
+static int64_t n_used_of(const Geom& g, const int64_t* idx_b_r) {
+    return idx_b_r[(size_t)g.ntile * g.nte * g.nte - 1];
+}
+
+// Stage host arrays for the drop-in entry points.
+static int stage_inputs(const int* rhoc, const int64_t* idx_b_r, const int16_t* xp, bool reuse_xp,
+                        int64_t* n_used) {
+    Context& C = ctx();
+    const Geom& g = C.g;
+    hipStream_t s = C.stream;
+    const int64_t n = n_used_of(g, idx_b_r);
+    if (n < 0 || n >= ((int64_t)1 << 31)) {
+        set_error("invalid particle count %lld", (long long)n);
+        return CUBE2_ERR_CAPACITY;
+    }
+    const size_t nr = (size_t)g.ntile * g.nte * g.nte * g.nte, ni = (size_t)g.ntile * g.nte * g.nte;
+    C2_TRY(C.d_rhoc.reserve(nr * sizeof(int)));
+    C2_TRY(C.d_idx.reserve(ni * sizeof(int64_t)));
+    C2_HIP(hipMemcpyAsync(C.d_rhoc.ptr, rhoc, nr * sizeof(int), hipMemcpyHostToDevice, s));
+    C2_HIP(hipMemcpyAsync(C.d_idx.ptr, idx_b_r, ni * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    if (!(reuse_xp && C.staged_xp_host == xp && C.staged_xp_n == n)) {
+        C2_TRY(C.d_xp.reserve((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(int16_t)));
+        if (n) C2_HIP(hipMemcpyAsync(C.d_xp.ptr, xp, (size_t)n * 3 * sizeof(int16_t), hipMemcpyHostToDevice, s));
+        C.staged_xp_host = xp;
+        C.staged_xp_n = n;
+    }
+    *n_used = n;
+    return CUBE2_OK;
+}
+
+[[noreturn]] static void die(const char* who, bool abort_) {
+    fprintf(stderr, "%s: %s\n", who, last_error());
+    if (abort_) std::abort();
+    std::exit(EXIT_FAILURE);
+}
+
+extern "C" {
+
+void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* ratio_sf, int* rhoc,
+                       int64_t* idx_b_r, int16_t* xp, float* vp, float mass_p_cdm, float a_mid,
+                       float dt, float* f2max, float* vmax, double* ptotal, double* ttotal) {
+    (void)isort;  // PP updates are disjoint per subtile; order is irrelevant
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    if (pp_validate(ires, ixyz3, ratio_sf) != CUBE2_OK) die("[PP]", false);
+    if (ensure_init() != CUBE2_OK) die("[PP]", false);
+    Context& C = ctx();
+    int64_t n = 0;
+    if (stage_inputs(rhoc, idx_b_r, xp, C.dv_state == 1, &n) != CUBE2_OK) die("[PP]", false);
+    if (C.d_vp.reserve((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)) != CUBE2_OK) die("[PP]", false);
+    if (n && hipMemcpyAsync(C.d_vp.ptr, vp, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice,
+                            C.stream) != hipSuccess) {
+        set_error("H2D of vp failed");
+        die("[PP]", false);
+    }
+    if (pp_force(ires, ixyz3, apm3, ratio_sf, C.d_rhoc.as<int>(), C.d_idx.as<int64_t>(),
+                 C.d_xp.as<int16_t>(), C.d_vp.as<float>(), n, mass_p_cdm, a_mid, dt, f2max, vmax,
+                 ptotal, ttotal, nullptr) != CUBE2_OK)
+        die("[PP]", false);
+    if (n && hipMemcpy(vp, C.d_vp.ptr, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("D2H of vp failed");
+        die("[PP]", false);
+    }
+    C.staged_xp_host = nullptr;  // a new kick re-uploads positions
+}
+
+void c_pm3_force_kernel(int* isort, int* ires, int* ixyz3, int* ratio_sf, int* rhoc,
+                        int64_t* idx_b_r, int16_t* xp, float* g2, float* g4, float* g6, float* g8,
+                        float* g12, float* vp, float mass, float amid, float dt, float* f2max,
+                        float* vmax, double* ptotal) {
+    (void)vp;  // the Delta-v stays pending until c_pp_force_kernel (pm3_fft_cuda.cu:519)
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    if (ensure_init() != CUBE2_OK) die("PM3", true);
+    Context& C = ctx();
+    const Geom& g = C.g;
+    int64_t n = 0;
+    if (stage_inputs(rhoc, idx_b_r, xp, false, &n) != CUBE2_OK) die("PM3", true);
+    const float* host_g[5] = {g2, g4, g6, g8, g12};
+    const float* dev_g[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool need[5] = {false, false, false, false, false};
+    for (int it = 0; it < g.ns3; it++)
+        if (ires[it] >= 2 && ires[it] <= 6) need[ires[it] - 2] = true;
+    for (int L = 2; L <= 6; L++) {
+        if (!need[L - 2]) continue;
+        const int nn = (g.ntt * g.p.ratio_cs + 2 * g.p.ratio_cs) * ratio_sf[L - 1];
+        const size_t count = (size_t)(nn / 2 + 1) * nn * nn;
+        if (!host_g[L - 2] || green_cache_get(host_g[L - 2], count, &dev_g[L - 2]) != CUBE2_OK)
+            die("PM3", true);
+    }
+    if (pm3_force(isort, ires, ixyz3, ratio_sf, C.d_rhoc.as<int>(), C.d_idx.as<int64_t>(),
+                  C.d_xp.as<int16_t>(), n, dev_g, mass, amid, dt, f2max, ptotal) != CUBE2_OK)
+        die("PM3", true);
+    vmax[0] = vmax[1] = vmax[2] = 0.0f;  // pm3_fft_cuda.cu:552
+}
+
+void c_pm3_fft_finalize(void) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    pm3_release();
+    green_cache_release();
+    Context& C = ctx();
+    C.d_xp.release();
+    C.staged_xp_host = nullptr;
+}
+
+void c_gpu_velocity_finalize(void) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    pp_release();
+    Context& C = ctx();
+    C.dv64.release();
+    C.d_vp.release();
+    C.d_rhoc.release();
+    C.d_idx.release();
+    C.dv_state = 0;
+    C.dv_n_used = 0;
+}
+
+int cube2_pm3_force_dev(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+                        const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                        int64_t n_used, const float* const* d_green, float mass, float amid,
+                        float dt, float* f2max, double* ptotal) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return pm3_force(isort, ires, ixyz3, ratio_sf, d_rhoc, d_idx_b_r, d_xp, n_used, d_green, mass,
+                     amid, dt, f2max, ptotal);
+}
+
+int cube2_pp_force_dev(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+                       const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                       float* d_vp, int64_t n_used, float mass, float a_mid, float dt,
+                       float* f2max, float* vmax, double* ptotal, double* ttotal,
+                       uint64_t* pair_counts) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return pp_force(ires, ixyz3, apm3, ratio_sf, d_rhoc, d_idx_b_r, d_xp, d_vp, n_used, mass,
+                    a_mid, dt, f2max, vmax, ptotal, ttotal, pair_counts);
+}
+
+int cube2_velocity_apply_dev(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp,
+                             int64_t n_used, float* vmax) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return velocity_apply(d_rhoc, d_idx_b_r, d_vp, n_used, vmax);
+}
+
+int cube2_green_3d_dev(float* d_G, int n, float apm_soft, float apm_range, float gridsize) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return green_3d(d_G, n, apm_soft, apm_range, gridsize);
+}
+
+int cube2_fft3d_dev(float* d_data, int n, int nb, int dir) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return fft3d(d_data, n, nb, dir);
+}
+
+int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort, int* napm) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    Context& C = ctx();
+    if (!C.params_set) {
+        cube2_params_t p;
+        cube2_get_params(&p);
+    }
+    const Geom& g = C.g;
+    if (!rhoc || !ixyz3 || !ires || !isort) {
+        set_error("null argument");
+        return CUBE2_ERR_ARG;
+    }
+    std::vector<float> neg(g.ns3);
+    int nap[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int it = 0; it < g.ns3; it++) {
+        const int* x = ixyz3 + 6 * it;
+        for (int q = 0; q < 3; q++)
+            if (x[q] < 1 || x[q] > g.p.nns || x[q + 3] < 1 || x[q + 3] > g.p.nnt) {
+                set_error("invalid ixyz3 row %d", it);
+                return CUBE2_ERR_ARG;
+            }
+        const int t = ((x[5] - 1) * g.p.nnt + (x[4] - 1)) * g.p.nnt + (x[3] - 1);
+        const int* rc = rhoc + (size_t)t * g.nte * g.nte * g.nte;
+        int mx = 0;
+        for (int k = (x[2] - 1) * g.ntt; k < x[2] * g.ntt; k++)
+            for (int j = (x[1] - 1) * g.ntt; j < x[1] * g.ntt; j++)
+                for (int i = (x[0] - 1) * g.ntt; i < x[0] * g.ntt; i++)
+                    mx = std::max(mx, rc[((size_t)(k + g.ncb) * g.nte + (j + g.ncb)) * g.nte + (i + g.ncb)]);
+        const float p = (float)mx / 64.0f;  // exact: thresholds are maxval < 1536/7296/20480/86400
+        int L = p < 24.0f ? 2 : p < 114.0f ? 3 : p < 320.0f ? 4 : p < 1350.0f ? 5 : 6;
+        ires[it] = L;
+        nap[L - 1]++;
+        neg[it] = -p;
+    }
+    heap_index(g.ns3, neg.data(), isort);
+    if (napm)
+        for (int q = 0; q < 7; q++) napm[q] = nap[q];
+    return CUBE2_OK;
+}
+
+int cube2_pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys_sorted,
+                            uint32_t* slot, int32_t* cell_start) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return pp_debug_celllist(it, rcp, S, N, keys_sorted, slot, cell_start);
+}
+
+int cube2_pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf,
+                        const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                        int64_t n_used, const float* d_green, float mass, float* rho, float* phi,
+                        float* force, float* f2max) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return pm3_debug_job(it, ires, ixyz3, ratio_sf, d_rhoc, d_idx_b_r, d_xp, n_used, d_green, mass,
+                         rho, phi, force, f2max);
+}
+
+}  // extern "C"

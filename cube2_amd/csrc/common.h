@@ -1,0 +1,97 @@
+// Shared definitions of the MI355X short-range force library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cube2_short.h"
+
+namespace cube2 {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define C2_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            ::cube2::set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_),  \
+                               __FILE__, __LINE__, #call);                           \
+            return CUBE2_ERR_HIP;                                                    \
+        }                                                                            \
+    } while (0)
+
+#define C2_TRY(expr)                  \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != CUBE2_OK) return rc_; \
+    } while (0)
+
+// ------------------------------------------------------------- geometry
+struct Geom {
+    cube2_params_t p;
+    int nc, nt, ntt, ncb, nte, nsub, ns3, ntile;
+};
+Geom make_geom(const cube2_params_t& p);
+
+// --------------------------------------------------- device memory pool
+// Grow-only device buffers owned by the library (freed by the finalizers).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ------------------------------------------------------------ profiling
+// Per-stage GPU time, recorded with events on the library stream when
+// profiling is on (cube2_profile_enable).
+struct StageTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    const char* name = nullptr;
+    bool on = false;
+    void begin(const char* stage, hipStream_t s);
+    void end(hipStream_t s);
+};
+void profile_add(const char* stage, double ms, double bytes, double flops);
+bool profiling();
+
+// --------------------------------------------------------- context
+struct Context {
+    bool params_set = false;
+    Geom g{};
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int pp_mode = CUBE2_PP_FAST;
+    // velocity accumulator (dv64, FP64, 3 per slot) shared by PM3 and PP
+    DevBuf dv64;
+    int64_t dv_n_used = 0;
+    int dv_state = 0;  // 0 none pending, 1 PM3 has written a pending Delta-v
+    // host-pointer drop-in staging
+    DevBuf d_xp, d_vp, d_rhoc, d_idx;
+    const int16_t* staged_xp_host = nullptr;
+    int64_t staged_xp_n = -1;
+};
+Context& ctx();
+int ensure_init();   // params + HIP device + library stream
+void ensure_params();  // params only (no HIP call)
+
+// ------------------------------------------------------------- helpers
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Quantised position of a particle inside its coarse cell, as the
+// reference decodes it: ((int16)(xp + ishift) + rshift) * x_resolution
+// with ishift = -2^15, rshift = 0.5 - ishift, x_resolution = 2^-16
+// (parameters.f90:87-88,118; pp_force_kernel.cu:764-769).
+__host__ __device__ inline double frac16(int16_t v) {
+    return ((double)(int16_t)((int64_t)v - 32768) + 32768.5) * (1.0 / 65536.0);
+}
+
+}  // namespace cube2

@@ -1,0 +1,269 @@
+// Library state: geometry, stream, errors, grow-only device buffers and
+// the per-stage event profiler.
+#include <cstdarg>
+#include <map>
+#include <mutex>
+
+#include "common.h"
+#include "profile.h"
+
+namespace cube2 {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+const char* last_error() { return g_err.c_str(); }
+
+Geom make_geom(const cube2_params_t& p) {
+    Geom g;
+    g.p = p;
+    g.nc = p.ng / p.ratio_cs;
+    g.nt = g.nc / p.nnt;
+    g.ntt = g.nt / p.nns;
+    g.ncb = p.ngb / p.ratio_cs;
+    g.nte = g.nt + 2 * g.ncb;
+    g.nsub = p.nnt * p.nns;
+    g.ns3 = g.nsub * g.nsub * g.nsub;
+    g.ntile = p.nnt * p.nnt * p.nnt;
+    return g;
+}
+
+int DevBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return CUBE2_OK;
+    if (ptr) {
+        C2_HIP(hipFree(ptr));
+        ptr = nullptr;
+        cap = 0;
+    }
+    size_t want = bytes + bytes / 8 + 256;
+    C2_HIP(hipMalloc(&ptr, want));
+    cap = want;
+    return CUBE2_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+Context& ctx() {
+    static Context c;
+    return c;
+}
+
+static cube2_params_t default_params() {
+    // The reference's compiled defaults (cuda_parameters_generated.h).
+    cube2_params_t p;
+    p.ng = 512;
+    p.nnt = 4;
+    p.nns = 4;
+    p.ratio_cs = 4;
+    p.np_nc = 4;
+    p.ngb = 16;
+    p.ncore = 32;
+    p.body_centered_cubic = 0;
+    p.image_buffer = 2.0;
+    p.tile_buffer = 3.0;
+    p.app = 0.06;
+    return p;
+}
+
+void ensure_params() {
+    Context& c = ctx();
+    if (!c.params_set) {
+        c.g = make_geom(default_params());
+        c.params_set = true;
+    }
+}
+
+int ensure_init() {
+    Context& c = ctx();
+    ensure_params();
+    if (!c.stream) {
+        int n = 0;
+        C2_HIP(hipGetDeviceCount(&n));
+        if (n < 1) {
+            set_error("no HIP device is visible");
+            return CUBE2_ERR_HIP;
+        }
+        C2_HIP(hipGetDevice(&c.device));
+        C2_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    }
+    return CUBE2_OK;
+}
+
+// ------------------------------------------------------------ profiling
+struct Stat {
+    double ms = 0, bytes = 0, flops = 0;
+    int64_t n = 0;
+};
+static std::map<std::string, Stat> g_stats;
+static bool g_prof = false;
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+    double bytes, flops;
+};
+static std::vector<Pending> g_pending;
+static std::vector<hipEvent_t> g_pool;
+
+bool profiling() { return g_prof; }
+
+static hipEvent_t get_event() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+Stage::Stage(const char* name, double bytes, double flops) {
+    if (!g_prof) return;
+    on_ = true;
+    name_ = name;
+    bytes_ = bytes;
+    flops_ = flops;
+    a_ = get_event();
+    (void)hipEventRecord(a_, ctx().stream);
+}
+
+void Stage::set_work(double bytes, double flops) {
+    bytes_ = bytes;
+    flops_ = flops;
+}
+
+Stage::~Stage() {
+    if (!on_) return;
+    hipEvent_t b = get_event();
+    (void)hipEventRecord(b, ctx().stream);
+    g_pending.push_back({name_, a_, b, bytes_, flops_});
+}
+
+void profile_flush() {
+    for (auto& p : g_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            Stat& s = g_stats[p.name];
+            s.ms += ms;
+            s.n += 1;
+            s.bytes += p.bytes;
+            s.flops += p.flops;
+        }
+        g_pool.push_back(p.a);
+        g_pool.push_back(p.b);
+    }
+    g_pending.clear();
+}
+
+void profile_add(const char* stage, double ms, double bytes, double flops) {
+    Stat& s = g_stats[stage];
+    s.ms += ms;
+    s.n += 1;
+    s.bytes += bytes;
+    s.flops += flops;
+}
+
+int sync_and_flush() {
+    C2_HIP(hipStreamSynchronize(ctx().stream));
+    profile_flush();
+    return CUBE2_OK;
+}
+
+}  // namespace cube2
+
+using namespace cube2;
+
+static std::mutex g_api_mutex;
+
+extern "C" {
+
+int cube2_set_params(const cube2_params_t* p) {
+    std::lock_guard<std::mutex> l(g_api_mutex);
+    if (!p || p->ng <= 0 || p->nnt <= 0 || p->nns <= 0 || p->ratio_cs <= 0 || p->ngb <= 0 ||
+        p->ng % p->ratio_cs || (p->ng / p->ratio_cs) % p->nnt ||
+        (p->ng / p->ratio_cs / p->nnt) % p->nns || p->ngb % p->ratio_cs || p->app < 0) {
+        set_error("inconsistent geometry");
+        return CUBE2_ERR_GEOMETRY;
+    }
+    Context& c = ctx();
+    c.g = make_geom(*p);
+    c.params_set = true;
+    c.dv_state = 0;
+    c.staged_xp_host = nullptr;
+    return CUBE2_OK;
+}
+
+int cube2_get_params(cube2_params_t* p) {
+    std::lock_guard<std::mutex> l(g_api_mutex);
+    Context& c = ctx();
+    if (!c.params_set) {
+        c.g = make_geom(default_params());
+        c.params_set = true;
+    }
+    *p = c.g.p;
+    return CUBE2_OK;
+}
+
+int cube2_set_pp_mode(int mode) {
+    if (mode != CUBE2_PP_EXACT && mode != CUBE2_PP_FAST) {
+        set_error("unknown PP mode %d", mode);
+        return CUBE2_ERR_ARG;
+    }
+    ctx().pp_mode = mode;
+    return CUBE2_OK;
+}
+
+int cube2_profile_enable(int on) {
+    g_prof = on != 0;
+    return CUBE2_OK;
+}
+
+int cube2_profile_reset(void) {
+    g_stats.clear();
+    return CUBE2_OK;
+}
+
+int cube2_profile_get(const char* stage, double* ms, int64_t* launches, double* bytes,
+                      double* flops) {
+    auto it = g_stats.find(stage ? stage : "");
+    if (it == g_stats.end()) {
+        set_error("no profile for stage %s", stage ? stage : "(null)");
+        return CUBE2_ERR_ARG;
+    }
+    if (ms) *ms = it->second.ms;
+    if (launches) *launches = it->second.n;
+    if (bytes) *bytes = it->second.bytes;
+    if (flops) *flops = it->second.flops;
+    return CUBE2_OK;
+}
+
+int cube2_profile_stages(char* buf, int len) {
+    std::string s;
+    for (auto& kv : g_stats) {
+        if (!s.empty()) s += ",";
+        s += kv.first;
+    }
+    if (!buf || len <= 0) return CUBE2_ERR_ARG;
+    snprintf(buf, (size_t)len, "%s", s.c_str());
+    return CUBE2_OK;
+}
+
+int cube2_synchronize(void) {
+    C2_TRY(ensure_init());
+    return sync_and_flush();
+}
+
+const char* cube2_last_error(void) { return last_error(); }
+
+}  // extern "C"

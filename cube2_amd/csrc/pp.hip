@@ -1,0 +1,509 @@
+// PP: cell lists by counting sort and the softened pair sum, all parent
+// tiles of the image in one pass.
+//
+// Reference semantics (pp_force_kernel.cu:625-958, pp_force_kernel.c:134-391):
+//   * a parent tile is gridded with PP cells of 1/rcp coarse cells, rcp the
+//     finest level among its subtiles (pp_force_kernel.cu:641-661);
+//   * particles of the tile's coarse cells plus one halo cell are binned by
+//     key = (i2*S + i1)*S + i0, i_d = floor(rcp*x_d) + rcp, S = nt*rcp + 2*rcp
+//     (pp_force_kernel.cu:764-810) and stably sorted (:858-861);
+//   * each physical particle i sums (F_ra(r,app) - F_ra(r,range_i)) d/r over
+//     the j of the (2R+1)^3 cells around it with 0 < r^2 < range_i^2,
+//     R = ceil(range_i*rcp/ratio_cs) (:192-250);
+//   * a *= m^2, f2max = max |a|^2, Delta-v = a*a_mid*dt/(6 pi) (:252-301).
+//
+// MI355X design: the particles of one coarse cell are contiguous in xp (the
+// spine layout) and all fall into that cell's rcp^3 PP cells, so one
+// wavefront per coarse cell builds the histogram in LDS (no global atomics),
+// a device scan turns counts into cell starts, and the same wavefront
+// scatters its particles to their sorted slots with ballot-ranked, order-
+// preserving offsets (a stable counting sort == the reference's stable radix
+// sort).  Sorted particles are one float4 stream (x, y, z, range) with a
+// parallel slot/cell-id stream.  The pair kernel walks the 3 contiguous
+// x-runs of each (y,z) neighbour row.  The pending PM3 Delta-v (FP64) and the
+// PP Delta-v are applied to vp in one FP64 add per component.
+#include <algorithm>
+#include <chrono>
+
+#include "common.h"
+#include "pp_kernels.h"
+#include "profile.h"
+#include "scan.h"
+
+namespace cube2 {
+
+struct PPState {
+    DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
+    std::vector<PPTile> h_tiles;
+    int64_t c_tot = 0;
+    int n_tot = 0;
+    bool valid = false;
+};
+static PPState g_pp;
+
+// ---------------------------------------------------------------- kernels
+
+// nzero[t][k][j][i] = idx_b_r[t][k][j] - sum_{q>=i} rhoc[t][k][j][q]
+// (pp_force_kernel.cu:744-750): first slot of each coarse cell.
+__global__ void k_nzero(const int* __restrict__ rhoc, const int64_t* __restrict__ idx,
+                        int64_t* __restrict__ nz, int nte, int64_t nrows) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int* row = rhoc + r * nte;
+    int64_t* out = nz + r * nte;
+    int64_t s = 0, end = idx[r];
+    for (int i = nte - 1; i >= 0; i--) {
+        s += row[i];
+        out[i] = end - s;
+    }
+}
+
+struct CellRef {
+    int t, ci, cj, ck;  // tile and extended-local coarse coords in [0, nt+2)
+    int64_t cell;       // index into rhoc / nzero
+};
+
+__device__ inline CellRef coarse_cell(int64_t w, int nt, int ncb, int nte) {
+    const int ne = nt + 2;
+    const int64_t per = (int64_t)ne * ne * ne;
+    CellRef c;
+    c.t = (int)(w / per);
+    int64_t r = w - (int64_t)c.t * per;
+    c.ci = (int)(r % ne);
+    c.cj = (int)((r / ne) % ne);
+    c.ck = (int)(r / ((int64_t)ne * ne));
+    const int i = c.ci + ncb - 1, j = c.cj + ncb - 1, k = c.ck + ncb - 1;
+    c.cell = (((int64_t)c.t * nte + k) * nte + j) * nte + i;
+    return c;
+}
+
+// PP cell coordinate of one axis: floor(rcp * x0) + rcp with
+// x0 = (float)((i - ncb) + frac) (pp_force_kernel.cu:764-803).
+__device__ inline int pp_index(float x0, int rcp) {
+    return (int)floorf(__fmul_rn((float)rcp, x0)) + rcp;
+}
+
+__device__ inline float decode_coarse(int rel, int16_t v) {
+    return (float)((double)rel + frac16(v));
+}
+
+constexpr int kWaves = 4;
+constexpr int kMaxSub = 12 * 12 * 12;
+
+// One wavefront per coarse cell of each tile's extended region: histogram
+// of its particles over its rcp^3 PP cells; writes every one of those
+// counts (the cells tile the tile's S^3 grid, so no clearing is needed).
+__global__ __launch_bounds__(256) void k_pp_count(const int* __restrict__ rhoc,
+                                                  const int64_t* __restrict__ nzero,
+                                                  const int16_t* __restrict__ xp,
+                                                  const PPTile* __restrict__ tiles, int nt,
+                                                  int ncb, int nte, int64_t ncells, int64_t n_used,
+                                                  int* __restrict__ counts,
+                                                  int* __restrict__ err) {
+    __shared__ int bins[kWaves][kMaxSub];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
+    const bool ok = w < ncells;
+    CellRef c{};
+    int rcp = 1, S = 1, nb = 0;
+    int64_t off = 0;
+    if (ok) {
+        c = coarse_cell(w, nt, ncb, nte);
+        PPTile T = tiles[c.t];
+        rcp = T.rcp;
+        S = T.S;
+        off = T.cell_off;
+        nb = rcp * rcp * rcp;
+        for (int b = lane; b < nb; b += 64) bins[wave][b] = 0;
+    }
+    __syncthreads();
+    if (ok) {
+        int np = rhoc[c.cell];
+        const int64_t base = nzero[c.cell];
+        if (np < 0 || base < 0 || base + np > n_used) {  // rhoc / idx_b_r inconsistent
+            if (lane == 0) atomicOr(err, 2);
+            np = 0;
+        }
+        const int ri = c.ci - 1, rj = c.cj - 1, rk = c.ck - 1;
+        for (int l = lane; l < np; l += 64) {
+            const int16_t* p = xp + 3 * (base + l);
+            int bx = pp_index(decode_coarse(ri, p[0]), rcp) - c.ci * rcp;
+            int by = pp_index(decode_coarse(rj, p[1]), rcp) - c.cj * rcp;
+            int bz = pp_index(decode_coarse(rk, p[2]), rcp) - c.ck * rcp;
+            if ((unsigned)bx >= (unsigned)rcp || (unsigned)by >= (unsigned)rcp ||
+                (unsigned)bz >= (unsigned)rcp) {
+                atomicOr(err, 1);
+                bx = min(max(bx, 0), rcp - 1);
+                by = min(max(by, 0), rcp - 1);
+                bz = min(max(bz, 0), rcp - 1);
+            }
+            atomicAdd(&bins[wave][(bz * rcp + by) * rcp + bx], 1);
+        }
+    }
+    __syncthreads();
+    if (ok) {
+        for (int b = lane; b < nb; b += 64) {
+            int bx = b % rcp, by = (b / rcp) % rcp, bz = b / (rcp * rcp);
+            int64_t key = ((int64_t)(c.ck * rcp + bz) * S + (c.cj * rcp + by)) * S + (c.ci * rcp + bx);
+            counts[off + key] = bins[wave][b];
+        }
+    }
+}
+
+// Same traversal; scatters each particle to cell_start[key] + (rank among
+// the earlier particles of its PP cell), preserving slot order within a
+// cell.  Ranks come from ballots over the wavefront and an LDS cursor per
+// PP cell.
+__global__ __launch_bounds__(256) void k_pp_scatter(
+    const int* __restrict__ rhoc, const int64_t* __restrict__ nzero, const int16_t* __restrict__ xp,
+    const PPTile* __restrict__ tiles, const int* __restrict__ ires_loc,
+    const float* __restrict__ apm3, int nt, int ncb, int nte, int nnt, int nns, int ntt,
+    int ratio_cs, int64_t ncells, const int* __restrict__ cell_start, float4* __restrict__ pos4,
+    uint32_t* __restrict__ slot, uint32_t* __restrict__ cellid) {
+    __shared__ int cur[kWaves][kMaxSub];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
+    const bool ok = w < ncells;
+    CellRef c{};
+    int rcp = 1, S = 1, nb = 0;
+    int64_t off = 0;
+    if (ok) {
+        c = coarse_cell(w, nt, ncb, nte);
+        PPTile T = tiles[c.t];
+        rcp = T.rcp;
+        S = T.S;
+        off = T.cell_off;
+        nb = rcp * rcp * rcp;
+        for (int b = lane; b < nb; b += 64) cur[wave][b] = 0;
+    }
+    __syncthreads();
+    if (!ok) return;  // no block-level barrier below
+    const int np = rhoc[c.cell];
+    if (np == 0) return;
+    const int64_t base = nzero[c.cell];
+    const bool phys = c.ci >= 1 && c.ci <= nt && c.cj >= 1 && c.cj <= nt && c.ck >= 1 && c.ck <= nt;
+    float range = 0.0f;
+    if (phys) {
+        const int tx = c.t % nnt, ty = (c.t / nnt) % nnt, tz = c.t / (nnt * nnt);
+        const int sx = (c.ci - 1) / ntt, sy = (c.cj - 1) / ntt, sz = (c.ck - 1) / ntt;
+        const int loc = (((tz * nnt + ty) * nnt + tx) * nns + sz) * nns * nns + sy * nns + sx;
+        range = apm3[ires_loc[loc] - 1];
+    }
+    const int ri = c.ci - 1, rj = c.cj - 1, rk = c.ck - 1;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int l0 = 0; l0 < np; l0 += 64) {
+        const int l = l0 + lane;
+        const bool valid = l < np;
+        int b = -1;
+        float x0 = 0, x1 = 0, x2 = 0;
+        int64_t key = 0;
+        if (valid) {
+            const int16_t* p = xp + 3 * (base + l);
+            x0 = decode_coarse(ri, p[0]);
+            x1 = decode_coarse(rj, p[1]);
+            x2 = decode_coarse(rk, p[2]);
+            int i0 = pp_index(x0, rcp), i1 = pp_index(x1, rcp), i2 = pp_index(x2, rcp);
+            int bx = min(max(i0 - c.ci * rcp, 0), rcp - 1);
+            int by = min(max(i1 - c.cj * rcp, 0), rcp - 1);
+            int bz = min(max(i2 - c.ck * rcp, 0), rcp - 1);
+            b = (bz * rcp + by) * rcp + bx;
+            key = ((int64_t)(c.ck * rcp + bz) * S + (c.cj * rcp + by)) * S + (c.ci * rcp + bx);
+        }
+        uint64_t todo = __ballot(valid);
+        int mypos = 0;
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int bl = __shfl(b, leader, 64);
+            const uint64_t m = __ballot(valid && b == bl);
+            int old = 0;
+            if (lane == leader) {
+                old = cur[wave][bl];
+                cur[wave][bl] = old + __popcll(m);
+            }
+            old = __shfl(old, leader, 64);
+            if (valid && b == bl) mypos = old + __popcll(m & lt);
+            todo &= ~m;
+        }
+        if (valid) {
+            const int dst = cell_start[off + key] + mypos;
+            pos4[dst] = make_float4((float)ratio_cs * x0, (float)ratio_cs * x1,
+                                    (float)ratio_cs * x2, range);
+            slot[dst] = (uint32_t)(base + l);
+            cellid[dst] = (uint32_t)(off + key);
+        }
+    }
+}
+
+// Physical particles per subtile location (pp_force_kernel.cu:691-705).
+__global__ void k_pp_ptotal(const int* __restrict__ rhoc, int nt, int ncb, int nte, int nnt,
+                            int nns, int ntt, unsigned long long* __restrict__ ploc) {
+    const int64_t per = (int64_t)nt * nt * nt;
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= per * nnt * nnt * nnt) return;
+    const int t = (int)(w / per);
+    int64_t r = w - (int64_t)t * per;
+    const int i = (int)(r % nt), j = (int)((r / nt) % nt), k = (int)(r / ((int64_t)nt * nt));
+    const int v = rhoc[(((int64_t)t * nte + k + ncb) * nte + j + ncb) * nte + i + ncb];
+    if (!v) return;
+    const int tx = t % nnt, ty = (t / nnt) % nnt, tz = t / (nnt * nnt);
+    const int loc = (((tz * nnt + ty) * nnt + tx) * nns + k / ntt) * nns * nns + (j / ntt) * nns + i / ntt;
+    atomicAdd(&ploc[loc], (unsigned long long)v);
+}
+
+// First sorted index of every tile (for per-tile counts).
+__global__ void k_tile_first(const int* __restrict__ cell_start, const PPTile* __restrict__ tiles,
+                             int ntile, int64_t c_tot, int* __restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntile) return;
+    out[t] = cell_start[t < ntile ? tiles[t].cell_off : c_tot];
+}
+
+// ------------------------------------------------------------ host side
+
+static int location_maps(const Geom& g, const int* ires, const int* ixyz3, std::vector<int>& iloc,
+                         std::vector<int>& owner) {
+    iloc.assign(g.ns3, 0);
+    owner.assign(g.ns3, -1);
+    const int nnt = g.p.nnt, nns = g.p.nns;
+    for (int it = 0; it < g.ns3; it++) {
+        const int* x = ixyz3 + 6 * it;
+        int sx = x[0] - 1, sy = x[1] - 1, sz = x[2] - 1, tx = x[3] - 1, ty = x[4] - 1, tz = x[5] - 1;
+        if (sx < 0 || sx >= nns || sy < 0 || sy >= nns || sz < 0 || sz >= nns || tx < 0 ||
+            tx >= nnt || ty < 0 || ty >= nnt || tz < 0 || tz >= nnt || ires[it] < 2 ||
+            ires[it] > 6) {
+            set_error("invalid PP subtile metadata at %d (ires=%d)", it, ires[it]);
+            return CUBE2_ERR_ARG;
+        }
+        int loc = (((tz * nnt + ty) * nnt + tx) * nns + sz) * nns * nns + sy * nns + sx;
+        iloc[loc] = ires[it];
+        owner[loc] = it;
+    }
+    return CUBE2_OK;
+}
+
+int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf) {
+    ensure_params();
+    if (!ires || !ixyz3 || !ratio_sf) {
+        set_error("null argument");
+        return CUBE2_ERR_ARG;
+    }
+    std::vector<int> iloc, owner;
+    C2_TRY(location_maps(ctx().g, ires, ixyz3, iloc, owner));
+    for (int L = 1; L < 7; L++)
+        if (ratio_sf[L] <= 0 || ratio_sf[L] > 12) {
+            set_error("ratio_sf[%d]=%d outside 1..12", L, ratio_sf[L]);
+            return CUBE2_ERR_ARG;
+        }
+    return CUBE2_OK;
+}
+
+int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+             const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
+             int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
+             double* ptotal, double* ttotal, uint64_t* pair_counts) {
+    ensure_params();
+    Context& C = ctx();
+    const Geom& g = C.g;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ires || !ixyz3 || !apm3 || !ratio_sf || !f2max || !vmax || !ptotal) {
+        set_error("null argument");
+        return CUBE2_ERR_ARG;
+    }
+    if (n_used < 0 || n_used >= ((int64_t)1 << 31)) {
+        set_error("invalid PP particle count %lld", (long long)n_used);
+        return CUBE2_ERR_CAPACITY;
+    }
+    std::vector<int> iloc, owner;
+    C2_TRY(location_maps(g, ires, ixyz3, iloc, owner));
+    for (int L = 1; L < 7; L++)
+        if (ratio_sf[L] <= 0 || ratio_sf[L] > 12) {
+            set_error("ratio_sf[%d]=%d outside 1..12", L, ratio_sf[L]);
+            return CUBE2_ERR_ARG;
+        }
+    if (C.dv_state == 1 && C.dv_n_used != n_used) {
+        set_error("PM3/PP particle count changed within a kick");
+        return CUBE2_ERR_STATE;
+    }
+    C2_TRY(ensure_init());
+    hipStream_t s = C.stream;
+    // per-tile grid (pp_force_kernel.cu:641-661)
+    const int ntile = g.ntile, nsub3 = g.p.nns * g.p.nns * g.p.nns;
+    std::vector<PPTile>& T = g_pp.h_tiles;
+    T.assign(ntile, PPTile{});
+    int64_t off = 0;
+    for (int t = 0; t < ntile; t++) {
+        int rcp = ratio_sf[1];
+        for (int q = 0; q < nsub3; q++) rcp = std::max(rcp, ratio_sf[iloc[t * nsub3 + q] - 1]);
+        T[t].rcp = rcp;
+        T[t].S = g.nt * rcp + 2 * rcp;
+        T[t].cell_off = off;
+        off += (int64_t)T[t].S * T[t].S * T[t].S;
+    }
+    const int64_t c_tot = off;
+    if (c_tot >= ((int64_t)1 << 32) - 1) {
+        set_error("PP cell grid too large (%lld cells)", (long long)c_tot);
+        return CUBE2_ERR_CAPACITY;
+    }
+    const int64_t ncell_rows = (int64_t)ntile * g.nte * g.nte;
+    const int64_t ncells_all = ncell_rows * g.nte;
+    const int64_t ncells_ext = (int64_t)ntile * (g.nt + 2) * (g.nt + 2) * (g.nt + 2);
+    const size_t ncap = (size_t)std::max<int64_t>(n_used, 1);
+    C2_TRY(g_pp.nzero.reserve(ncells_all * sizeof(int64_t)));
+    C2_TRY(g_pp.tiles.reserve(ntile * sizeof(PPTile)));
+    C2_TRY(g_pp.ires_loc.reserve(g.ns3 * sizeof(int)));
+    C2_TRY(g_pp.apm3.reserve(8 * sizeof(float)));
+    C2_TRY(g_pp.counts.reserve((c_tot + 1) * sizeof(int)));
+    C2_TRY(g_pp.pos4.reserve(ncap * sizeof(float4)));
+    C2_TRY(g_pp.slot.reserve(ncap * sizeof(uint32_t)));
+    C2_TRY(g_pp.cellid.reserve(ncap * sizeof(uint32_t)));
+    C2_TRY(g_pp.red.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long)));
+    C2_TRY(g_pp.tile_first.reserve((ntile + 1) * sizeof(int)));
+    C2_HIP(hipMemcpyAsync(g_pp.tiles.ptr, T.data(), ntile * sizeof(PPTile), hipMemcpyHostToDevice, s));
+    C2_HIP(hipMemcpyAsync(g_pp.ires_loc.ptr, iloc.data(), g.ns3 * sizeof(int), hipMemcpyHostToDevice, s));
+    C2_HIP(hipMemcpyAsync(g_pp.apm3.ptr, apm3, 7 * sizeof(float), hipMemcpyHostToDevice, s));
+    PPRed* red = g_pp.red.as<PPRed>();
+    unsigned long long* ploc = reinterpret_cast<unsigned long long*>(red + 1);
+    C2_HIP(hipMemsetAsync(red, 0, sizeof(PPRed) + g.ns3 * sizeof(unsigned long long), s));
+    g_pp.valid = false;
+    {
+        Stage st("pp_nzero", (double)ncells_all * 12 + ncell_rows * 8);
+        k_nzero<<<ceil_div(ncell_rows, 256), 256, 0, s>>>(d_rhoc, d_idx, g_pp.nzero.as<int64_t>(),
+                                                         g.nte, ncell_rows);
+        C2_HIP(hipGetLastError());
+    }
+    if (ncells_ext > 0) {
+        Stage st("pp_count", (double)n_used * 6 + c_tot * 4.0);
+        k_pp_count<<<ceil_div(ncells_ext, kWaves), 256, 0, s>>>(
+            d_rhoc, g_pp.nzero.as<int64_t>(), d_xp, g_pp.tiles.as<PPTile>(), g.nt, g.ncb, g.nte,
+            ncells_ext, n_used, g_pp.counts.as<int>(), &red->err);
+        C2_HIP(hipGetLastError());
+    }
+    {
+        Stage st("pp_scan", (double)c_tot * 8);
+        C2_TRY(exclusive_scan(g_pp.counts.as<int>(), c_tot, g_pp.scan_tmp, s));
+    }
+    int n_tot = 0, err = 0;
+    C2_HIP(hipMemcpyAsync(&n_tot, g_pp.counts.as<int>() + c_tot, sizeof(int), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(&err, &red->err, sizeof(int), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipStreamSynchronize(s));
+    if (err & 2) {
+        set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)n_used);
+        return CUBE2_ERR_ARG;
+    }
+    if (err) {
+        set_error("a particle's PP key fell outside its coarse cell (geometry too large for FP32 keys)");
+        return CUBE2_ERR_CAPACITY;
+    }
+    if (n_tot < 0 || n_tot > n_used) {
+        set_error("cell lists hold %d particles, more than n_used=%lld", n_tot, (long long)n_used);
+        return CUBE2_ERR_ARG;
+    }
+    if (ncells_ext > 0) {
+        Stage st("pp_scatter", (double)n_tot * (6 + 24));
+        k_pp_scatter<<<ceil_div(ncells_ext, kWaves), 256, 0, s>>>(
+            d_rhoc, g_pp.nzero.as<int64_t>(), d_xp, g_pp.tiles.as<PPTile>(), g_pp.ires_loc.as<int>(),
+            g_pp.apm3.as<float>(), g.nt, g.ncb, g.nte, g.p.nnt, g.p.nns, g.ntt, g.p.ratio_cs,
+            ncells_ext, g_pp.counts.as<int>(), g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>(),
+            g_pp.cellid.as<uint32_t>());
+        C2_HIP(hipGetLastError());
+    }
+    {
+        const int64_t nphys = (int64_t)ntile * g.nt * g.nt * g.nt;
+        if (nphys > 0)
+            k_pp_ptotal<<<ceil_div(nphys, 256), 256, 0, s>>>(d_rhoc, g.nt, g.ncb, g.nte, g.p.nnt,
+                                                              g.p.nns, g.ntt, ploc);
+        C2_HIP(hipGetLastError());
+        k_tile_first<<<ceil_div(ntile + 1, 64), 64, 0, s>>>(g_pp.counts.as<int>(), g_pp.tiles.as<PPTile>(),
+                                                            ntile, c_tot, g_pp.tile_first.as<int>());
+        C2_HIP(hipGetLastError());
+    }
+    const double* dv = (C.dv_state == 1) ? C.dv64.as<double>() : nullptr;
+    PPArgs A;
+    A.pos4 = g_pp.pos4.as<float4>();
+    A.slot = g_pp.slot.as<uint32_t>();
+    A.cellid = g_pp.cellid.as<uint32_t>();
+    A.cell_start = g_pp.counts.as<int>();
+    A.tiles = g_pp.tiles.as<PPTile>();
+    A.ntile = ntile;
+    A.n = n_tot;
+    A.ratio_cs = g.p.ratio_cs;
+    A.app = (float)g.p.app;
+    A.mass = mass;
+    A.a_mid = a_mid;
+    A.dt = dt;
+    A.dv64 = dv;
+    A.vp = d_vp;
+    A.red = red;
+    if (n_tot > 0) C2_TRY(launch_pp_pairs(A, C.pp_mode, s));
+    // results
+    PPRed h;
+    std::vector<unsigned long long> hp(g.ns3);
+    std::vector<int> tf(ntile + 1);
+    C2_HIP(hipMemcpyAsync(&h, red, sizeof(PPRed), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(hp.data(), ploc, g.ns3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(tf.data(), g_pp.tile_first.ptr, (ntile + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    C2_TRY(sync_and_flush());
+    *f2max = __builtin_bit_cast(float, h.f2max_bits);
+    for (int d = 0; d < 3; d++) vmax[d] = __builtin_bit_cast(float, h.vmax_bits[d]);
+    for (int it = 0; it < g.ns3; it++) ptotal[it] = 0.0;
+    for (int loc = 0; loc < g.ns3; loc++) ptotal[owner[loc]] += (double)hp[loc];
+    if (pair_counts) {
+        pair_counts[0] = h.p_cand;
+        pair_counts[1] = h.p_acc;
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (ttotal) {
+        for (int it = 0; it < g.ns3; it++) ttotal[it] = 0.0;
+        for (int t = 0; t < ntile; t++) {
+            double share = n_tot > 0 ? (double)(tf[t + 1] - tf[t]) / n_tot : 0.0;
+            ttotal[owner[t * nsub3]] = wall * share;
+        }
+    }
+    g_pp.c_tot = c_tot;
+    g_pp.n_tot = n_tot;
+    g_pp.valid = true;
+    C.dv_state = 0;
+    return CUBE2_OK;
+}
+
+int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t* slot,
+                      int32_t* cell_start) {
+    if (!g_pp.valid || it < 0 || it >= (int)g_pp.h_tiles.size()) {
+        set_error("no PP cell list for tile %d", it);
+        return CUBE2_ERR_STATE;
+    }
+    const PPTile& T = g_pp.h_tiles[it];
+    const int64_t ncell = (int64_t)T.S * T.S * T.S;
+    hipStream_t s = ctx().stream;
+    int a0 = 0, a1 = 0;
+    C2_HIP(hipMemcpyAsync(&a0, g_pp.counts.as<int>() + T.cell_off, sizeof(int), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(&a1, g_pp.counts.as<int>() + T.cell_off + ncell, sizeof(int), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipStreamSynchronize(s));
+    if (rcp) *rcp = T.rcp;
+    if (S) *S = T.S;
+    if (N) *N = a1 - a0;
+    const int n = a1 - a0;
+    if (keys && n > 0) {
+        std::vector<uint32_t> cid(n);
+        C2_HIP(hipMemcpy(cid.data(), g_pp.cellid.as<uint32_t>() + a0, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (int q = 0; q < n; q++) keys[q] = (uint32_t)(cid[q] - T.cell_off);
+    }
+    if (slot && n > 0)
+        C2_HIP(hipMemcpy(slot, g_pp.slot.as<uint32_t>() + a0, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (cell_start) {
+        C2_HIP(hipMemcpy(cell_start, g_pp.counts.as<int>() + T.cell_off, (ncell + 1) * sizeof(int),
+                         hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q <= ncell; q++) cell_start[q] -= a0;
+    }
+    return CUBE2_OK;
+}
+
+void pp_release() {
+    PPState& p = g_pp;
+    for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
+                      &p.slot, &p.cellid, &p.red, &p.tile_first})
+        b->release();
+    p.valid = false;
+}
+
+}  // namespace cube2

@@ -1,0 +1,36 @@
+// PP device structures shared by the cell-list build and the pair kernels.
+#pragma once
+#include "common.h"
+
+namespace cube2 {
+
+struct PPTile {
+    int rcp;           // PP cells per coarse cell (max level of the tile's subtiles)
+    int S;             // cells per dim of the tile grid, nt*rcp + 2*rcp
+    long long cell_off;  // first cell of the tile in the concatenated grids
+};
+
+struct PPRed {
+    unsigned long long p_cand, p_acc;  // examined / accepted ordered pairs
+    unsigned f2max_bits;               // max |a|^2 (non-negative float bits)
+    unsigned vmax_bits[3];             // max |vp| per component
+    int err;
+    int pad;
+};
+
+struct PPArgs {
+    const float4* pos4;        // sorted (x, y, z, range); range 0 = halo source
+    const uint32_t* slot;      // sorted -> xp/vp slot
+    const uint32_t* cellid;    // sorted -> global cell (tile offset + key)
+    const int* cell_start;     // c_tot + 1 exclusive starts
+    const PPTile* tiles;
+    int ntile, n, ratio_cs;
+    float app, mass, a_mid, dt;
+    const double* dv64;        // pending PM3 Delta-v (FP64) or nullptr
+    float* vp;
+    PPRed* red;
+};
+
+int launch_pp_pairs(const PPArgs& a, int mode, hipStream_t s);
+
+}  // namespace cube2

@@ -1,0 +1,161 @@
+"""Host-side mirror of the kick driver's short-range part (kick.f90:107-236).
+
+``ShortRangeKick`` holds one image's layout on the device and runs
+level selection -> PM3 -> PP through the library's C ABI, the way
+kick.f90 calls c_pm3_force_kernel then c_pp_force_kernel, and derives the
+time-step limits dt_pm3, dt_pp (kick.f90:180,227) and dt_vmax
+(kick.f90:238) in FP64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .geometry import APM3, APM3F, RATIO_SF, Geometry, ixyz3
+from .layout import Layout
+
+G_GRID = 1.0 / 6.0 / math.pi   # variables.f90:11
+VBUF = 0.9                     # parameters.f90:83
+
+
+def _np_ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def select_levels(geom: Geometry, rhoc: np.ndarray, ix: np.ndarray | None = None):
+    """ires, isort (1-based), napm of kick.f90:107-131."""
+    _lib.set_params(geom)
+    ix = ixyz3(geom) if ix is None else np.ascontiguousarray(ix, np.int32)
+    ires = np.zeros(geom.ns3, np.int32)
+    isort = np.zeros(geom.ns3, np.int32)
+    napm = np.zeros(7, np.int32)
+    rhoc = np.ascontiguousarray(rhoc, np.int32)
+    _lib.check(_lib.lib().cube2_select_levels(_np_ptr(rhoc), _np_ptr(ix), _np_ptr(ires),
+                                               _np_ptr(isort), _np_ptr(napm)))
+    return ires, isort, napm
+
+
+@dataclass
+class KickResult:
+    f2max_pm3: float = 0.0
+    f2max_pp: float = 0.0
+    vmax: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+    ptotal_pm3: np.ndarray | None = None
+    ptotal_pp: np.ndarray | None = None
+    ttotal_pp: np.ndarray | None = None
+    p_cand: int = 0
+    p_acc: int = 0
+
+    def dt_limits(self, a_mid: float) -> dict:
+        """dt_pm3, dt_pp, dt_vmax in FP64 (kick.f90:180,227,238)."""
+        def lim(f2):
+            return 0.5 * math.sqrt(1.0 / (math.sqrt(f2) * a_mid * G_GRID)) if f2 > 0 else math.inf
+        vm = float(np.max(self.vmax))
+        return dict(dt_pm3=lim(float(self.f2max_pm3)), dt_pp=lim(float(self.f2max_pp)),
+                    dt_vmax=VBUF * 14 / vm if vm > 0 else math.inf)
+
+
+class ShortRangeKick:
+    """One image resident in HBM; each ``kick()`` runs PM3 then PP."""
+
+    def __init__(self, layout: Layout, device: str = "cuda:0", greens: dict | None = None):
+        import torch
+        self.torch = torch
+        self.geom = layout.geom
+        self.layout = layout
+        self.dev = torch.device(device)
+        _lib.set_params(self.geom)
+        self.d_rhoc = torch.from_numpy(np.ascontiguousarray(layout.rhoc)).to(self.dev)
+        self.d_idx = torch.from_numpy(np.ascontiguousarray(layout.idx_b_r)).to(self.dev)
+        self.d_xp = torch.from_numpy(np.ascontiguousarray(layout.xp)).to(self.dev)
+        self.d_vp = torch.from_numpy(np.ascontiguousarray(layout.vp)).to(self.dev)
+        self.n_used = layout.n_used
+        self.ix = ixyz3(self.geom)
+        self.ires, self.isort, self.napm = select_levels(self.geom, layout.rhoc, self.ix)
+        self.greens = {} if greens is None else dict(greens)
+        torch.cuda.synchronize()
+
+    def set_levels(self, ires: np.ndarray) -> None:
+        self.ires = np.ascontiguousarray(ires, np.int32)
+        self.isort = np.argsort(-self.ires, kind="stable").astype(np.int32) + 1
+
+    def green(self, level: int):
+        """Device Gk3 table of a level, built once with the FP64 Green kernel."""
+        if level not in self.greens:
+            g = self.geom
+            n = g.nft(level)
+            t = self.torch.empty((n, n, n // 2 + 1), dtype=self.torch.float32, device=self.dev)
+            _lib.check(_lib.lib().cube2_green_3d_dev(
+                t.data_ptr(), n, float(APM3F), float(APM3F * RATIO_SF[level - 1]),
+                float(np.float32(1.0) / np.float32(RATIO_SF[level - 1]))))
+            self.greens[level] = t
+        return self.greens[level]
+
+    def pm3(self, mass: float, a_mid: float, dt: float, res: KickResult) -> None:
+        L = _lib.lib()
+        _lib.set_params(self.geom)
+        used = sorted(set(int(x) for x in self.ires))
+        arr = (C.c_void_p * 5)()
+        for lev in used:
+            arr[lev - 2] = self.green(lev).data_ptr()
+        f2 = np.zeros(1, np.float32)
+        pt = np.zeros(self.geom.ns3, np.float64)
+        _lib.check(L.cube2_pm3_force_dev(
+            _np_ptr(self.isort), _np_ptr(self.ires), _np_ptr(self.ix), _np_ptr(RATIO_SF),
+            self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_xp.data_ptr(), self.n_used,
+            arr, float(mass), float(a_mid), float(dt), _np_ptr(f2), _np_ptr(pt)))
+        res.f2max_pm3 = float(f2[0])
+        res.ptotal_pm3 = pt
+
+    def pp(self, mass: float, a_mid: float, dt: float, res: KickResult) -> None:
+        L = _lib.lib()
+        _lib.set_params(self.geom)
+        f2 = np.zeros(1, np.float32)
+        vm = np.zeros(3, np.float32)
+        pt = np.zeros(self.geom.ns3, np.float64)
+        tt = np.zeros(self.geom.ns3, np.float64)
+        cnt = np.zeros(2, np.uint64)
+        _lib.check(L.cube2_pp_force_dev(
+            _np_ptr(self.ires), _np_ptr(self.ix), _np_ptr(APM3), _np_ptr(RATIO_SF),
+            self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_xp.data_ptr(),
+            self.d_vp.data_ptr(), self.n_used, float(mass), float(a_mid), float(dt),
+            _np_ptr(f2), _np_ptr(vm), _np_ptr(pt), _np_ptr(tt), _np_ptr(cnt)))
+        res.f2max_pp = float(f2[0])
+        res.vmax = vm
+        res.ptotal_pp = pt
+        res.ttotal_pp = tt
+        res.p_cand, res.p_acc = int(cnt[0]), int(cnt[1])
+
+    def kick(self, mass: float, a_mid: float, dt: float, pm3: bool = True, pp: bool = True) -> KickResult:
+        res = KickResult()
+        if pm3:
+            self.pm3(mass, a_mid, dt, res)
+        if pp:
+            self.pp(mass, a_mid, dt, res)
+        elif pm3:
+            vm = np.zeros(3, np.float32)
+            _lib.check(_lib.lib().cube2_velocity_apply_dev(
+                self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_vp.data_ptr(),
+                self.n_used, _np_ptr(vm)))
+            res.vmax = vm
+        return res
+
+    def vp_host(self) -> np.ndarray:
+        return self.d_vp.cpu().numpy()
+
+    def celllist(self, it: int) -> dict:
+        L = _lib.lib()
+        rcp, S, N = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(L.cube2_pp_debug_celllist(it, C.addressof(rcp), C.addressof(S), C.addressof(N),
+                                             None, None, None))
+        n, s = N.value, S.value
+        keys = np.zeros(max(n, 1), np.uint32)
+        slot = np.zeros(max(n, 1), np.uint32)
+        start = np.zeros(s ** 3 + 1, np.int32)
+        _lib.check(L.cube2_pp_debug_celllist(it, None, None, None, _np_ptr(keys), _np_ptr(slot),
+                                             _np_ptr(start)))
+        return dict(rcp=rcp.value, S=s, N=n, keys=keys[:n], slot=slot[:n], cell_start=start)

@@ -1,0 +1,164 @@
+/*
+ * libcube2_short -- MI355X (gfx950) short-range force path of CUBE2:
+ * fine-mesh PM3 and direct-summation PP of the kick (kick.f90:107-236).
+ *
+ * C ABI.  Plain pointers and sizes only.  The first four entry points keep
+ * the names, argument order and meaning of the reference's bind(C)
+ * interfaces so the Fortran host links against this library unchanged.
+ * Errors in those four print to stderr and terminate the process, as the
+ * reference does (pp_force_kernel.cu:68-76 exit(EXIT_FAILURE);
+ * pm3_fft_cuda.cu:16-31 abort()).  Every cube2_* entry point instead returns
+ * a status code (CUBE2_OK == 0) and leaves a message in cube2_last_error().
+ *
+ * Array layouts (Fortran column-major seen from C):
+ *   rhoc     int32  [nnt^3][nte][nte][nte]   (rhoc(i,j,k,tx,ty,tz))
+ *   idx_b_r  int64  [nnt^3][nte][nte]        (idx_b_r(j,k,tx,ty,tz))
+ *   xp       int16  [N_used][3]              N_used = idx_b_r[last]
+ *   vp       float  [N_used][3]
+ *   ixyz3    int32  [ns3][6] = {sx,sy,sz,tx,ty,tz}, 1-based
+ *   ires     int32  [ns3] in 2..6;  isort int32 [ns3], 1-based subtile ids
+ *   Gk3_L    float  [n][n][n/2+1] (Gk3(n/2+1,n,n)), n = nfp+2*nfb at level L
+ * with nte = nt + 2*ncb, nt = ng/ratio_cs/nnt, ncb = ngb/ratio_cs, ns3 = (nnt*nns)^3.
+ */
+#ifndef CUBE2_SHORT_H
+#define CUBE2_SHORT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CUBE2_OK = 0,
+    CUBE2_ERR_ARG = 1,      /* invalid argument or metadata */
+    CUBE2_ERR_GEOMETRY = 2, /* geometry not set or inconsistent */
+    CUBE2_ERR_CAPACITY = 3, /* size beyond a supported limit */
+    CUBE2_ERR_HIP = 4,      /* HIP runtime failure */
+    CUBE2_ERR_STATE = 5     /* call order (velocity accumulator) violated */
+};
+
+enum { CUBE2_PP_EXACT = 0, CUBE2_PP_FAST = 1 };
+
+/* ---------------------------------------------------------------------
+ * Drop-in entry points (host pointers; the Fortran caller owns every array)
+ * ------------------------------------------------------------------- */
+
+/* Replaces pp_force_kernel.cu:960-963 (interface kick.f90:13-30, call site
+ * kick.f90:202-205).  PP over every parent tile; applies the pending PM3
+ * Delta-v together with the PP Delta-v to vp; f2max = max |a*m^2|^2,
+ * vmax[d] = max |vp[d]| over physical particles; ptotal[it] = physical
+ * particles of subtile it; ttotal[first subtile of each parent tile] =
+ * that tile's share of the PP wall time.  isort is ignored, as in the
+ * reference (pp_force_kernel.cu:1049). */
+void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* ratio_sf, int* rhoc,
+                       int64_t* idx_b_r, int16_t* xp, float* vp, float mass_p_cdm, float a_mid,
+                       float dt, float* f2max, float* vmax, double* ptotal, double* ttotal);
+
+/* Replaces pm3_fft_cuda.cu:516-520 (interface kick.f90:31-45, call site
+ * kick.f90:144-146).  PM3 over every subtile; leaves the Delta-v pending
+ * for the next c_pp_force_kernel (vp is not touched); f2max = max over
+ * subtile interiors of |F|^2; vmax = 0; ptotal[it] = physical particles of
+ * subtile it (pm3_fft_cuda.cu:436-437).  g2..g12 are the Gk3 tables of
+ * levels 2..6, cached on the device by (pointer, size) as the reference
+ * does (pm3_fft_cuda.cu:153-159): their contents must not change at a
+ * given address. */
+void c_pm3_force_kernel(int* isort, int* ires, int* ixyz3, int* ratio_sf, int* rhoc,
+                        int64_t* idx_b_r, int16_t* xp, float* g2, float* g4, float* g6, float* g8,
+                        float* g12, float* vp, float mass, float amid, float dt, float* f2max,
+                        float* vmax, double* ptotal);
+
+/* Replaces pm3_fft_cuda.cu:621-628 (finalize.f90:19-20): frees PM3 device
+ * workspaces and the cached Green's tables. */
+void c_pm3_fft_finalize(void);
+
+/* Replaces pp_force_kernel.cu:539-547 (finalize.f90:19): frees PP device
+ * workspaces and the velocity accumulator. */
+void c_gpu_velocity_finalize(void);
+
+/* ---------------------------------------------------------------------
+ * Runtime geometry.  Replaces the compile-time cuda_parameters_generated.h
+ * (its eleven names, tools/gen_cuda_parameters.py:9-21).  Until it is
+ * called the library uses the reference's default values.
+ * ------------------------------------------------------------------- */
+typedef struct {
+    int ng, nnt, nns, ratio_cs, np_nc, ngb, ncore, body_centered_cubic;
+    double image_buffer, tile_buffer, app;
+} cube2_params_t;
+
+int cube2_set_params(const cube2_params_t* p);
+int cube2_get_params(cube2_params_t* p);
+
+/* PP arithmetic: CUBE2_PP_EXACT evaluates every pair with the reference C
+ * kernel's operation order and IEEE sqrt/divide (bit-identical Delta-v);
+ * CUBE2_PP_FAST (default) uses the fast kernel (relative RMS <= 1e-5). */
+int cube2_set_pp_mode(int mode);
+
+/* ---------------------------------------------------------------------
+ * Device-resident variants (no counterpart in the reference; they are the
+ * same path with particle and cell arrays already in HBM).  rhoc, idx_b_r,
+ * xp, vp are device pointers; isort/ires/ixyz3/apm3/ratio_sf and outputs
+ * f2max/vmax/ptotal/ttotal are host pointers.  n_used = idx_b_r[last].
+ * Work runs on the library stream; calls return after it completes.
+ * ------------------------------------------------------------------- */
+int cube2_pm3_force_dev(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+                        const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                        int64_t n_used, const float* const* d_green /* [5]: levels 2..6 */,
+                        float mass, float amid, float dt, float* f2max, double* ptotal);
+
+int cube2_pp_force_dev(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+                       const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                       float* d_vp, int64_t n_used, float mass, float a_mid, float dt,
+                       float* f2max, float* vmax, double* ptotal, double* ttotal,
+                       uint64_t* pair_counts /* [2] P_cand, P_acc; may be NULL */);
+
+/* Applies a pending PM3 Delta-v without PP (the reference leaves it
+ * pending until PP, pp_force_kernel.cu:496-498); vmax over physical slots. */
+int cube2_velocity_apply_dev(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp,
+                             int64_t n_used, float* vmax);
+
+/* Green_3D (Green/Green.f90:1-80) in FP64 on the device, stored FP32:
+ * d_G[n][n][n/2+1] for apm_soft, apm_range, gridsize as in
+ * initialize.f90:142-168. */
+int cube2_green_3d_dev(float* d_G, int n, float apm_soft, float apm_range, float gridsize);
+
+/* Level selection and job order of kick.f90:107-131 (host arrays). */
+int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort, int* napm);
+
+/* ---------------------------------------------------------------------
+ * Inspection (tests and benchmarks)
+ * ------------------------------------------------------------------- */
+/* Cell list of parent tile `it` from the last PP call: sizes first
+ * (pointers may be NULL), then copies.  keys_sorted[N] (key in the tile's
+ * S^3 grid), slot[N] (xp index), cell_start[S^3+1] (tile-relative). */
+int cube2_pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys_sorted,
+                            uint32_t* slot, int32_t* cell_start);
+
+/* One PM3 subtile job's stage outputs from a fresh computation:
+ * rho[n^3] (density incl. mass), phi[n^3] (potential / n^3),
+ * force[3][(nfp+2)^3]; host pointers, any may be NULL. */
+int cube2_pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf,
+                        const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                        int64_t n_used, const float* d_green, float mass, float* rho, float* phi,
+                        float* force, float* f2max);
+
+/* Batched 3-D FFT of nb real cubes of size n (in place, padded x rows of
+ * n+2 floats, FFTW/cuFFT in-place layout): dir=-1 R2C, +1 C2R, unnormalised. */
+int cube2_fft3d_dev(float* d_data, int n, int nb, int dir);
+
+int cube2_profile_enable(int on);
+int cube2_profile_reset(void);
+/* Accumulated GPU time (ms), launch count and algorithmic bytes / flops of
+ * a stage; returns CUBE2_ERR_ARG for an unknown stage. */
+int cube2_profile_get(const char* stage, double* ms, int64_t* launches, double* bytes,
+                      double* flops);
+/* Names of the recorded stages, comma separated, into buf. */
+int cube2_profile_stages(char* buf, int len);
+
+int cube2_synchronize(void);
+const char* cube2_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+"""PP on the GPU through the C ABI, against the oracle and the reference's
+golden vectors.  Integer work (keys, sorted order, cell starts, pair
+counts) is compared bit for bit; the exact mode's Delta-v too; the fast
+mode within a relative RMS of 1e-5."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_pp_case
+
+pytestmark = pytest.mark.gpu
+
+FAST_RTOL = 1e-5   # relative RMS of Delta-v, fast vs exact arithmetic
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+    assert torch.cuda.is_available()
+    from cube2_amd import _lib
+    _lib.lib()
+    yield _lib
+    _lib.set_pp_mode(_lib.PP_FAST)
+
+
+def _dropin_pp(lib, geom, lay_like, ires, mass, amid, dt):
+    from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
+    lib.set_params(geom)
+    ix = ixyz3(geom)
+    isort = np.arange(1, geom.ns3 + 1, dtype=np.int32)
+    vp = np.ascontiguousarray(lay_like["vp_in"]).copy()
+    f2 = np.zeros(1, np.float32)
+    vm = np.zeros(3, np.float32)
+    pt = np.zeros(geom.ns3)
+    tt = np.zeros(geom.ns3)
+    a = lambda x: np.ascontiguousarray(x).ctypes.data  # noqa: E731
+    rhoc = np.ascontiguousarray(lay_like["rhoc"], np.int32)
+    idx = np.ascontiguousarray(lay_like["idx_b_r"], np.int64)
+    xp = np.ascontiguousarray(lay_like["xp"], np.int16)
+    lib.lib().c_pp_force_kernel(a(isort), a(ires), a(ix), a(APM3), a(RATIO_SF), a(rhoc), a(idx),
+                                a(xp), vp.ctypes.data, mass, amid, dt, f2.ctypes.data,
+                                vm.ctypes.data, pt.ctypes.data, tt.ctypes.data)
+    return vp, float(f2[0]), vm, pt
+
+
+@pytest.mark.parametrize("name", ["pp_case1", "pp_case2"])
+def test_dropin_pp_bit_exact_vs_reference_golden(lib, name):
+    from cube2_amd.geometry import REFERENCE_DEFAULT as G
+    c = load_pp_case(name)
+    lib.set_pp_mode(lib.PP_EXACT)
+    ires = np.full(G.ns3, 2, np.int32)
+    vp, f2, vm, pt = _dropin_pp(lib, G, c, ires, c["mass"], c["a_mid"], c["dt"])
+    assert np.array_equal(vp.view(np.uint32), c["vp_out"].view(np.uint32))
+    assert np.float32(f2) == np.float32(c["f2max"])
+    assert np.array_equal(vm, c["vmax"])
+    per_tile = pt.reshape(G.nnt, G.nnt, G.nnt, -1).sum(-1)
+    assert np.array_equal(per_tile, c["ptotal_tile"].reshape(G.nnt, G.nnt, G.nnt).transpose(2, 1, 0))
+
+
+def _dev_pp(lib, lay, ires=None, mode=None):
+    from cube2_amd.kick import ShortRangeKick
+    if mode is not None:
+        lib.set_pp_mode(mode)
+    k = ShortRangeKick(lay)
+    if ires is not None:
+        k.set_levels(ires)
+    res = k.kick(1.0, 0.02, 1e-3, pm3=False, pp=True)
+    return k, res
+
+
+def _oracle_pp(oracle_lib, lay, ires):
+    from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
+    g = lay.geom
+    return oracle_lib.pp_force(g, ires, ixyz3(g), APM3, RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
+                               lay.vp, 1.0, 0.02, 1e-3)
+
+
+@pytest.fixture(scope="module")
+def c0_layout():
+    from cube2_amd.geometry import C0
+    from cube2_amd.layout import build_layout
+    from cube2_amd.synth import config0
+    pos, vel = config0(seed=0)
+    return build_layout(pos, vel, C0)
+
+
+@pytest.fixture(scope="module")
+def mixed_layout():
+    """4 parent tiles x 8 subtiles, clustered, levels 2..6 mixed per tile."""
+    from cube2_amd.geometry import Geometry
+    from cube2_amd.layout import build_layout
+    g = Geometry(ng=64, nnt=2, nns=2, ratio_cs=4, ngb=8, app=0.05)
+    rng = np.random.default_rng(17)
+    pos = np.concatenate([rng.uniform(0, 64, (30000, 3)),
+                          np.array([31.5, 30.0, 33.0]) + rng.normal(0, 0.6, (6000, 3)),
+                          np.array([10.0, 50.0, 20.0]) + rng.normal(0, 1.5, (4000, 3))]) % 64
+    lay = build_layout(pos, rng.normal(0, 1, pos.shape).astype(np.float32), g)
+    ires = rng.integers(2, 7, g.ns3).astype(np.int32)
+    return lay, ires
+
+
+def test_c0_exact_bit_identical_to_oracle(lib, oracle_lib, c0_layout):
+    lay = c0_layout
+    k, res = _dev_pp(lib, lay, mode=lib.PP_EXACT)
+    o = _oracle_pp(oracle_lib, lay, k.ires)
+    vp = k.vp_host()
+    assert np.array_equal(vp.view(np.uint32), o["vp"].view(np.uint32))
+    assert (res.p_cand, res.p_acc) == (o["p_cand"], o["p_acc"])
+    assert np.float32(res.f2max_pp) == np.float32(o["f2max"])
+    assert np.array_equal(res.vmax, o["vmax"])
+    assert np.array_equal(res.ptotal_pp, o["ptotal"])
+
+
+def test_c0_cell_list_bit_identical(lib, oracle_lib, c0_layout):
+    from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
+    lay = c0_layout
+    k, _ = _dev_pp(lib, lay, mode=lib.PP_EXACT)
+    got = k.celllist(0)
+    want = oracle_lib.pp_tile_celllist(lay.geom, k.ires, ixyz3(lay.geom), APM3, RATIO_SF,
+                                       lay.rhoc, lay.idx_b_r, lay.xp, 0)
+    assert (got["rcp"], got["S"], got["N"]) == (want["rcp"], want["S"], want["N"])
+    assert np.array_equal(got["keys"], want["keys"])
+    assert np.array_equal(got["slot"], want["slot"])
+    assert np.array_equal(got["cell_start"], want["cell_start"])
+
+
+def test_mixed_levels_exact_and_cell_lists(lib, oracle_lib, mixed_layout):
+    from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
+    lay, ires = mixed_layout
+    k, res = _dev_pp(lib, lay, ires=ires, mode=lib.PP_EXACT)
+    o = _oracle_pp(oracle_lib, lay, ires)
+    vp = k.vp_host()
+    assert np.array_equal(vp.view(np.uint32), o["vp"].view(np.uint32))
+    assert (res.p_cand, res.p_acc) == (o["p_cand"], o["p_acc"])
+    assert np.array_equal(res.ptotal_pp, o["ptotal"])
+    for it in range(lay.geom.ntile):
+        got = k.celllist(it)
+        want = oracle_lib.pp_tile_celllist(lay.geom, ires, ixyz3(lay.geom), APM3, RATIO_SF,
+                                           lay.rhoc, lay.idx_b_r, lay.xp, it)
+        assert (got["rcp"], got["S"], got["N"]) == (want["rcp"], want["S"], want["N"]), it
+        assert np.array_equal(got["keys"], want["keys"]), it
+        assert np.array_equal(got["slot"], want["slot"]), it
+        assert np.array_equal(got["cell_start"], want["cell_start"]), it
+
+
+def _rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-300)))
+
+
+@pytest.mark.parametrize("case", ["c0", "mixed"])
+def test_fast_mode_within_tolerance(lib, oracle_lib, c0_layout, mixed_layout, case):
+    lay, ires = (c0_layout, None) if case == "c0" else mixed_layout
+    k, res = _dev_pp(lib, lay, ires=ires, mode=lib.PP_FAST)
+    o = _oracle_pp(oracle_lib, lay, k.ires)
+    phys = lay.physical_mask()
+    dv_gpu = (k.vp_host()[phys].astype(np.float64) - lay.vp[phys])
+    dv_ref = (o["vp"][phys].astype(np.float64) - lay.vp[phys])
+    err = _rel_rms(dv_gpu, dv_ref)
+    print(f"{case}: fast-mode relative RMS of Delta-v vs exact = {err:.3e}")
+    assert err <= FAST_RTOL
+    assert abs(res.f2max_pp - o["f2max"]) <= 1e-5 * o["f2max"]
+    assert res.p_cand == o["p_cand"]
+    assert abs(res.p_acc - o["p_acc"]) <= max(2, 1e-5 * o["p_acc"])
+
+
+def test_empty_and_single_particle(lib):
+    from cube2_amd.geometry import Geometry
+    from cube2_amd.layout import build_layout
+    g = Geometry(ng=32, nnt=2, nns=1, ratio_cs=4, ngb=8, app=0.05)
+    for n in (0, 1):
+        pos = np.full((n, 3), 7.3)
+        lay = build_layout(pos, np.ones((n, 3), np.float32), g)
+        if lay.n_used == 0:
+            continue  # nothing to stage on the device
+        k, res = _dev_pp(lib, lay, mode=lib.PP_FAST)
+        assert res.f2max_pp == 0.0 and res.p_acc == 0
+        assert np.array_equal(k.vp_host(), lay.vp)
+        assert res.ptotal_pp.sum() == n
+
+
+def test_buffers_regrow_between_calls(lib, oracle_lib, mixed_layout, c0_layout):
+    # small -> large -> small again, exact mode, each against the oracle
+    lay_m, ires = mixed_layout
+    for lay, lv in ((lay_m, ires), (c0_layout, None), (lay_m, ires)):
+        k, res = _dev_pp(lib, lay, ires=lv, mode=lib.PP_EXACT)
+        o = _oracle_pp(oracle_lib, lay, k.ires)
+        assert np.array_equal(k.vp_host().view(np.uint32), o["vp"].view(np.uint32))
