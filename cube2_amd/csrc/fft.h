@@ -1,0 +1,32 @@
+// Batched 3-D real FFTs of PM3 fine meshes, in place in the FFTW/cuFFT
+// padded layout: nb cubes of n^3 reals, x rows of n+2 floats, which after
+// the forward transform hold (n/2+1) complex values [kz][ky][kx].
+#pragma once
+#include "common.h"
+
+namespace cube2 {
+
+struct FFTPlan {
+    int n = 0;
+    int nst = 0;
+    int rad[16] = {0};
+    float2* tw = nullptr;  // exp(-2 pi i t / n), t < n (device)
+};
+
+// Plan for size n (radices 4, 2, 3, 5; cached; freed by fft_release()).
+int fft_plan(int n, const FFTPlan** out);
+void fft_release();
+
+// R2C along x for every row pair, scaling the real input by `scale`.
+int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, float scale, hipStream_t s);
+// C2R along x for every row pair.
+int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s);
+// Complex transforms along y (axis 1) or z (axis 2) of the half spectrum.
+//   mode 0: forward, 1: inverse, 2: forward, multiply by green[kz][ky][kx], inverse.
+int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
+             const float* green, hipStream_t s);
+
+// Full batched transforms (dir -1: R2C, +1: C2R), unnormalised.
+int fft3d_batch(float* d, int n, int nb, int dir, hipStream_t s);
+
+}  // namespace cube2

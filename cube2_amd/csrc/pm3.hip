@@ -1,24 +1,669 @@
-// PM3 (placeholder until the fine-mesh chain lands): every entry reports
-// that the stage is unavailable rather than computing anything.
+// PM3: fine-mesh particle-mesh force of every subtile (kick.f90:138-193,
+// pm_thread kick.f90:403-543, potential_to_force :545-568; GPU reference
+// pm3_fft_cuda.cu:276-401,402-591).
+//
+// Per kick the subtiles are grouped by level (one mesh size n per level)
+// and each group runs as batches of jobs sized to a memory budget:
+//   deposit   one wavefront per coarse cell of a subtile plus its one-cell
+//             halo; FP64 mesh coordinate, FP32 TSC weights (kick.f90:437-445),
+//             27 hardware FP32 atomic adds into the job's padded mesh;
+//   FFT       x R2C (scaled by the particle mass), y, z+Green+z^-1, y^-1,
+//             x C2R (fft.hip);
+//   gradient  phi/n^3, 4-point difference with the real(4) weights of
+//             parameters.f90:49-50, 3 force planes over [0, nfp+1]^3, max
+//             |F|^2 over the interior;
+//   interp    one wavefront per physical coarse cell, TSC gather of the 27
+//             force points, Delta-v = F*a_mid*dt/6/pi accumulated in FP64.
+// The Delta-v stays pending in the library's FP64 accumulator until PP
+// applies it (pp_force_kernel.cu:496-498).
+#include <algorithm>
+#include <map>
+
 #include "common.h"
+#include "fft.h"
+#include "profile.h"
 
 namespace cube2 {
-static int unavailable() {
-    set_error("PM3 is not available in this build");
-    return CUBE2_ERR_STATE;
+
+int compute_nzero(const int* d_rhoc, const int64_t* d_idx, const int64_t** out, hipStream_t s);
+
+struct PM3Job {
+    int t;      // parent tile (x-major)
+    int u[3];   // subtile index inside the tile
+    int it;     // subtile id (ixyz3 row)
+};
+
+struct PM3State {
+    DevBuf rho, force, jobs, red, ptot;
+};
+static PM3State g_pm3;
+
+struct Green {
+    const float* host;
+    size_t count;
+    float* dev;
+};
+static std::vector<Green> g_greens;
+
+// ---------------------------------------------------------------- kernels
+
+struct MeshArgs {
+    const int* rhoc;
+    const int64_t* nzero;
+    const int16_t* xp;
+    const PM3Job* jobs;
+    int njobs, nt, ntt, ncb, nte, nnt, ratio_cs;
+    int n, nfp, nfb, rsf;
+    int64_t n_used;
+};
+
+// TSC of one axis (kick.f90:439-445): centre cell Fortran index c+1 where
+// c = floor(x); FP32 weights from the FP64 coordinate.
+__device__ inline void tsc_axis(double x, int& c, float w[3]) {
+#pragma clang fp contract(off)
+    const double fl = floor(x);
+    c = (int)fl;
+    const float l3 = (float)(x - fl);
+    w[0] = (1.0f - l3) * (1.0f - l3) / 2.0f;
+    w[2] = l3 * l3 / 2.0f;
+    w[1] = 1.0f - w[0] - w[2];
 }
-int pm3_force(const int*, const int*, const int*, const int*, const int*, const int64_t*,
-              const int16_t*, int64_t, const float* const*, float, float, float, float*, double*) {
-    return unavailable();
+
+__device__ inline double mesh_coord(const MeshArgs& A, int c_nte, int16_t v, int u) {
+    const double xc = (double)(c_nte - A.ncb) + frac16(v);
+    return xc * (double)(A.ratio_cs * A.rsf) - (double)(u * A.nfp);
 }
-int pm3_debug_job(int, const int*, const int*, const int*, const int*, const int64_t*,
-                  const int16_t*, int64_t, const float*, float, float*, float*, float*, float*) {
-    return unavailable();
+
+constexpr int kWaves = 4;
+
+// Wavefront per (job, coarse cell of the subtile + 1 halo cell).
+__global__ __launch_bounds__(256) void k_pm3_deposit(MeshArgs A, float* __restrict__ rho, size_t rs,
+                                                     int* __restrict__ err) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ne = A.ntt + 2;
+    const int64_t per = (int64_t)ne * ne * ne;
+    const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
+    if (w >= per * A.njobs) return;
+    const int jb = (int)(w / per);
+    const int r = (int)(w - (int64_t)jb * per);
+    const PM3Job J = A.jobs[jb];
+    const int ci = J.u[0] * A.ntt + A.ncb - 1 + r % ne;
+    const int cj = J.u[1] * A.ntt + A.ncb - 1 + (r / ne) % ne;
+    const int ck = J.u[2] * A.ntt + A.ncb - 1 + r / (ne * ne);
+    const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
+    const int np = A.rhoc[cell];
+    const int64_t base = A.nzero[cell];
+    if (np <= 0) return;
+    if (base < 0 || base + np > A.n_used) {
+        if (lane == 0) atomicOr(err, 2);
+        return;
+    }
+    const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
+    float* R = rho + (size_t)jb * rs;
+    for (int l = lane; l < np; l += 64) {
+        const int16_t* p = A.xp + 3 * (base + l);
+        int cx, cy, cz;
+        float wx[3], wy[3], wz[3];
+        tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
+        tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
+        tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+#pragma unroll
+        for (int a3 = 0; a3 < 3; a3++) {
+            const int Z = cz + a3 + off;
+            if ((unsigned)Z >= (unsigned)n) continue;
+#pragma unroll
+            for (int a2 = 0; a2 < 3; a2++) {
+                const int Y = cy + a2 + off;
+                if ((unsigned)Y >= (unsigned)n) continue;
+                float* row = R + ((size_t)Z * n + Y) * (n + 2);
+#pragma unroll
+                for (int a1 = 0; a1 < 3; a1++) {
+                    const int X = cx + a1 + off;
+                    if ((unsigned)X >= (unsigned)n) continue;
+                    unsafeAtomicAdd(row + X, wx[a1] * wy[a2] * wz[a3]);
+                }
+            }
+        }
+    }
 }
-int velocity_apply(const int*, const int64_t*, float*, int64_t, float*) { return unavailable(); }
-int green_3d(float*, int, float, float, float) { return unavailable(); }
-int fft3d(float*, int, int, int) { return unavailable(); }
-void pm3_release() {}
-int green_cache_get(const float*, size_t, const float**) { return unavailable(); }
-void green_cache_release() {}
+
+// Force planes over Fortran indices [0, nfp+1]^3 (kick.f90:545-568) and the
+// interior max |F|^2 (kick.f90:494).
+__global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi, size_t rs,
+                                                  float* __restrict__ force, int nb, int n,
+                                                  int nfp, int nfb, float n3,
+                                                  unsigned* __restrict__ f2bits) {
+    const int nf = nfp + 2;
+    const size_t fs = (size_t)nf * nf * nf;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float f2 = 0.0f;
+    if (q < (int64_t)nb * (int64_t)fs) {
+        const int jb = (int)(q / (int64_t)fs);
+        const size_t u = (size_t)(q - (int64_t)jb * (int64_t)fs);
+        const int i0 = (int)(u % nf), j0 = (int)((u / nf) % nf), k0 = (int)(u / ((size_t)nf * nf));
+        const float* P = phi + (size_t)jb * rs;
+        const int x = i0 + nfb - 1, y = j0 + nfb - 1, z = k0 + nfb - 1;
+        const size_t sr = n + 2, sp = (size_t)n * (n + 2);
+        const size_t c = (size_t)z * sp + (size_t)y * sr + x;
+        // real(4) weights of alpha = 4/3 (parameters.f90:49-50)
+        const float al = 4.0f / 3.0f;
+        const float w0 = (al - 1.0f) / 4.0f, w1 = -al / 2.0f, w2 = al / 2.0f, w3 = (1.0f - al) / 4.0f;
+        float fx, fy, fz;
+        {
+#pragma clang fp contract(off)
+            fx = P[c - 2] / n3 * w0 + P[c - 1] / n3 * w1 + P[c + 1] / n3 * w2 + P[c + 2] / n3 * w3;
+            fy = P[c - 2 * sr] / n3 * w0 + P[c - sr] / n3 * w1 + P[c + sr] / n3 * w2 + P[c + 2 * sr] / n3 * w3;
+            fz = P[c - 2 * sp] / n3 * w0 + P[c - sp] / n3 * w1 + P[c + sp] / n3 * w2 + P[c + 2 * sp] / n3 * w3;
+        }
+        float* F = force + (size_t)jb * 3 * fs;
+        F[u] = fx;
+        F[fs + u] = fy;
+        F[2 * fs + u] = fz;
+        if (i0 >= 1 && i0 <= nfp && j0 >= 1 && j0 <= nfp && k0 >= 1 && k0 <= nfp)
+            f2 = fx * fx + fy * fy + fz * fz;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) f2 = fmaxf(f2, __shfl_xor(f2, o, 64));
+    if ((threadIdx.x & 63) == 0 && f2 > 0.0f) atomicMax(f2bits, __float_as_uint(f2));
+}
+
+// Wavefront per (job, physical coarse cell): TSC gather and FP64 Delta-v.
+__global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __restrict__ force,
+                                                    double* __restrict__ dv64, int accumulate,
+                                                    float amid, float dt,
+                                                    unsigned long long* __restrict__ ptot) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t per = (int64_t)A.ntt * A.ntt * A.ntt;
+    const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
+    if (w >= per * A.njobs) return;
+    const int jb = (int)(w / per);
+    const int r = (int)(w - (int64_t)jb * per);
+    const PM3Job J = A.jobs[jb];
+    const int ci = J.u[0] * A.ntt + A.ncb + r % A.ntt;
+    const int cj = J.u[1] * A.ntt + A.ncb + (r / A.ntt) % A.ntt;
+    const int ck = J.u[2] * A.ntt + A.ncb + r / (A.ntt * A.ntt);
+    const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
+    const int np = A.rhoc[cell];
+    const int64_t base = A.nzero[cell];
+    if (np <= 0 || base < 0 || base + np > A.n_used) return;
+    if (lane == 0) atomicAdd(&ptot[jb], (unsigned long long)np);
+    const int nf = A.nfp + 2;
+    const size_t fs = (size_t)nf * nf * nf;
+    const float* F = force + (size_t)jb * 3 * fs;
+    for (int l = lane; l < np; l += 64) {
+        const int64_t ip = base + l;
+        const int16_t* p = A.xp + 3 * ip;
+        int cx, cy, cz;
+        float wx[3], wy[3], wz[3];
+        tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
+        tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
+        tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+        float X = 0.0f, Y = 0.0f, Z = 0.0f;
+#pragma unroll
+        for (int a3 = 0; a3 < 3; a3++)
+#pragma unroll
+            for (int a2 = 0; a2 < 3; a2++)
+#pragma unroll
+                for (int a1 = 0; a1 < 3; a1++) {
+#pragma clang fp contract(off)
+                    const size_t u = ((size_t)(cz + a3) * nf + (cy + a2)) * nf + (cx + a1);
+                    X = X + F[u] * wx[a1] * wy[a2] * wz[a3];
+                    Y = Y + F[fs + u] * wx[a1] * wy[a2] * wz[a3];
+                    Z = Z + F[2 * fs + u] * wx[a1] * wy[a2] * wz[a3];
+                }
+        const float acc[3] = {X, Y, Z};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double t;
+            {
+#pragma clang fp contract(off)
+                t = (double)(acc[c] * amid * dt / 6.0f) / 3.14159265358979323846;
+            }
+            if (accumulate) dv64[3 * ip + c] += t;
+            else dv64[3 * ip + c] = t;
+        }
+    }
+}
+
+// Pending Delta-v applied without PP; vmax over physical slots.
+__global__ __launch_bounds__(256) void k_velocity_apply(const int* __restrict__ rhoc,
+                                                        const int64_t* __restrict__ nzero, int nt,
+                                                        int ncb, int nte, int ntile, int64_t n_used,
+                                                        const double* __restrict__ dv64,
+                                                        float* __restrict__ vp,
+                                                        unsigned* __restrict__ vbits) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t per = (int64_t)nt * nt * nt;
+    const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
+    float v0 = 0, v1 = 0, v2 = 0;
+    if (w < per * ntile) {
+        const int t = (int)(w / per);
+        const int r = (int)(w - (int64_t)t * per);
+        const int i = ncb + r % nt, j = ncb + (r / nt) % nt, k = ncb + r / (nt * nt);
+        const int64_t cell = (((int64_t)t * nte + k) * nte + j) * nte + i;
+        const int np = rhoc[cell];
+        const int64_t base = nzero[cell];
+        if (np > 0 && base >= 0 && base + np <= n_used)
+            for (int l = lane; l < np; l += 64) {
+                const size_t ip = (size_t)(base + l) * 3;
+                float nv[3];
+                for (int c = 0; c < 3; c++) {
+                    double v = (double)vp[ip + c];
+                    if (dv64) v += dv64[ip + c];
+                    nv[c] = (float)v;
+                    vp[ip + c] = nv[c];
+                }
+                v0 = fmaxf(v0, fabsf(nv[0]));
+                v1 = fmaxf(v1, fabsf(nv[1]));
+                v2 = fmaxf(v2, fabsf(nv[2]));
+            }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v0 = fmaxf(v0, __shfl_xor(v0, o, 64));
+        v1 = fmaxf(v1, __shfl_xor(v1, o, 64));
+        v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+    }
+    if (lane == 0) {
+        if (v0 > 0) atomicMax(&vbits[0], __float_as_uint(v0));
+        if (v1 > 0) atomicMax(&vbits[1], __float_as_uint(v1));
+        if (v2 > 0) atomicMax(&vbits[2], __float_as_uint(v2));
+    }
+}
+
+// Green_3D (Green/Green.f90:1-80) in FP64; kshift = 0, n1 = n/2+1.  The
+// Fortran real(4) literals and arguments are kept as float values promoted
+// to double: 1./nglobal, 2./15, alpha = 4./3, 1-alpha, gridsize**2.
+__global__ __launch_bounds__(256) void k_green(float* __restrict__ G, int n, float apm_soft,
+                                               float apm_range, float gridsize) {
+    const int nh = n / 2 + 1;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nh * n * n) return;
+    const int k1 = (int)(q % nh), k2 = (int)((q / nh) % n), k3 = (int)(q / ((int64_t)nh * n));
+    const double PI = 3.14159265358979323846;
+    const double base = 2.0 * PI * (double)(1.0f / (float)n);
+    auto fs = [&](int t) { return base * (double)(((t + n / 2) % n) - n / 2); };
+    const double kv[3] = {fs(k1), fs(k2), fs(k3)};
+    const double c215 = (double)(2.0f / 15.0f);
+    const float alf = 4.0f / 3.0f;
+    const double alpha = (double)alf, oma = (double)(1.0f - alf);
+    double D[3], u2n = 1.0;
+    for (int c = 0; c < 3; c++) {
+        const double s = sin(kv[c] / 2);
+        u2n *= 1 - s * s + c215 * (s * s) * (s * s);
+        D[c] = alpha * sin(kv[c]) + oma * sin(2 * kv[c]) / 2;
+    }
+    double ur[3] = {0, 0, 0};
+    for (int i3 = -2; i3 <= 2; i3++)
+        for (int i2 = -2; i2 <= 2; i2++)
+            for (int i1 = -2; i1 <= 2; i1++) {
+                const double kn[3] = {kv[0] + 2 * PI * i1, kv[1] + 2 * PI * i2, kv[2] + 2 * PI * i3};
+                const double km = sqrt(kn[0] * kn[0] + kn[1] * kn[1] + kn[2] * kn[2]);
+                if (km == 0) continue;  // only at k = 0, zeroed below
+                double u = 1.0;
+                for (int c = 0; c < 3; c++) u *= kn[c] == 0 ? 1.0 : sin(kn[c] / 2) / (kn[c] / 2);
+                const double u2 = u * u, uk2 = u2 * u2 * u2;  // **(2p+2), p = 2 (TSC)
+                double y = (double)apm_soft * km / 2;
+                const double ss = 12 * pow(y, -4.0) * (2 - 2 * cos(y) - y * sin(y));
+                double sr;
+                if (apm_range == 0) sr = 0;
+                else {
+                    y = (double)apm_range * km / 2;
+                    sr = 12 * pow(y, -4.0) * (2 - 2 * cos(y) - y * sin(y));
+                }
+                const double f = (ss * ss - sr * sr) / (km * km);
+                for (int c = 0; c < 3; c++) ur[c] += uk2 * kn[c] * f;
+            }
+    const double num = D[0] * ur[0] + D[1] * ur[1] + D[2] * ur[2];
+    const double den = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+    const double gs2 = (double)(gridsize * gridsize);
+    double g = 4 * PI * num / den / (u2n * u2n) / gs2;
+    const bool corner = (k1 == 0 || k1 == n / 2) && (k2 == 0 || k2 == n / 2) && (k3 == 0 || k3 == n / 2);
+    if (corner) g = 0.0;  // Green.f90:71
+    G[q] = (float)g;
+}
+
+// ------------------------------------------------------------ host side
+
+struct LevelGroup {
+    int L;
+    std::vector<PM3Job> jobs;
+};
+
+static int make_groups(const Geom& g, const int* isort, const int* ires, const int* ixyz3,
+                       std::vector<LevelGroup>& groups) {
+    groups.clear();
+    std::vector<int> seen(g.ns3, 0);
+    std::map<int, size_t> idx;
+    for (int o = 0; o < g.ns3; o++) {
+        const int it = isort ? isort[o] - 1 : o;
+        if (it < 0 || it >= g.ns3 || seen[it]) {
+            set_error("isort is not a permutation of 1..ns3 (entry %d)", o);
+            return CUBE2_ERR_ARG;
+        }
+        seen[it] = 1;
+        const int* x = ixyz3 + 6 * it;
+        const int L = ires[it];
+        for (int q = 0; q < 3; q++)
+            if (x[q] < 1 || x[q] > g.p.nns || x[q + 3] < 1 || x[q + 3] > g.p.nnt || L < 2 || L > 6) {
+                set_error("invalid PM3 subtile metadata at %d (ires=%d)", it, L);
+                return CUBE2_ERR_ARG;
+            }
+        PM3Job J;
+        J.t = ((x[5] - 1) * g.p.nnt + (x[4] - 1)) * g.p.nnt + (x[3] - 1);
+        for (int q = 0; q < 3; q++) J.u[q] = x[q] - 1;
+        J.it = it;
+        if (!idx.count(L)) {
+            idx[L] = groups.size();
+            groups.push_back({L, {}});
+        }
+        groups[idx[L]].jobs.push_back(J);
+    }
+    return CUBE2_OK;
+}
+
+static size_t g_budget = (size_t)16 << 30;  // bytes of mesh + force planes per batch
+
+struct DebugOut {
+    float *rho = nullptr, *phi = nullptr, *force = nullptr;
+};
+
+// Runs one batch of jobs of one level; f2bits and ptot are device scalars.
+static int run_batch(const MeshArgs& base, const std::vector<PM3Job>& jobs, size_t j0, size_t nb,
+                     const float* green, float mass, float amid, float dt, int accumulate,
+                     double* dv64, unsigned* f2bits, unsigned long long* ptot, int* err,
+                     DebugOut* dbg, hipStream_t s) {
+    const int n = base.n, nfp = base.nfp;
+    const size_t rs = (size_t)(n + 2) * n * n;
+    const size_t fs = (size_t)(nfp + 2) * (nfp + 2) * (nfp + 2);
+    C2_TRY(g_pm3.rho.reserve(nb * rs * sizeof(float)));
+    C2_TRY(g_pm3.force.reserve(nb * 3 * fs * sizeof(float)));
+    C2_TRY(g_pm3.jobs.reserve(nb * sizeof(PM3Job)));
+    C2_HIP(hipMemcpyAsync(g_pm3.jobs.ptr, jobs.data() + j0, nb * sizeof(PM3Job), hipMemcpyHostToDevice, s));
+    MeshArgs A = base;
+    A.jobs = g_pm3.jobs.as<PM3Job>();
+    A.njobs = (int)nb;
+    float* rho = g_pm3.rho.as<float>();
+    float* force = g_pm3.force.as<float>();
+    const FFTPlan* P = nullptr;
+    C2_TRY(fft_plan(n, &P));
+    {
+        Stage st("pm3_clear", (double)nb * rs * 4);
+        C2_HIP(hipMemsetAsync(rho, 0, nb * rs * sizeof(float), s));
+    }
+    {
+        const int64_t waves = (int64_t)nb * (base.ntt + 2) * (base.ntt + 2) * (base.ntt + 2);
+        Stage st("pm3_deposit", (double)nb * rs * 4);
+        k_pm3_deposit<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, rho, rs, err);
+        C2_HIP(hipGetLastError());
+    }
+    if (dbg && dbg->rho) {
+        std::vector<float> h(rs);
+        C2_HIP(hipMemcpyAsync(h.data(), rho, rs * sizeof(float), hipMemcpyDeviceToHost, s));
+        C2_HIP(hipStreamSynchronize(s));
+        for (int z = 0; z < n; z++)
+            for (int y = 0; y < n; y++)
+                for (int x = 0; x < n; x++)
+                    dbg->rho[((size_t)z * n + y) * n + x] = h[((size_t)z * n + y) * (n + 2) + x] * mass;
+    }
+    const double mesh_bytes = (double)nb * rs * 4;
+    {
+        Stage st("pm3_fft_x_r2c", 2 * mesh_bytes);
+        C2_TRY(fft_x_r2c(rho, *P, (int)nb, rs, mass, s));
+    }
+    {
+        Stage st("pm3_fft_y", 2 * mesh_bytes);
+        C2_TRY(fft_cols(rho, *P, (int)nb, rs, 1, 0, nullptr, s));
+    }
+    {
+        Stage st("pm3_fft_z_green", 2 * mesh_bytes + (double)(n / 2 + 1) * n * n * 4);
+        C2_TRY(fft_cols(rho, *P, (int)nb, rs, 2, 2, green, s));
+    }
+    {
+        Stage st("pm3_fft_y_inv", 2 * mesh_bytes);
+        C2_TRY(fft_cols(rho, *P, (int)nb, rs, 1, 1, nullptr, s));
+    }
+    {
+        Stage st("pm3_fft_x_c2r", 2 * mesh_bytes);
+        C2_TRY(fft_x_c2r(rho, *P, (int)nb, rs, s));
+    }
+    const float n3 = (float)n * (float)n * (float)n;
+    if (dbg && dbg->phi) {
+        std::vector<float> h(rs);
+        C2_HIP(hipMemcpyAsync(h.data(), rho, rs * sizeof(float), hipMemcpyDeviceToHost, s));
+        C2_HIP(hipStreamSynchronize(s));
+        for (int z = 0; z < n; z++)
+            for (int y = 0; y < n; y++)
+                for (int x = 0; x < n; x++)
+                    dbg->phi[((size_t)z * n + y) * n + x] = h[((size_t)z * n + y) * (n + 2) + x] / n3;
+    }
+    {
+        Stage st("pm3_grad", mesh_bytes + (double)nb * 3 * fs * 4);
+        k_pm3_grad<<<ceil_div(nb * fs, 256), 256, 0, s>>>(rho, rs, force, (int)nb, n, nfp, base.nfb,
+                                                          n3, f2bits);
+        C2_HIP(hipGetLastError());
+    }
+    if (dbg && dbg->force) {
+        C2_HIP(hipMemcpyAsync(dbg->force, force, 3 * fs * sizeof(float), hipMemcpyDeviceToHost, s));
+        C2_HIP(hipStreamSynchronize(s));
+    }
+    if (dv64) {
+        const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt;
+        Stage st("pm3_interp", (double)nb * 3 * fs * 4);
+        k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, amid, dt, ptot);
+        C2_HIP(hipGetLastError());
+    }
+    return CUBE2_OK;
+}
+
+static MeshArgs mesh_args(const Geom& g, const int* d_rhoc, const int64_t* nz, const int16_t* d_xp,
+                          int64_t n_used, int rsf) {
+    MeshArgs A{};
+    A.rhoc = d_rhoc;
+    A.nzero = nz;
+    A.xp = d_xp;
+    A.nt = g.nt;
+    A.ntt = g.ntt;
+    A.ncb = g.ncb;
+    A.nte = g.nte;
+    A.nnt = g.p.nnt;
+    A.ratio_cs = g.p.ratio_cs;
+    A.rsf = rsf;
+    A.nfp = g.ntt * g.p.ratio_cs * rsf;
+    A.nfb = g.p.ratio_cs * rsf;
+    A.n = A.nfp + 2 * A.nfb;
+    A.n_used = n_used;
+    return A;
+}
+
+int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
+              const float* const* d_green, float mass, float amid, float dt, float* f2max,
+              double* ptotal) {
+    ensure_params();
+    Context& C = ctx();
+    const Geom& g = C.g;
+    if (!ires || !ixyz3 || !ratio_sf || !f2max || !ptotal || !d_green) {
+        set_error("null argument");
+        return CUBE2_ERR_ARG;
+    }
+    if (n_used < 0 || n_used >= ((int64_t)1 << 31)) {
+        set_error("invalid PM3 particle count %lld", (long long)n_used);
+        return CUBE2_ERR_CAPACITY;
+    }
+    std::vector<LevelGroup> groups;
+    C2_TRY(make_groups(g, isort, ires, ixyz3, groups));
+    for (auto& G : groups)
+        if (!d_green[G.L - 2]) {
+            set_error("no Green's table for level %d", G.L);
+            return CUBE2_ERR_ARG;
+        }
+    C2_TRY(ensure_init());
+    hipStream_t s = C.stream;
+    if (C.dv_state == 1 && C.dv_n_used != n_used) {
+        set_error("PM3 particle count changed while a Delta-v is pending");
+        return CUBE2_ERR_STATE;
+    }
+    const int accumulate = C.dv_state == 1 ? 1 : 0;
+    C2_TRY(C.dv64.reserve((size_t)std::max<int64_t>(n_used, 1) * 3 * sizeof(double)));
+    const int64_t* nz = nullptr;
+    C2_TRY(compute_nzero(d_rhoc, d_idx, &nz, s));
+    C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
+    unsigned* f2bits = g_pm3.red.as<unsigned>();
+    int* err = reinterpret_cast<int*>(f2bits + 1);
+    C2_HIP(hipMemsetAsync(f2bits, 0, 4 * sizeof(unsigned), s));
+    size_t maxjobs = 0;
+    for (auto& G : groups) maxjobs = std::max(maxjobs, G.jobs.size());
+    C2_TRY(g_pm3.ptot.reserve(std::max<size_t>(g.ns3, 1) * sizeof(unsigned long long)));
+    unsigned long long* ptot = g_pm3.ptot.as<unsigned long long>();
+    C2_HIP(hipMemsetAsync(ptot, 0, g.ns3 * sizeof(unsigned long long), s));
+    std::vector<unsigned long long> hp(g.ns3, 0);
+    std::vector<int> job_it;  // subtile of each ptot slot
+    size_t slot_base = 0;
+    for (auto& G : groups) {
+        const int rsf = ratio_sf[G.L - 1];
+        MeshArgs A = mesh_args(g, d_rhoc, nz, d_xp, n_used, rsf);
+        const size_t rs = (size_t)(A.n + 2) * A.n * A.n;
+        const size_t fsz = (size_t)(A.nfp + 2) * (A.nfp + 2) * (A.nfp + 2);
+        const size_t per_job = (rs + 3 * fsz) * sizeof(float);
+        const size_t batch = std::max<size_t>(1, std::min(G.jobs.size(), g_budget / per_job));
+        for (size_t j0 = 0; j0 < G.jobs.size(); j0 += batch) {
+            const size_t nb = std::min(batch, G.jobs.size() - j0);
+            for (size_t q = 0; q < nb; q++) job_it.push_back(G.jobs[j0 + q].it);
+            C2_TRY(run_batch(A, G.jobs, j0, nb, d_green[G.L - 2], mass, amid, dt, accumulate,
+                             C.dv64.as<double>(), f2bits, ptot + slot_base, err, nullptr, s));
+            slot_base += nb;
+        }
+    }
+    unsigned h[4];
+    C2_HIP(hipMemcpyAsync(h, f2bits, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(hp.data(), ptot, g.ns3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    C2_TRY(sync_and_flush());
+    if (h[1]) {
+        set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)n_used);
+        return CUBE2_ERR_ARG;
+    }
+    *f2max = __builtin_bit_cast(float, h[0]);
+    for (int it = 0; it < g.ns3; it++) ptotal[it] = 0.0;
+    for (size_t q = 0; q < job_it.size(); q++) ptotal[job_it[q]] = (double)hp[q];
+    C.dv_state = 1;
+    C.dv_n_used = n_used;
+    return CUBE2_OK;
+}
+
+int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf, const int* d_rhoc,
+                  const int64_t* d_idx, const int16_t* d_xp, int64_t n_used, const float* d_green,
+                  float mass, float* rho, float* phi, float* force, float* f2max) {
+    ensure_params();
+    Context& C = ctx();
+    const Geom& g = C.g;
+    if (it < 0 || it >= g.ns3 || !d_green) {
+        set_error("bad subtile %d or no Green's table", it);
+        return CUBE2_ERR_ARG;
+    }
+    std::vector<LevelGroup> groups;
+    C2_TRY(make_groups(g, nullptr, ires, ixyz3, groups));
+    C2_TRY(ensure_init());
+    hipStream_t s = C.stream;
+    PM3Job J{};
+    for (auto& G : groups)
+        for (auto& j : G.jobs)
+            if (j.it == it) J = j;
+    const int L = ires[it];
+    MeshArgs A = mesh_args(g, d_rhoc, nullptr, d_xp, n_used, ratio_sf[L - 1]);
+    const int64_t* nz = nullptr;
+    C2_TRY(compute_nzero(d_rhoc, d_idx, &nz, s));
+    A.nzero = nz;
+    C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
+    unsigned* f2bits = g_pm3.red.as<unsigned>();
+    C2_HIP(hipMemsetAsync(f2bits, 0, 4 * sizeof(unsigned), s));
+    std::vector<PM3Job> one{J};
+    DebugOut dbg{rho, phi, force};
+    C2_TRY(run_batch(A, one, 0, 1, d_green, mass, 0.0f, 0.0f, 0, nullptr, f2bits, nullptr,
+                     reinterpret_cast<int*>(f2bits + 1), &dbg, s));
+    unsigned h[4];
+    C2_HIP(hipMemcpyAsync(h, f2bits, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    C2_TRY(sync_and_flush());
+    if (f2max) *f2max = __builtin_bit_cast(float, h[0]);
+    return CUBE2_OK;
+}
+
+int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t n_used,
+                   float* vmax) {
+    C2_TRY(ensure_init());
+    Context& C = ctx();
+    const Geom& g = C.g;
+    hipStream_t s = C.stream;
+    if (C.dv_state == 1 && C.dv_n_used != n_used) {
+        set_error("particle count changed while a Delta-v is pending");
+        return CUBE2_ERR_STATE;
+    }
+    const int64_t* nz = nullptr;
+    C2_TRY(compute_nzero(d_rhoc, d_idx, &nz, s));
+    C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
+    unsigned* vb = g_pm3.red.as<unsigned>();
+    C2_HIP(hipMemsetAsync(vb, 0, 4 * sizeof(unsigned), s));
+    const int64_t waves = (int64_t)g.ntile * g.nt * g.nt * g.nt;
+    k_velocity_apply<<<ceil_div(waves, kWaves), 256, 0, s>>>(
+        d_rhoc, nz, g.nt, g.ncb, g.nte, g.ntile, n_used,
+        C.dv_state == 1 ? C.dv64.as<double>() : nullptr, d_vp, vb);
+    C2_HIP(hipGetLastError());
+    unsigned h[4];
+    C2_HIP(hipMemcpyAsync(h, vb, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    C2_TRY(sync_and_flush());
+    for (int d = 0; d < 3; d++) vmax[d] = __builtin_bit_cast(float, h[d]);
+    C.dv_state = 0;
+    return CUBE2_OK;
+}
+
+int green_3d(float* d_G, int n, float apm_soft, float apm_range, float gridsize) {
+    if (!d_G || n < 2 || n % 2) {
+        set_error("bad Green's table request (n=%d)", n);
+        return CUBE2_ERR_ARG;
+    }
+    C2_TRY(ensure_init());
+    hipStream_t s = ctx().stream;
+    const int64_t tot = (int64_t)(n / 2 + 1) * n * n;
+    {
+        Stage st("green_3d", (double)tot * 4);
+        k_green<<<ceil_div(tot, 256), 256, 0, s>>>(d_G, n, apm_soft, apm_range, gridsize);
+        C2_HIP(hipGetLastError());
+    }
+    return sync_and_flush();
+}
+
+int fft3d(float* d, int n, int nb, int dir) {
+    C2_TRY(ensure_init());
+    C2_TRY(fft3d_batch(d, n, nb, dir, ctx().stream));
+    return sync_and_flush();
+}
+
+int green_cache_get(const float* host, size_t count, const float** dev) {
+    for (auto& G : g_greens)
+        if (G.host == host && G.count == count) {
+            *dev = G.dev;
+            return CUBE2_OK;
+        }
+    Green G{host, count, nullptr};
+    C2_HIP(hipMalloc(&G.dev, count * sizeof(float)));
+    C2_HIP(hipMemcpy(G.dev, host, count * sizeof(float), hipMemcpyHostToDevice));
+    g_greens.push_back(G);
+    *dev = G.dev;
+    return CUBE2_OK;
+}
+
+void green_cache_release() {
+    for (auto& G : g_greens) (void)hipFree(G.dev);
+    g_greens.clear();
+}
+
+void pm3_release() {
+    g_pm3.rho.release();
+    g_pm3.force.release();
+    g_pm3.jobs.release();
+    g_pm3.red.release();
+    g_pm3.ptot.release();
+    fft_release();
+}
+
 }  // namespace cube2

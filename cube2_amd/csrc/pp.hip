@@ -281,6 +281,20 @@ static int location_maps(const Geom& g, const int* ires, const int* ixyz3, std::
     return CUBE2_OK;
 }
 
+// Cell base slots of the whole image (shared with PM3).
+int compute_nzero(const int* d_rhoc, const int64_t* d_idx, const int64_t** out, hipStream_t s) {
+    const Geom& g = ctx().g;
+    const int64_t rows = (int64_t)g.ntile * g.nte * g.nte;
+    C2_TRY(g_pp.nzero.reserve(std::max<int64_t>(rows * g.nte, 1) * sizeof(int64_t)));
+    if (rows > 0) {
+        Stage st("nzero", (double)rows * g.nte * 12 + rows * 8);
+        k_nzero<<<ceil_div(rows, 256), 256, 0, s>>>(d_rhoc, d_idx, g_pp.nzero.as<int64_t>(), g.nte, rows);
+        C2_HIP(hipGetLastError());
+    }
+    *out = g_pp.nzero.as<int64_t>();
+    return CUBE2_OK;
+}
+
 int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf) {
     ensure_params();
     if (!ires || !ixyz3 || !ratio_sf) {
@@ -344,11 +358,8 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
         set_error("PP cell grid too large (%lld cells)", (long long)c_tot);
         return CUBE2_ERR_CAPACITY;
     }
-    const int64_t ncell_rows = (int64_t)ntile * g.nte * g.nte;
-    const int64_t ncells_all = ncell_rows * g.nte;
     const int64_t ncells_ext = (int64_t)ntile * (g.nt + 2) * (g.nt + 2) * (g.nt + 2);
     const size_t ncap = (size_t)std::max<int64_t>(n_used, 1);
-    C2_TRY(g_pp.nzero.reserve(ncells_all * sizeof(int64_t)));
     C2_TRY(g_pp.tiles.reserve(ntile * sizeof(PPTile)));
     C2_TRY(g_pp.ires_loc.reserve(g.ns3 * sizeof(int)));
     C2_TRY(g_pp.apm3.reserve(8 * sizeof(float)));
@@ -365,12 +376,8 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
     unsigned long long* ploc = reinterpret_cast<unsigned long long*>(red + 1);
     C2_HIP(hipMemsetAsync(red, 0, sizeof(PPRed) + g.ns3 * sizeof(unsigned long long), s));
     g_pp.valid = false;
-    {
-        Stage st("pp_nzero", (double)ncells_all * 12 + ncell_rows * 8);
-        k_nzero<<<ceil_div(ncell_rows, 256), 256, 0, s>>>(d_rhoc, d_idx, g_pp.nzero.as<int64_t>(),
-                                                         g.nte, ncell_rows);
-        C2_HIP(hipGetLastError());
-    }
+    const int64_t* nzp = nullptr;
+    C2_TRY(compute_nzero(d_rhoc, d_idx, &nzp, s));
     if (ncells_ext > 0) {
         Stage st("pp_count", (double)n_used * 6 + c_tot * 4.0);
         k_pp_count<<<ceil_div(ncells_ext, kWaves), 256, 0, s>>>(

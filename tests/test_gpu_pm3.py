@@ -1,0 +1,209 @@
+"""PM3 on the GPU through the C ABI against numpy's FFT and the oracle.
+Tolerances (FP32 GPU path vs FP64 oracle): relative RMS <= 1e-5 for the
+potential, force planes and Delta-v; <= 2e-6 for the FFT; Green's tables
+(FP64 on both sides, stored FP32) agree to 2 ulp."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL_PM3 = 1e-5
+RTOL_FFT = 2e-6
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-300)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+    assert torch.cuda.is_available()
+    from cube2_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+@pytest.mark.parametrize("n", [48, 80, 144, 160, 240])
+def test_fft_r2c_c2r_match_numpy(lib, n):
+    import torch
+    nb = 2
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((nb, n, n, n)).astype(np.float32)
+    buf = np.zeros((nb, n, n, n + 2), np.float32)
+    buf[..., :n] = x
+    d = torch.from_numpy(buf).cuda()
+    lib.check(lib.lib().cube2_fft3d_dev(d.data_ptr(), n, nb, -1))
+    got = d.cpu().numpy().view(np.complex64).astype(np.complex128)      # (nb, n, n, n/2+1)
+    want = np.fft.rfftn(x.astype(np.float64), axes=(1, 2, 3))
+    err = rel_rms(np.abs(got - want), np.abs(want)) if False else \
+        float(np.sqrt((np.abs(got - want) ** 2).sum() / (np.abs(want) ** 2).sum()))
+    print(f"n={n}: R2C relative RMS {err:.2e}")
+    assert err < RTOL_FFT
+    lib.check(lib.lib().cube2_fft3d_dev(d.data_ptr(), n, nb, +1))
+    back = d.cpu().numpy()[..., :n].astype(np.float64) / n ** 3
+    err2 = rel_rms(back, x)
+    print(f"n={n}: round trip relative RMS {err2:.2e}")
+    assert err2 < RTOL_FFT
+
+
+@pytest.mark.parametrize("rsf,n", [(2, 48), (4, 96), (2, 80)])
+def test_green_fp64_kernel_matches_oracle(lib, oracle_lib, rsf, n):
+    import torch
+    gs = np.float32(1) / np.float32(rsf)
+    t = torch.empty((n, n, n // 2 + 1), dtype=torch.float32, device="cuda")
+    lib.check(lib.lib().cube2_green_3d_dev(t.data_ptr(), n, 3.5, 3.5 * rsf, float(gs)))
+    got = t.cpu().numpy()
+    want = oracle_lib.green_3d(n, 3.5, 3.5 * rsf, gs)
+    assert np.allclose(got, want, rtol=2.5e-7, atol=0)
+
+
+def _green(lib, geom, level):
+    import torch
+    from cube2_amd.geometry import APM3F, RATIO_SF
+    n = geom.nft(level)
+    t = torch.empty((n, n, n // 2 + 1), dtype=torch.float32, device="cuda")
+    rsf = RATIO_SF[level - 1]
+    lib.check(lib.lib().cube2_green_3d_dev(t.data_ptr(), n, float(APM3F), float(APM3F * rsf),
+                                           float(np.float32(1) / np.float32(rsf))))
+    return t
+
+
+@pytest.fixture(scope="module")
+def c0_layout():
+    from cube2_amd.geometry import C0
+    from cube2_amd.layout import build_layout
+    from cube2_amd.synth import config0
+    pos, vel = config0(seed=0)
+    return build_layout(pos, vel, C0)
+
+
+def test_pm3_stages_match_oracle(lib, oracle_lib, c0_layout):
+    import torch
+    from cube2_amd.geometry import RATIO_SF, ixyz3
+    lay = c0_layout
+    g = lay.geom
+    lib.set_params(g)
+    G = _green(lib, g, 2)
+    ires = np.full(g.ns3, 2, np.int32)
+    ix = ixyz3(g)
+    n, nf = g.nft(2), g.nfp(2) + 2
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in
+         dict(rhoc=lay.rhoc, idx=lay.idx_b_r, xp=lay.xp).items()}
+    rho = np.zeros((n, n, n), np.float32)
+    phi = np.zeros((n, n, n), np.float32)
+    force = np.zeros((3, nf, nf, nf), np.float32)
+    f2 = np.zeros(1, np.float32)
+    mass = 1.25
+    lib.check(lib.lib().cube2_pm3_debug_job(0, ires.ctypes.data, ix.ctypes.data, RATIO_SF.ctypes.data,
+                                            d["rhoc"].data_ptr(), d["idx"].data_ptr(),
+                                            d["xp"].data_ptr(), lay.n_used, G.data_ptr(), mass,
+                                            rho.ctypes.data, phi.ctypes.data, force.ctypes.data,
+                                            f2.ctypes.data))
+    o = oracle_lib.pm3_job(g, 0, ires, ix, RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
+                           G.cpu().numpy(), mass)
+    e_rho, e_phi, e_f = rel_rms(rho, o["rho"]), rel_rms(phi, o["phi"]), rel_rms(force, o["force"])
+    print(f"rho {e_rho:.2e} phi {e_phi:.2e} force {e_f:.2e} f2 {f2[0]:.6e} vs {o['f2max']:.6e}")
+    assert e_rho < 1e-6
+    assert e_phi < RTOL_PM3
+    assert e_f < RTOL_PM3
+    assert abs(f2[0] - o["f2max"]) < 3e-5 * o["f2max"]
+
+
+def _pm3_oracle_dv(oracle_lib, lay, ires, isort, greens, mass=1.0, amid=0.02, dt=1e-3):
+    from cube2_amd.geometry import RATIO_SF, ixyz3
+    return oracle_lib.pm3_force(lay.geom, isort, ires, ixyz3(lay.geom), RATIO_SF, lay.rhoc,
+                                lay.idx_b_r, lay.xp, greens, mass, amid, dt)
+
+
+def test_pm3_kick_c0_matches_oracle(lib, oracle_lib, c0_layout):
+    from cube2_amd.kick import ShortRangeKick
+    lay = c0_layout
+    k = ShortRangeKick(lay)
+    res = k.kick(1.0, 0.02, 1e-3, pm3=True, pp=False)     # PM3 then apply
+    greens = {L: k.green(L).cpu().numpy() for L in set(int(x) for x in k.ires)}
+    o = _pm3_oracle_dv(oracle_lib, lay, k.ires, k.isort, greens)
+    phys = lay.physical_mask()
+    dv = k.vp_host()[phys].astype(np.float64) - lay.vp[phys]
+    err = rel_rms(dv, o["dv64"][phys])
+    print(f"C0 PM3 Delta-v relative RMS {err:.2e}")
+    assert err < RTOL_PM3
+    assert abs(res.f2max_pm3 - o["f2max"]) < 3e-5 * o["f2max"]
+    assert np.array_equal(res.ptotal_pm3, o["ptotal"])
+
+
+@pytest.fixture(scope="module")
+def mixed_layout():
+    from cube2_amd.geometry import Geometry
+    from cube2_amd.layout import build_layout
+    g = Geometry(ng=64, nnt=2, nns=2, ratio_cs=4, ngb=8, app=0.05)
+    rng = np.random.default_rng(23)
+    pos = np.concatenate([rng.uniform(0, 64, (20000, 3)),
+                          np.array([31.5, 30.0, 33.0]) + rng.normal(0, 0.7, (3000, 3))]) % 64
+    lay = build_layout(pos, rng.normal(0, 1, pos.shape).astype(np.float32), g)
+    ires = rng.integers(2, 5, g.ns3).astype(np.int32)          # levels 2..4 (n = 48, 96, 144)
+    return lay, ires
+
+
+def test_pm3_mixed_levels_match_oracle(lib, oracle_lib, mixed_layout):
+    from cube2_amd.kick import ShortRangeKick
+    lay, ires = mixed_layout
+    k = ShortRangeKick(lay)
+    k.set_levels(ires)
+    res = k.kick(1.3, 0.05, 2e-3, pm3=True, pp=False)
+    greens = {L: k.green(L).cpu().numpy() for L in set(int(x) for x in ires)}
+    o = _pm3_oracle_dv(oracle_lib, lay, ires, k.isort, greens, 1.3, 0.05, 2e-3)
+    phys = lay.physical_mask()
+    dv = k.vp_host()[phys].astype(np.float64) - lay.vp[phys]
+    err = rel_rms(dv, o["dv64"][phys])
+    print(f"mixed PM3 Delta-v relative RMS {err:.2e}")
+    assert err < RTOL_PM3
+    assert np.array_equal(res.ptotal_pm3, o["ptotal"])
+
+
+def test_dropin_pm3_then_pp_matches_oracle(lib, oracle_lib, mixed_layout):
+    """c_pm3_force_kernel then c_pp_force_kernel with host arrays, as kick.f90 calls them."""
+    from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
+    from cube2_amd.kick import ShortRangeKick
+    lay, ires = mixed_layout
+    g = lay.geom
+    kk = ShortRangeKick(lay)
+    greens = {L: kk.green(L).cpu().numpy() for L in (2, 3, 4, 5, 6) if L <= 4}
+    lib.set_params(g)
+    lib.set_pp_mode(lib.PP_EXACT)
+    ix = ixyz3(g)
+    isort = np.arange(1, g.ns3 + 1, dtype=np.int32)
+    vp = lay.vp.copy()
+    f2 = np.zeros(1, np.float32)
+    vm = np.zeros(3, np.float32)
+    pt = np.zeros(g.ns3)
+    tt = np.zeros(g.ns3)
+    dummy = np.zeros(1, np.float32)
+    gp = [greens.get(L, dummy) for L in (2, 3, 4, 5, 6)]
+    a = lambda x: x.ctypes.data  # noqa: E731
+    mass, amid, dt = 1.0, 0.02, 1e-3
+    L = lib.lib()
+    L.c_pm3_force_kernel(a(isort), a(ires), a(ix), a(RATIO_SF), a(lay.rhoc), a(lay.idx_b_r),
+                         a(lay.xp), *[a(x) for x in gp], a(vp), mass, amid, dt, a(f2), a(vm), a(pt))
+    f2_pm3 = float(f2[0])
+    assert np.array_equal(vp, lay.vp)                 # PM3 leaves vp untouched
+    L.c_pp_force_kernel(a(isort), a(ires), a(ix), a(APM3), a(RATIO_SF), a(lay.rhoc),
+                        a(lay.idx_b_r), a(lay.xp), a(vp), mass, amid, dt, a(f2), a(vm), a(pt),
+                        a(tt))
+    o3 = _pm3_oracle_dv(oracle_lib, lay, ires, isort, greens, mass, amid, dt)
+    o = oracle_lib.pp_force(g, ires, ix, APM3, RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp, lay.vp,
+                            mass, amid, dt, dv64=o3["dv64"])
+    phys = lay.physical_mask()
+    err = rel_rms(vp[phys].astype(np.float64) - lay.vp[phys], o["vp"][phys].astype(np.float64) - lay.vp[phys])
+    print(f"drop-in PM3+PP Delta-v relative RMS {err:.2e}")
+    assert err < RTOL_PM3
+    assert abs(f2_pm3 - o3["f2max"]) < 3e-5 * o3["f2max"]
+    assert np.float32(f2[0]) == np.float32(o["f2max"])
+    assert np.allclose(vm, o["vmax"], rtol=1e-5)
+    lib.set_pp_mode(lib.PP_FAST)
