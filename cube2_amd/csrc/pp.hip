@@ -303,9 +303,9 @@ int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf) {
     }
     std::vector<int> iloc, owner;
     C2_TRY(location_maps(ctx().g, ires, ixyz3, iloc, owner));
-    for (int L = 1; L < 7; L++)
-        if (ratio_sf[L] <= 0 || ratio_sf[L] > 12) {
-            set_error("ratio_sf[%d]=%d outside 1..12", L, ratio_sf[L]);
+    for (int L = 2; L <= 6; L++)  // the levels ires can select (kick.f90:115-127)
+        if (ratio_sf[L - 1] <= 0 || ratio_sf[L - 1] > 12) {
+            set_error("ratio_sf[%d]=%d outside 1..12", L - 1, ratio_sf[L - 1]);
             return CUBE2_ERR_ARG;
         }
     return CUBE2_OK;
@@ -329,9 +329,9 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
     }
     std::vector<int> iloc, owner;
     C2_TRY(location_maps(g, ires, ixyz3, iloc, owner));
-    for (int L = 1; L < 7; L++)
-        if (ratio_sf[L] <= 0 || ratio_sf[L] > 12) {
-            set_error("ratio_sf[%d]=%d outside 1..12", L, ratio_sf[L]);
+    for (int L = 2; L <= 6; L++)  // the levels ires can select (kick.f90:115-127)
+        if (ratio_sf[L - 1] <= 0 || ratio_sf[L - 1] > 12) {
+            set_error("ratio_sf[%d]=%d outside 1..12", L - 1, ratio_sf[L - 1]);
             return CUBE2_ERR_ARG;
         }
     if (C.dv_state == 1 && C.dv_n_used != n_used) {

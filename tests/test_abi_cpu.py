@@ -105,3 +105,24 @@ def test_select_levels_matches_oracle(oracle_lib):
     assert np.array_equal(ires, o_ires)
     assert np.array_equal(isort, o_isort)
     assert np.array_equal(napm, o_napm)
+
+
+def test_pp_accepts_reference_ratio_table_without_gpu():
+    # ratio_sf carries 16 for the unused level 7 (parameters.f90:22); valid
+    # metadata passes validation and only then needs the device.
+    import torch
+    from cube2_amd import _lib
+    from cube2_amd.geometry import APM3, RATIO_SF, C0, ixyz3
+    if torch.cuda.is_available():
+        pytest.skip("device present: the call would run")
+    _lib.set_params(C0)
+    ires = np.full(C0.ns3, 2, np.int32)
+    ix = ixyz3(C0)
+    f2 = np.zeros(1, np.float32)
+    vm = np.zeros(3, np.float32)
+    pt = np.zeros(C0.ns3)
+    rc = _lib.lib().cube2_pp_force_dev(ires.ctypes.data, ix.ctypes.data, APM3.ctypes.data,
+                                      RATIO_SF.ctypes.data, None, None, None, None, 0, 1.0, 0.02,
+                                      1e-3, f2.ctypes.data, vm.ctypes.data, pt.ctypes.data, None,
+                                      None)
+    assert rc == _lib.ERR_HIP, _lib.lib().cube2_last_error()

@@ -80,8 +80,8 @@ def c0_layout():
     from cube2_amd.geometry import C0
     from cube2_amd.layout import build_layout
     from cube2_amd.synth import config0
-    pos, vel = config0(seed=0)
-    return build_layout(pos, vel, C0)
+    pos, _ = config0(seed=0)
+    return build_layout(pos, None, C0)     # zero velocities: Delta-v reads back exactly
 
 
 def test_pm3_stages_match_oracle(lib, oracle_lib, c0_layout):
@@ -146,7 +146,7 @@ def mixed_layout():
     rng = np.random.default_rng(23)
     pos = np.concatenate([rng.uniform(0, 64, (20000, 3)),
                           np.array([31.5, 30.0, 33.0]) + rng.normal(0, 0.7, (3000, 3))]) % 64
-    lay = build_layout(pos, rng.normal(0, 1, pos.shape).astype(np.float32), g)
+    lay = build_layout(pos, None, g)
     ires = rng.integers(2, 5, g.ns3).astype(np.int32)          # levels 2..4 (n = 48, 96, 144)
     return lay, ires
 
