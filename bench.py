@@ -1,0 +1,212 @@
+"""Short-range kick benchmark on one MI355X.
+
+Workload: 256^3 particles in a 300 Mpc/h box (60% Zel'dovich lattice at
+z=49, 40% in 2000 Plummer spheres), fine mesh 512^3 as 8^3 subtiles of 64^3
+fine cells (nnt=4, nns=2, 6-coarse-cell tile buffer), synthetic and seeded.
+One step = one kick: level selection -> PM3 over all 512 subtiles -> PP over
+all 64 parent tiles with the velocity update, inputs resident in HBM.
+
+Prints one JSON line: kick ms/step (value, lower is better) plus PP pair
+interactions per second, the roofline of the dominant kernel from HIP events
+on the library stream, and a CPU baseline measured on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (packed)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pp_flops(p_cand: int, p_acc: int) -> float:
+    """Algorithmic PP work: 8 flops per examined candidate (3 sub, 1 mul,
+    2 FMA), 36 per accepted pair (rsqrt, r, 2x13 for F_ra, sub, scale, 3 FMA)."""
+    return 8.0 * p_cand + 36.0 * p_acc
+
+
+def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s: float):
+    """Oracle (C/OpenMP restatement, same two-level structure as the
+    reference) on a bounded sample: PP over the lightest parent tiles until
+    ~budget_s, PM3 over a few level-2 subtiles; each extrapolated to the full
+    kick by its algorithmic work (PP flops, PM3 mesh points)."""
+    import oracle
+    from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
+    g = lay.geom
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    nteam = 4
+    nnest = max(1, ncpu // nteam)
+    # PP: lightest tiles first (cost proxy: sum of coarse counts squared)
+    proxy = (lay.rhoc.reshape(g.ntile, -1).astype(np.float64) ** 2).sum(1)
+    order = np.argsort(proxy, kind="stable")
+    t_pp, f_pp, tiles, px = 0.0, 0.0, [], 0.0
+    for t in order:
+        if tiles and t_pp / px * proxy[t] > 2 * (budget_s - t_pp):
+            break  # this tile alone would overrun the sample budget
+        t0 = time.perf_counter()
+        o = oracle.pp_force(g, ires, ixyz3(g), APM3, RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
+                            lay.vp, mass, amid, dt, tiles=(int(t), 1), nteam=1,
+                            nnest=ncpu)
+        t_pp += time.perf_counter() - t0
+        f_pp += pp_flops(o["p_cand"], o["p_acc"])
+        px += max(proxy[t], 1.0)
+        tiles.append(int(t))
+        if t_pp >= budget_s * 0.75:
+            break
+    pp_rate = f_pp / t_pp
+    pp_ms = pp_flops(*p_full) / pp_rate * 1e3
+    # PM3: level-2 subtiles (n = 80), timed, extrapolated by mesh points
+    l2 = [i for i in range(g.ns3) if ires[i] == 2][:4]
+    sub_isort = np.array([i + 1 for i in l2] + [i + 1 for i in range(g.ns3) if i not in l2],
+                         np.int32)
+    t0 = time.perf_counter()
+    done = 0
+    for it in l2:
+        oracle.pm3_job(g, it, ires, ixyz3(g), RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
+                       greens_host[2], mass, want_rho=False, want_phi=False, want_force=True)
+        done += 1
+        if time.perf_counter() - t0 > budget_s * 0.25:
+            break
+    t_pm3 = time.perf_counter() - t0
+    pts_done = done * g.nft(2) ** 3
+    pts_all = sum(g.nft(int(L)) ** 3 for L in ires)
+    pm3_ms = t_pm3 / pts_done * pts_all * 1e3
+    del sub_isort
+    return dict(value=round(pp_ms + pm3_ms, 1), unit="ms/step", cores=ncpu, kind="port",
+                sample=(f"oracle PP on {len(tiles)} of {g.ntile} parent tiles "
+                        f"({t_pp:.1f} s, {pp_rate / 1e9:.2f} GFLOP/s algorithmic, extrapolated "
+                        f"by PP flops: {pp_ms:.0f} ms) + oracle PM3 on {done} level-2 subtiles "
+                        f"({t_pm3:.1f} s, extrapolated by mesh points: {pm3_ms:.0f} ms); "
+                        f"OpenMP threads {ncpu}"))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pp-mode", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this path runs on a single MI355X; "
+              "multi-GPU sharding is by parent tile and is not part of this build.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from cube2_amd import _lib
+    from cube2_amd.geometry import C1
+    from cube2_amd.kick import ShortRangeKick
+    from cube2_amd.layout import build_layout
+    from cube2_amd.synth import config1
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    t0 = time.perf_counter()
+    pos, vel = config1(seed=args.seed)
+    lay = build_layout(pos, vel, C1)
+    del pos, vel
+    log(f"inputs: {lay.n_particles} particles, {lay.n_used} slots ({time.perf_counter() - t0:.1f} s)")
+    _lib.set_pp_mode(_lib.PP_EXACT if args.pp_mode == "exact" else _lib.PP_FAST)
+    k = ShortRangeKick(lay)
+    levels = sorted(set(int(x) for x in k.ires))
+    for L in levels:
+        k.green(L)
+    torch.cuda.synchronize()
+    log(f"levels {dict(zip(*np.unique(k.ires, return_counts=True)))}; Green tables built")
+    mass, amid, dt = 1.0, 0.02, 1e-3
+
+    for w in range(args.warmup):
+        t1 = time.perf_counter()
+        res = k.kick(mass, amid, dt)
+        log(f"warmup {w}: {1e3 * (time.perf_counter() - t1):.1f} ms "
+            f"(P_cand {res.p_cand:.3e}, P_acc {res.p_acc:.3e})")
+
+    _lib.profile(True)
+    _lib.profile_reset()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for s in range(args.steps):
+        res = k.kick(mass, amid, dt)
+        log(f"step {s} done")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    stats = _lib.profile_stats()
+    _lib.profile(False)
+    ms_step = 1e3 * wall / args.steps
+
+    pp_name = "pp_pairs_exact" if args.pp_mode == "exact" else "pp_pairs_fast"
+    pp_ms = stats[pp_name]["ms"] / args.steps
+    flops = pp_flops(res.p_cand, res.p_acc)
+    pair_rate = res.p_acc / (pp_ms * 1e-3)
+    # dominant kernel by GPU time
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    name, st = dom
+    per_launch_ms = st["ms"] / st["launches"]
+    if name.startswith("pp_pairs"):
+        achieved = flops / (pp_ms * 1e-3) / 1e12
+        roof = dict(kernel=name, bound="valu", achieved=round(achieved, 3),
+                    peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None,
+                    avg_launch_ms=round(per_launch_ms, 3))
+    else:
+        achieved = st["bytes"] / (st["ms"] * 1e-3) / 1e9
+        roof = dict(kernel=name, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_ms=round(per_launch_ms, 3))
+    stages = {n: round(v["ms"] / args.steps, 3) for n, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
+    log(f"per-step stage ms: {stages}")
+
+    cpu = None
+    if not args.no_cpu:
+        greens_host = {L: k.green(L).cpu().numpy() for L in levels if L == 2}
+        log("cpu baseline ...")
+        cpu = cpu_baseline(lay, k.ires, k.isort, greens_host, (res.p_cand, res.p_acc), mass, amid,
+                           dt, args.cpu_budget)
+
+    out = {
+        "metric": "kick ms/step + PP pair-int/s @256^3",
+        "value": round(ms_step, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 Green's tables, velocity accumulation, max reductions)",
+        "data": "synthetic (seeded Zel'dovich lattice + Plummer spheres)",
+        "config": {"workload": "C1: 256^3 particles, 300 Mpc/h, fine mesh 512^3 as 8^3 subtiles, "
+                               "ng=256 nnt=4 nns=2 ngb=24, PM3+PP kick",
+                   "pp_mode": args.pp_mode, "n_used_slots": int(lay.n_used),
+                   "levels": {int(a): int(b) for a, b in zip(*np.unique(k.ires, return_counts=True))}},
+        "pp_pair_interactions_per_s": float(f"{pair_rate:.4g}"),
+        "pp_ms_per_step": round(pp_ms, 3),
+        "p_cand": int(res.p_cand),
+        "p_acc": int(res.p_acc),
+        "stage_ms_per_step": stages,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
