@@ -16,7 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libcube2_short.so"
 _LIB = None
 
 OK, ERR_ARG, ERR_GEOMETRY, ERR_CAPACITY, ERR_HIP, ERR_STATE = range(6)
-PP_EXACT, PP_FAST = 0, 1
+PP_EXACT, PP_FAST, PP_REF = 0, 1, 2
 
 
 class Cube2Error(RuntimeError):
@@ -52,6 +52,7 @@ SIGNATURES = {
     "cube2_pp_debug_celllist": (C.c_int, [C.c_int] + [vp_] * 6),
     "cube2_pm3_debug_job": (C.c_int, [C.c_int] + [vp_] * 6 + [C.c_int64, vp_, C.c_float] + [vp_] * 4),
     "cube2_fft3d_dev": (C.c_int, [vp_, C.c_int, C.c_int, C.c_int]),
+    "cube2_selftest_div": (C.c_int, [C.c_uint64, C.c_int, vp_]),
     "cube2_profile_enable": (C.c_int, [C.c_int]),
     "cube2_profile_reset": (C.c_int, []),
     "cube2_profile_get": (C.c_int, [C.c_char_p, vp_, vp_, vp_, vp_]),

@@ -29,6 +29,7 @@ int fft3d(float* d, int n, int nb, int dir);
 void pm3_release();
 int green_cache_get(const float* host, size_t count, const float** dev);
 void green_cache_release();
+int selftest_div(uint64_t seed, int n, unsigned long long* mismatches);
 }  // namespace cube2
 
 using namespace cube2;
@@ -261,6 +262,15 @@ int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort
     if (napm)
         for (int q = 0; q < 7; q++) napm[q] = nap[q];
     return CUBE2_OK;
+}
+
+int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    if (n < 0 || !mismatches) {
+        set_error("bad self-test arguments");
+        return CUBE2_ERR_ARG;
+    }
+    return selftest_div(seed, n, mismatches);
 }
 
 int cube2_pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys_sorted,

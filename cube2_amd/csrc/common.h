@@ -69,7 +69,7 @@ struct Context {
     Geom g{};
     int device = -1;
     hipStream_t stream = nullptr;
-    int pp_mode = CUBE2_PP_FAST;
+    int pp_mode = CUBE2_PP_REF;
     // velocity accumulator (dv64, FP64, 3 per slot) shared by PM3 and PP
     DevBuf dv64;
     int64_t dv_n_used = 0;

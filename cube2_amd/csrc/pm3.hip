@@ -22,6 +22,7 @@
 #include "common.h"
 #include "fft.h"
 #include "profile.h"
+#include "reduce.h"
 
 namespace cube2 {
 
@@ -128,22 +129,24 @@ __global__ __launch_bounds__(256) void k_pm3_deposit(MeshArgs A, float* __restri
 }
 
 // Force planes over Fortran indices [0, nfp+1]^3 (kick.f90:545-568) and the
-// interior max |F|^2 (kick.f90:494).
+// interior max |F|^2 (kick.f90:494).  Block = 64 x-points x 4 rows; grid =
+// (x tiles * y tiles, z, job): 32-bit index math, coalesced along x.
 __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi, size_t rs,
-                                                  float* __restrict__ force, int nb, int n,
-                                                  int nfp, int nfb, float n3,
+                                                  float* __restrict__ force, int n, int nfp,
+                                                  int nfb, float n3,
                                                   unsigned* __restrict__ f2bits) {
     const int nf = nfp + 2;
-    const size_t fs = (size_t)nf * nf * nf;
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int xt = (nf + 63) >> 6;
+    const int i0 = (int)(blockIdx.x % xt) * 64 + (threadIdx.x & 63);
+    const int j0 = (int)(blockIdx.x / xt) * 4 + (threadIdx.x >> 6);
+    const int k0 = blockIdx.y;
+    const int jb = blockIdx.z;
     float f2 = 0.0f;
-    if (q < (int64_t)nb * (int64_t)fs) {
-        const int jb = (int)(q / (int64_t)fs);
-        const size_t u = (size_t)(q - (int64_t)jb * (int64_t)fs);
-        const int i0 = (int)(u % nf), j0 = (int)((u / nf) % nf), k0 = (int)(u / ((size_t)nf * nf));
+    if (i0 < nf && j0 < nf) {
         const float* P = phi + (size_t)jb * rs;
         const int x = i0 + nfb - 1, y = j0 + nfb - 1, z = k0 + nfb - 1;
-        const size_t sr = n + 2, sp = (size_t)n * (n + 2);
+        const int sr = n + 2;
+        const size_t sp = (size_t)n * (n + 2);
         const size_t c = (size_t)z * sp + (size_t)y * sr + x;
         // real(4) weights of alpha = 4/3 (parameters.f90:49-50)
         const float al = 4.0f / 3.0f;
@@ -155,6 +158,8 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
             fy = P[c - 2 * sr] / n3 * w0 + P[c - sr] / n3 * w1 + P[c + sr] / n3 * w2 + P[c + 2 * sr] / n3 * w3;
             fz = P[c - 2 * sp] / n3 * w0 + P[c - sp] / n3 * w1 + P[c + sp] / n3 * w2 + P[c + 2 * sp] / n3 * w3;
         }
+        const size_t fs = (size_t)nf * nf * nf;
+        const size_t u = ((size_t)k0 * nf + j0) * nf + i0;
         float* F = force + (size_t)jb * 3 * fs;
         F[u] = fx;
         F[fs + u] = fy;
@@ -162,9 +167,8 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
         if (i0 >= 1 && i0 <= nfp && j0 >= 1 && j0 <= nfp && k0 >= 1 && k0 <= nfp)
             f2 = fx * fx + fy * fy + fz * fz;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) f2 = fmaxf(f2, __shfl_xor(f2, o, 64));
-    if ((threadIdx.x & 63) == 0 && f2 > 0.0f) atomicMax(f2bits, __float_as_uint(f2));
+    const float fv[1] = {f2};
+    block_max_to<1>(f2bits, fv);
 }
 
 // Wavefront per (job, physical coarse cell): TSC gather and FP64 Delta-v.
@@ -258,17 +262,8 @@ __global__ __launch_bounds__(256) void k_velocity_apply(const int* __restrict__ 
                 v2 = fmaxf(v2, fabsf(nv[2]));
             }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        v0 = fmaxf(v0, __shfl_xor(v0, o, 64));
-        v1 = fmaxf(v1, __shfl_xor(v1, o, 64));
-        v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
-    }
-    if (lane == 0) {
-        if (v0 > 0) atomicMax(&vbits[0], __float_as_uint(v0));
-        if (v1 > 0) atomicMax(&vbits[1], __float_as_uint(v1));
-        if (v2 > 0) atomicMax(&vbits[2], __float_as_uint(v2));
-    }
+    const float vv[3] = {v0, v1, v2};
+    block_max_to<3>(vbits, vv);
 }
 
 // Green_3D (Green/Green.f90:1-80) in FP64; kshift = 0, n1 = n/2+1.  The
@@ -439,8 +434,9 @@ static int run_batch(const MeshArgs& base, const std::vector<PM3Job>& jobs, size
     }
     {
         Stage st("pm3_grad", mesh_bytes + (double)nb * 3 * fs * 4);
-        k_pm3_grad<<<ceil_div(nb * fs, 256), 256, 0, s>>>(rho, rs, force, (int)nb, n, nfp, base.nfb,
-                                                          n3, f2bits);
+        const int nfq = nfp + 2;
+        dim3 grid((unsigned)(((nfq + 63) / 64) * ((nfq + 3) / 4)), (unsigned)nfq, (unsigned)nb);
+        k_pm3_grad<<<grid, 256, 0, s>>>(rho, rs, force, n, nfp, base.nfb, n3, f2bits);
         C2_HIP(hipGetLastError());
     }
     if (dbg && dbg->force) {

@@ -32,8 +32,11 @@
 
 namespace cube2 {
 
+int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, bool ref, hipStream_t s);
+
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
+    DevBuf chunk_flags, chunks;
     std::vector<PPTile> h_tiles;
     int64_t c_tot = 0;
     int n_tot = 0;
@@ -250,6 +253,23 @@ __global__ void k_pp_ptotal(const int* __restrict__ rhoc, int nt, int ncb, int n
     atomicAdd(&ploc[loc], (unsigned long long)v);
 }
 
+// Work chunks of the pair kernel: <= 64 consecutive physical particles of
+// one PP cell.  flags[a] = 1 where a chunk starts.
+__global__ void k_chunk_flags(const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
+                              const int* __restrict__ cell_start, int n, int* __restrict__ flags) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    int f = 0;
+    if (pos4[a].w > 0.0f) f = ((a - cell_start[cellid[a]]) & 63) == 0;
+    flags[a] = f;
+}
+
+__global__ void k_chunk_emit(const int* __restrict__ scanned, int n, int* __restrict__ chunks) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    if (scanned[a + 1] != scanned[a]) chunks[scanned[a]] = a;
+}
+
 // First sorted index of every tile (for per-tile counts).
 __global__ void k_tile_first(const int* __restrict__ cell_start, const PPTile* __restrict__ tiles,
                              int ntile, int64_t c_tot, int* __restrict__ out) {
@@ -441,7 +461,28 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
     A.dv64 = dv;
     A.vp = d_vp;
     A.red = red;
-    if (n_tot > 0) C2_TRY(launch_pp_pairs(A, C.pp_mode, s));
+    if (n_tot > 0) {
+        if (C.pp_mode == CUBE2_PP_EXACT) {
+            C2_TRY(launch_pp_pairs(A, C.pp_mode, s));
+        } else {
+            C2_TRY(g_pp.chunk_flags.reserve(((size_t)n_tot + 1) * sizeof(int)));
+            C2_TRY(g_pp.chunks.reserve(((size_t)n_tot + 1) * sizeof(int)));
+            int* flags = g_pp.chunk_flags.as<int>();
+            int nchunks = 0;
+            {
+                Stage st("pp_chunks", (double)n_tot * 24);
+                k_chunk_flags<<<ceil_div(n_tot, 256), 256, 0, s>>>(A.pos4, A.cellid, A.cell_start,
+                                                                   n_tot, flags);
+                C2_HIP(hipGetLastError());
+                C2_TRY(exclusive_scan(flags, n_tot, g_pp.scan_tmp, s));
+                k_chunk_emit<<<ceil_div(n_tot, 256), 256, 0, s>>>(flags, n_tot, g_pp.chunks.as<int>());
+                C2_HIP(hipGetLastError());
+            }
+            C2_HIP(hipMemcpyAsync(&nchunks, flags + n_tot, sizeof(int), hipMemcpyDeviceToHost, s));
+            C2_HIP(hipStreamSynchronize(s));
+            C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), nchunks, C.pp_mode == CUBE2_PP_REF, s));
+        }
+    }
     // results
     PPRed h;
     std::vector<unsigned long long> hp(g.ns3);
@@ -508,7 +549,7 @@ int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t
 void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
-                      &p.slot, &p.cellid, &p.red, &p.tile_first})
+                      &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks})
         b->release();
     p.valid = false;
 }

@@ -12,6 +12,7 @@
 // f2max, vp = (float)((double)vp + dv64 + (double)(a*a_mid*dt/6)/pi), vmax.
 #include "pp_kernels.h"
 #include "profile.h"
+#include "reduce.h"
 
 namespace cube2 {
 
@@ -57,17 +58,6 @@ __device__ inline float fra_fast(float ep, float ir2, float c) {
     return ep > 1.0f ? hi : lo;
 }
 
-__device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // a *= m^2; f2; vp update in FP64 (pp_force_kernel.c:350-360).
 __device__ inline void pp_update(const PPArgs& A, uint32_t a, float ax, float ay, float az,
                                  float& f2, float& v0, float& v1, float& v2) {
@@ -94,20 +84,10 @@ __device__ inline void pp_update(const PPArgs& A, uint32_t a, float ax, float ay
 
 __device__ inline void pp_reduce(const PPArgs& A, float f2, float v0, float v1, float v2,
                                  unsigned long long pc, unsigned long long pa) {
-    f2 = wave_max(f2);
-    v0 = wave_max(v0);
-    v1 = wave_max(v1);
-    v2 = wave_max(v2);
-    pc = wave_sum(pc);
-    pa = wave_sum(pa);
-    if ((threadIdx.x & 63) == 0) {
-        if (f2 > 0) atomicMax(&A.red->f2max_bits, __float_as_uint(f2));
-        if (v0 > 0) atomicMax(&A.red->vmax_bits[0], __float_as_uint(v0));
-        if (v1 > 0) atomicMax(&A.red->vmax_bits[1], __float_as_uint(v1));
-        if (v2 > 0) atomicMax(&A.red->vmax_bits[2], __float_as_uint(v2));
-        if (pc) atomicAdd(&A.red->p_cand, pc);
-        if (pa) atomicAdd(&A.red->p_acc, pa);
-    }
+    const float mx[4] = {f2, v0, v1, v2};
+    block_max_to<4>(&A.red->f2max_bits, mx);   // f2max_bits, vmax_bits[3] are contiguous
+    const unsigned long long sm[2] = {pc, pa};
+    block_sum_to<2>(&A.red->p_cand, sm);        // p_cand, p_acc are contiguous
 }
 
 template <bool kExact>

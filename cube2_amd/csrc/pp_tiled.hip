@@ -1,0 +1,322 @@
+// PP pair kernel, work-compacted.
+//
+// One wavefront per chunk of <= 64 physical particles of one PP cell.  The
+// lanes are split into I i-slots x J j-slots (I the chunk size rounded up
+// to a power of two, I*J = 64), so a sparse cell still fills the wavefront
+// and a dense cell streams each j to all 64 i.  The j of the (2R+1)^2
+// neighbour rows are contiguous runs of the sorted particle stream.
+//
+// Phase 1 tests every candidate (dx, dy, dz, r^2 in the reference's
+// operation order).  Accepted pairs are appended to a per-wavefront LDS
+// queue (ballot + prefix count); whenever 64 are queued, phase 2 evaluates
+// 64 pair forces on a full wavefront and adds them into the i's LDS
+// accumulators.  Divergence is therefore paid on the cheap test, not on the
+// force law.
+//
+// kRef: the pair force is the reference C kernel's, operation for operation
+//   (pp_force_kernel.c:308-318, F_ra :45-74), with each correctly rounded
+//   float division a/b evaluated as RN_f(a * y), y an FP64 reciprocal of b
+//   accurate to 2^-52.  The quotient of two floats is either exactly
+//   representable or at least 2^-49 (relative) away from every rounding
+//   midpoint, so this equals IEEE a/b.  The pair forces are bit-identical to
+//   the reference; only the order of the per-i sums differs (<= ~4e-7
+//   relative RMS).
+// !kRef: FMA/rsqrt arithmetic with F_ra in Horner form.
+#include "pp_kernels.h"
+#include "profile.h"
+#include "reduce.h"
+
+namespace cube2 {
+
+constexpr int kQ = 128;  // queue entries per wavefront
+
+// FP64 reciprocal with two Newton steps from v_rcp_f64: relative error
+// below 2^-52 for normal inputs.
+__device__ inline double recip64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(e, y, y);
+    e = fma(-x, y, 1.0);
+    return fma(e, y, y);
+}
+
+__device__ inline float div_by(float a, double y) { return (float)((double)a * y); }
+
+struct FraK {
+    float a;       // softening diameter
+    double inv_a;  // 1/a in FP64
+    float c;       // (1/35)/(a*a), as basic_functions.f08:103 rounds it
+};
+
+// F_ra(r, a) of basic_functions.f08:95-109 with the reference's rounding;
+// inv_rr = 1/RN(r*r) in FP64.
+__device__ inline float fra_ref(float r, const FraK& K, double inv_rr) {
+#pragma clang fp contract(off)
+    if (K.a == 0.0f) return (float)inv_rr;
+    const float ep = div_by(2.0f * r, K.inv_a);
+    if (ep > 2.0f) return (float)inv_rr;
+    if (ep > 1.0f) {
+        const float e2 = ep * ep, e3 = e2 * ep, e4 = e3 * ep, e5 = e4 * ep, e6 = e5 * ep;
+        const float t12 = (float)(12.0 * recip64((double)e2));
+        return K.c * (t12 - 224.0f + 896.0f * ep - 840.0f * e2 + 224.0f * e3 + 70.0f * e4 -
+                      48.0f * e5 + 7.0f * e6);
+    }
+    if (ep > 0.0f) {
+        const float e3 = ep * ep * ep, e4 = e3 * ep, e5 = e4 * ep, e6 = e5 * ep;
+        return K.c * (224.0f * ep - 224.0f * e3 + 70.0f * e4 + 48.0f * e5 - 21.0f * e6);
+    }
+    return 0.0f;
+}
+
+__device__ inline float fra_fast2(float ep, float ir2, float c) {
+    if (ep > 2.0f) return ir2;
+    const float hi = fmaf(3.0f / 35.0f, ir2,
+                          c * fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, 7.0f, -48.0f),
+                                                                         70.0f), 224.0f), -840.0f),
+                                            896.0f), -224.0f));
+    const float e2 = ep * ep;
+    const float lo = c * ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+    return ep > 1.0f ? hi : lo;
+}
+
+template <bool kRef>
+__device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, float& fx, float& fy,
+                                  float& fz) {
+    if (kRef) {
+#pragma clang fp contract(off)
+        const float rm = sqrtf(e.w);
+        const double inv_rr = recip64((double)(rm * rm));
+        const float fpp = fra_ref(rm, Ka, inv_rr) - fra_ref(rm, Kr, inv_rr);
+        const double inv_rm = recip64((double)rm);
+        fx = div_by(fpp * e.x, inv_rm);
+        fy = div_by(fpp * e.y, inv_rm);
+        fz = div_by(fpp * e.z, inv_rm);
+    } else {
+        const float ir = __frsqrt_rn(e.w);
+        const float ir2 = ir * ir;
+        const float r = e.w * ir;
+        float fa = ir2;
+        if (r <= Ka.a) fa = fra_fast2(r * (float)(2.0 * Ka.inv_a), ir2, Ka.c);
+        const float f = (fa - fra_fast2(r * (float)(2.0 * Kr.inv_a), ir2, Kr.c)) * ir;
+        fx = f * e.x;
+        fy = f * e.y;
+        fz = f * e.z;
+    }
+}
+
+// Orders LDS accesses of the lanes of one wavefront (no block barrier: the
+// wavefronts of a block run independent trip counts).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct WaveLDS {
+    float4 q[kQ];
+    unsigned char qi[kQ];
+    float acc[64][3];
+};
+
+template <bool kRef>
+__device__ inline void flush(WaveLDS& L, int lane, int begin, int count, const FraK& Ka,
+                             const FraK& Kr) {
+    if (lane < count) {
+        const int s = (begin + lane) & (kQ - 1);
+        const float4 e = L.q[s];
+        const int ii = L.qi[s];
+        float fx, fy, fz;
+        pair_force<kRef>(e, Ka, Kr, fx, fy, fz);
+        atomicAdd(&L.acc[ii][0], fx);
+        atomicAdd(&L.acc[ii][1], fy);
+        atomicAdd(&L.acc[ii][2], fz);
+    }
+    wave_lds_sync();
+}
+
+__device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32_t c) {
+    int lo = 0, hi = ntile - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T[mid].cell_off <= (long long)c) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool kRef>
+__global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
+                                                  int nchunks) {
+    __shared__ WaveLDS lds[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLDS& L = lds[wave];
+    const int ck = blockIdx.x * 4 + wave;
+    float f2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f;
+    unsigned long long pc = 0, pa = 0;
+    if (ck < nchunks) {
+        const int a0 = chunks[ck];
+        const uint32_t cid = A.cellid[a0];
+        const int cend = A.cell_start[cid + 1];
+        const int ni = min(64, cend - a0);
+        int lgI = 0;
+        while ((1 << lgI) < ni) lgI++;
+        const int I = 1 << lgI, J = 64 >> lgI;
+        const int islot = lane & (I - 1), jslot = lane >> lgI;
+        const bool vi = islot < ni;
+        const float4 p = A.pos4[a0 + (vi ? islot : 0)];
+        const PPTile T = A.tiles[find_tile2(A.tiles, A.ntile, cid)];
+        const long long key = (long long)cid - T.cell_off;
+        const int S = T.S;
+        const int x = (int)(key % S), y = (int)((key / S) % S), z = (int)(key / ((long long)S * S));
+        const float range = p.w;  // one subtile per PP cell: same range for the chunk
+        int R;
+        float pp2;
+        {
+#pragma clang fp contract(off)
+            R = (int)ceilf(range * (float)T.rcp / (float)A.ratio_cs);
+            pp2 = range * range;
+        }
+        FraK Ka, Kr;
+        Ka.a = A.app;
+        Ka.inv_a = A.app > 0.0f ? 1.0 / (double)A.app : 0.0;
+        Kr.a = range;
+        Kr.inv_a = 1.0 / (double)range;
+        {
+#pragma clang fp contract(off)
+            Ka.c = 1.0f / 35.0f / (A.app * A.app);
+            Kr.c = 1.0f / 35.0f / (range * range);
+        }
+        for (int q = lane; q < 64 * 3; q += 64) (&L.acc[0][0])[q] = 0.0f;
+        wave_lds_sync();
+        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        int qhead = 0, qtail = 0;
+        const int xlo = max(x - R, 0), xhi = min(x + R, S - 1);
+        for (int dz = -R; dz <= R; dz++) {
+            const int zz = z + dz;
+            if (zz < 0 || zz >= S) continue;
+            for (int dy = -R; dy <= R; dy++) {
+                const int yy = y + dy;
+                if (yy < 0 || yy >= S) continue;
+                const long long row = T.cell_off + ((long long)zz * S + yy) * S;
+                const int j0 = A.cell_start[row + xlo], j1 = A.cell_start[row + xhi + 1];
+                if (vi) pc += (unsigned long long)((j1 - j0 - jslot + J - 1) / J);
+                for (int base = j0; base < j1; base += J) {
+                    const int j = base + jslot;
+                    bool ok = false;
+                    float4 e;
+                    if (vi && j < j1) {
+#pragma clang fp contract(off)
+                        const float4 qj = A.pos4[j];
+                        e.x = qj.x - p.x;
+                        e.y = qj.y - p.y;
+                        e.z = qj.z - p.z;
+                        e.w = e.x * e.x + e.y * e.y + e.z * e.z;
+                        ok = e.w > 0.0f && e.w < pp2;
+                    }
+                    const uint64_t m = __ballot(ok);
+                    if (m) {
+                        if (ok) {
+                            const int s = (qtail + __popcll(m & lt)) & (kQ - 1);
+                            L.q[s] = e;
+                            L.qi[s] = (unsigned char)islot;
+                        }
+                        qtail += __popcll(m);
+                        wave_lds_sync();
+                        if (qtail - qhead >= 64) {
+                            flush<kRef>(L, lane, qhead, 64, Ka, Kr);
+                            qhead += 64;
+                        }
+                    }
+                }
+            }
+        }
+        if (qtail > qhead) flush<kRef>(L, lane, qhead, qtail - qhead, Ka, Kr);
+        pa = (lane == 0) ? (unsigned long long)qtail : 0ull;
+        if (vi && jslot == 0) {
+#pragma clang fp contract(off)
+            pc -= 1;  // the particle itself
+            const int a = a0 + islot;
+            const float ax = L.acc[islot][0], ay = L.acc[islot][1], az = L.acc[islot][2];
+            const float a0m = ax * A.mass * A.mass, a1m = ay * A.mass * A.mass,
+                        a2m = az * A.mass * A.mass;
+            f2 = a0m * a0m + a1m * a1m + a2m * a2m;
+            const size_t ip = (size_t)A.slot[a] * 3;
+            const float acc[3] = {a0m, a1m, a2m};
+            float nv[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double term = (double)(acc[c] * A.a_mid * A.dt / 6.0f) / 3.14159265358979323846;
+                double v = (double)A.vp[ip + c];
+                if (A.dv64) v += A.dv64[ip + c];
+                nv[c] = (float)(v + term);
+                A.vp[ip + c] = nv[c];
+            }
+            v0 = fabsf(nv[0]);
+            v1 = fabsf(nv[1]);
+            v2 = fabsf(nv[2]);
+        }
+    }
+    const float mx[4] = {f2, v0, v1, v2};
+    block_max_to<4>(&A.red->f2max_bits, mx);
+    const unsigned long long sm[2] = {pc, pa};
+    block_sum_to<2>(&A.red->p_cand, sm);
+}
+
+int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, bool ref, hipStream_t s) {
+    Stage st(ref ? "pp_pairs_ref" : "pp_pairs_fastq");
+    const unsigned blocks = ceil_div((size_t)nchunks, 4);
+    if (blocks) {
+        if (ref) k_pp_tiled<true><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+        else k_pp_tiled<false><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+        C2_HIP(hipGetLastError());
+    }
+    return CUBE2_OK;
+}
+
+// --- arithmetic self-test: the reciprocal-based quotient against IEEE a/b.
+__global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint64_t s = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(t + 1);
+    auto nxt = [&]() {
+        s ^= s >> 12;
+        s ^= s << 25;
+        s ^= s >> 27;
+        return (uint32_t)((s * 0x2545F4914F6CDD1Dull) >> 32);
+    };
+    // normal floats with exponents in [-40, 40]
+    const uint32_t ma = nxt() & 0x7FFFFF, mb = nxt() & 0x7FFFFF;
+    const int ea = (int)(nxt() % 81) - 40 + 127, eb = (int)(nxt() % 81) - 40 + 127;
+    const float a = __uint_as_float(((uint32_t)ea << 23) | ma);
+    const float b = __uint_as_float(((uint32_t)eb << 23) | mb);
+    float ieee;
+    {
+#pragma clang fp contract(off)
+        ieee = a / b;
+    }
+    const float mine = div_by(a, recip64((double)b));
+    const float twelve = (float)(12.0 * recip64((double)b));
+    float ieee12;
+    {
+#pragma clang fp contract(off)
+        ieee12 = 12.0f / b;
+    }
+    if (__float_as_uint(ieee) != __float_as_uint(mine) || __float_as_uint(ieee12) != __float_as_uint(twelve))
+        atomicAdd(bad, 1ull);
+}
+
+int selftest_div(uint64_t seed, int n, unsigned long long* mismatches) {
+    C2_TRY(ensure_init());
+    hipStream_t s = ctx().stream;
+    unsigned long long* d = nullptr;
+    C2_HIP(hipMalloc(&d, sizeof(unsigned long long)));
+    C2_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+    k_selftest_div<<<ceil_div((size_t)n, 256), 256, 0, s>>>(seed, n, d);
+    C2_HIP(hipGetLastError());
+    C2_HIP(hipMemcpyAsync(mismatches, d, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipStreamSynchronize(s));
+    C2_HIP(hipFree(d));
+    return CUBE2_OK;
+}
+
+}  // namespace cube2

@@ -38,7 +38,7 @@ enum {
     CUBE2_ERR_STATE = 5     /* call order (velocity accumulator) violated */
 };
 
-enum { CUBE2_PP_EXACT = 0, CUBE2_PP_FAST = 1 };
+enum { CUBE2_PP_EXACT = 0, CUBE2_PP_FAST = 1, CUBE2_PP_REF = 2 };
 
 /* ---------------------------------------------------------------------
  * Drop-in entry points (host pointers; the Fortran caller owns every array)
@@ -89,9 +89,14 @@ typedef struct {
 int cube2_set_params(const cube2_params_t* p);
 int cube2_get_params(cube2_params_t* p);
 
-/* PP arithmetic: CUBE2_PP_EXACT evaluates every pair with the reference C
- * kernel's operation order and IEEE sqrt/divide (bit-identical Delta-v);
- * CUBE2_PP_FAST (default) uses the fast kernel (relative RMS <= 1e-5). */
+/* PP pair kernel:
+ *   CUBE2_PP_REF (default): the reference C kernel's pair arithmetic
+ *     operation for operation (bit-identical pair forces), work-compacted;
+ *     per-particle sums in another order (relative RMS ~4e-7 vs reference);
+ *   CUBE2_PP_EXACT: one lane per particle, the reference's summation order
+ *     too: Delta-v bit-identical to pp_force_kernel.c (slow, for checking);
+ *   CUBE2_PP_FAST: FMA/rsqrt arithmetic, work-compacted (its own rounding;
+ *     accuracy stated against an FP64 direct sum). */
 int cube2_set_pp_mode(int mode);
 
 /* ---------------------------------------------------------------------
@@ -145,6 +150,10 @@ int cube2_pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ra
 /* Batched 3-D FFT of nb real cubes of size n (in place, padded x rows of
  * n+2 floats, FFTW/cuFFT in-place layout): dir=-1 R2C, +1 C2R, unnormalised. */
 int cube2_fft3d_dev(float* d_data, int n, int nb, int dir);
+
+/* Checks the FP64-reciprocal quotient used by CUBE2_PP_REF against IEEE
+ * float division on n seeded random operand pairs; *mismatches = count. */
+int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches);
 
 int cube2_profile_enable(int on);
 int cube2_profile_reset(void);

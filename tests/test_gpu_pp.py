@@ -199,6 +199,37 @@ def test_fast_mode_accuracy_vs_fp64(lib, oracle_lib, c0_layout_still, mixed_layo
     assert abs(res.p_acc - o["p_acc"]) <= max(2, 1e-5 * o["p_acc"])
 
 
+@pytest.mark.parametrize("case", ["c0", "mixed"])
+def test_ref_mode_matches_reference_arithmetic(lib, oracle_lib, c0_layout_still, mixed_layout, case):
+    """Compacted kernel with the reference's pair arithmetic: identical pair
+    sets and counts, per-particle sums in another order (<= 1e-6)."""
+    from cube2_amd.layout import build_layout
+    if case == "c0":
+        lay, ires = c0_layout_still, None
+    else:
+        lay0, ires = mixed_layout
+        lay = build_layout(lay0.decoded_positions()[lay0.physical_mask()], None, lay0.geom)
+    k, res = _dev_pp(lib, lay, ires=ires, mode=lib.PP_REF)
+    o = _oracle_pp(oracle_lib, lay, k.ires)
+    phys = np.flatnonzero(lay.physical_mask())
+    err = _rel_rms(k.vp_host()[phys].astype(np.float64), o["vp"][phys].astype(np.float64))
+    print(f"{case}: reference-arithmetic compacted kernel vs reference: relative RMS {err:.2e}")
+    assert err <= 1e-6
+    assert (res.p_cand, res.p_acc) == (o["p_cand"], o["p_acc"])
+    assert abs(res.f2max_pp - o["f2max"]) <= 1e-5 * o["f2max"]
+    assert np.array_equal(res.ptotal_pp, o["ptotal"])
+
+
+def test_reciprocal_division_is_ieee(lib):
+    """The FP64-reciprocal quotient of the reference-arithmetic kernel equals
+    IEEE float division (a/b and 12/b) on 2^26 seeded random operands."""
+    import ctypes as C
+    bad = C.c_ulonglong(0)
+    for seed in (1, 2):
+        lib.check(lib.lib().cube2_selftest_div(seed, 1 << 25, C.addressof(bad)))
+        assert bad.value == 0
+
+
 def test_empty_and_single_particle(lib):
     from cube2_amd.geometry import Geometry
     from cube2_amd.layout import build_layout
