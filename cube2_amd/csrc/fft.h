@@ -10,6 +10,8 @@ struct FFTPlan {
     int n = 0;
     int nst = 0;
     int rad[16] = {0};
+    int pst[16] = {0};      // product of the radices before stage s
+    float invp[16] = {0};   // 1/pst[s], for division by a runtime constant
     float2* tw = nullptr;  // exp(-2 pi i t / n), t < n (device)
 };
 

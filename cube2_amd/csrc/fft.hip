@@ -1,8 +1,9 @@
 // Batched PM3 FFTs, hand-written for gfx950.
 //
 // One pass per axis; every pass stages a block of pencils in LDS and runs a
-// mixed-radix (4, 2, 3, 5) self-sorting Stockham transform there, so HBM sees
-// one read and one write of the mesh per pass.
+// mixed-radix (4, 2, 3, 5) self-sorting Stockham transform there, in place
+// with register staging, so HBM sees one read and one write of the mesh per
+// pass.
 //   x, forward : two real rows y, y+1 form one complex line z = a + i b;
 //                FFT(z) is split into the two half spectra A, B (R2C).
 //   y          : tiles of C consecutive kx columns x n rows.
@@ -92,190 +93,210 @@ struct DFT<5, S> {
     }
 };
 
-// One Stockham stage over L lines: element e of line l at l*ls + e*es.
-template <int R, int S>
-__device__ inline void stage(const float2* __restrict__ a, float2* __restrict__ b,
-                             const float2* __restrict__ tw, int n, int p, int L, int ls, int es,
-                             bool line_fast) {
-    const int m = n / R, ts = n / (p * R);
-    const int tasks = L * m;
-    for (int task = threadIdx.x; task < tasks; task += blockDim.x) {
-        int l, j;
-        if (line_fast) {
-            l = task % L;
-            j = task / L;
-        } else {
-            l = task / m;
-            j = task - l * m;
-        }
-        const int k = j % p;
-        float2 v[R];
-#pragma unroll
-        for (int q = 0; q < R; q++) v[q] = a[l * ls + (j + q * m) * es];
-        if (k) {
-#pragma unroll
-            for (int q = 1; q < R; q++) {
-                float2 w = tw[q * k * ts];
-                if (S > 0) w.y = -w.y;
-                v[q] = cmul(v[q], w);
-            }
-        }
-        DFT<R, S>::run(v);
-        const int base = (j - k) * R + k;
-#pragma unroll
-        for (int q = 0; q < R; q++) b[l * ls + (base + q * p) * es] = v[q];
-    }
+// j mod p for 0 <= j < 2^22 via a float reciprocal and one correction.
+__device__ inline int mod_p(int j, int p, float invp) {
+    int q = (int)((float)j * invp);
+    int r = j - q * p;
+    r += (r < 0) ? p : 0;
+    r -= (r >= p) ? p : 0;
+    return r;
 }
 
-// Transforms L lines held in LDS buffer a (b is the ping-pong partner);
-// returns the buffer holding the result.  Ends with a barrier.
-template <int S>
-__device__ float2* fft_lds(float2* a, float2* b, const float2* tw, const FFTPlan& P, int L, int ls,
-                           int es, bool line_fast) {
-    int p = 1;
-    for (int s = 0; s < P.nst; s++) {
-        const int r = P.rad[s];
-        if (r == 4) stage<4, S>(a, b, tw, P.n, p, L, ls, es, line_fast);
-        else if (r == 2) stage<2, S>(a, b, tw, P.n, p, L, ls, es, line_fast);
-        else if (r == 3) stage<3, S>(a, b, tw, P.n, p, L, ls, es, line_fast);
-        else stage<5, S>(a, b, tw, P.n, p, L, ls, es, line_fast);
-        __syncthreads();
-        float2* t = a;
-        a = b;
-        b = t;
-        p *= r;
+// One Stockham stage over L = 2^lgL lines held in one LDS buffer (element e
+// of line l at l*ls + e*es), in place: every thread first reads the inputs
+// of all its butterflies into registers, the block synchronises, then the
+// outputs are written.  Consecutive threads take consecutive lines.
+template <int R, int S>
+__device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ tw, int n, int p,
+                             float invp, int lgL, int ls, int es) {
+    constexpr int KMAX = (kFFTElems / R + kFFTThreads - 1) / kFFTThreads;
+    const int m = n / R, ts = n / (p * R);
+    const int L = 1 << lgL;
+    const int tasks = L * m;
+    float2 v[KMAX][R];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        const int task = threadIdx.x + k * kFFTThreads;
+        if (task < tasks) {
+            const int l = task & (L - 1), j = task >> lgL;
+#pragma unroll
+            for (int q = 0; q < R; q++) v[k][q] = a[l * ls + (j + q * m) * es];
+        }
     }
-    return a;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        const int task = threadIdx.x + k * kFFTThreads;
+        if (task < tasks) {
+            const int l = task & (L - 1), j = task >> lgL;
+            const int kk = mod_p(j, p, invp);
+            if (kk) {
+#pragma unroll
+                for (int q = 1; q < R; q++) {
+                    float2 w = tw[q * kk * ts];
+                    if (S > 0) w.y = -w.y;
+                    v[k][q] = cmul(v[k][q], w);
+                }
+            }
+            DFT<R, S>::run(v[k]);
+            const int base = (j - kk) * R + kk;
+#pragma unroll
+            for (int q = 0; q < R; q++) a[l * ls + (base + q * p) * es] = v[k][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Transforms 2^lgL lines held in LDS buffer a in place.  Ends with a barrier.
+template <int S>
+__device__ void fft_lds(float2* a, const float2* tw, const FFTPlan& P, int lgL, int ls, int es) {
+    for (int s = 0; s < P.nst; s++) {
+        const int r = P.rad[s], p = P.pst[s];
+        const float ip = P.invp[s];
+        if (r == 4) stage<4, S>(a, tw, P.n, p, ip, lgL, ls, es);
+        else if (r == 2) stage<2, S>(a, tw, P.n, p, ip, lgL, ls, es);
+        else if (r == 3) stage<3, S>(a, tw, P.n, p, ip, lgL, ls, es);
+        else stage<5, S>(a, tw, P.n, p, ip, lgL, ls, es);
+    }
 }
 
 __device__ inline void load_twiddles(float2* tw, const FFTPlan& P) {
     for (int t = threadIdx.x; t < P.n; t += blockDim.x) tw[t] = P.tw[t];
 }
 
+// Row pair g of the batch: first float of row y = 2*(g % half) of plane
+// z = (g / half) % n of job g / (n*half).
+__device__ inline float* row_pair(float* d, int g, int n, size_t rs) {
+    const int half = n >> 1;
+    const int per_job = n * half;
+    const int job = g / per_job;
+    const int rem = g - job * per_job;
+    const int z = rem / half, yp = rem - z * half;
+    return d + (size_t)job * rs + ((size_t)z * n + 2 * yp) * (n + 2);
+}
+
+// Lines are row pairs; 2^lgL of them per block, one wavefront per line at a
+// time along the row (coalesced).
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d, FFTPlan P,
-                                                           int nb, size_t rs, int PL, float scale) {
+                                                           int npairs, size_t rs, int lgL,
+                                                           float scale) {
     extern __shared__ float2 sm[];
-    const int n = P.n, nh = n / 2 + 1, half = n / 2;
+    const int n = P.n, nh = n / 2 + 1;
+    const int L = 1 << lgL;
     float2* tw = sm;
     float2* A = sm + n;
-    float2* B = A + (size_t)PL * n;
     load_twiddles(tw, P);
-    const int64_t npairs = (int64_t)nb * n * half;
-    const int64_t g0 = (int64_t)blockIdx.x * PL;
-    const int L = (int)min((int64_t)PL, npairs - g0);
-    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
-        const int l = e / n, t = e - l * n;
-        const int64_t g = g0 + l;
-        const int64_t job = g / ((int64_t)n * half), rem = g - job * n * half;
-        const int z = (int)(rem / half), yp = (int)(rem - (int64_t)z * half);
-        const float* ra = d + job * rs + ((size_t)z * n + 2 * yp) * (n + 2);
-        A[l * n + t] = make_float2(ra[t] * scale, ra[t + n + 2] * scale);
+    const int g0 = blockIdx.x * L;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int l = wave; l < L; l += kFFTThreads / 64) {
+        if (g0 + l >= npairs) break;
+        const float* ra = row_pair(d, g0 + l, n, rs);
+        for (int t = lane; t < n; t += 64)
+            A[l + t * L] = make_float2(ra[t] * scale, ra[t + n + 2] * scale);
     }
     __syncthreads();
-    const float2* R = fft_lds<-1>(A, B, tw, P, L, n, 1, false);
-    for (int e = threadIdx.x; e < L * nh; e += blockDim.x) {
-        const int l = e / nh, k = e - l * nh;
-        const int64_t g = g0 + l;
-        const int64_t job = g / ((int64_t)n * half), rem = g - job * n * half;
-        const int z = (int)(rem / half), yp = (int)(rem - (int64_t)z * half);
-        float2* ra = reinterpret_cast<float2*>(d + job * rs + ((size_t)z * n + 2 * yp) * (n + 2));
+    fft_lds<-1>(A, tw, P, lgL, 1, L);
+    for (int l = wave; l < L; l += kFFTThreads / 64) {
+        if (g0 + l >= npairs) break;
+        float2* ra = reinterpret_cast<float2*>(row_pair(d, g0 + l, n, rs));
         float2* rb = ra + nh;
-        const float2 zk = R[l * n + k], zn = R[l * n + (k == 0 ? 0 : n - k)];
-        ra[k] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-        rb[k] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+        for (int k = lane; k < nh; k += 64) {
+            const float2 zk = A[l + k * L], zn = A[l + (k == 0 ? 0 : n - k) * L];
+            ra[k] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+            rb[k] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+        }
     }
 }
 
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d, FFTPlan P,
-                                                           int nb, size_t rs, int PL) {
+                                                           int npairs, size_t rs, int lgL) {
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1, half = n / 2;
+    const int L = 1 << lgL;
     float2* tw = sm;
     float2* A = sm + n;
-    float2* B = A + (size_t)PL * n;
     load_twiddles(tw, P);
-    const int64_t npairs = (int64_t)nb * n * half;
-    const int64_t g0 = (int64_t)blockIdx.x * PL;
-    const int L = (int)min((int64_t)PL, npairs - g0);
-    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
-        const int l = e / n, k = e - l * n;
-        const int64_t g = g0 + l;
-        const int64_t job = g / ((int64_t)n * half), rem = g - job * n * half;
-        const int z = (int)(rem / half), yp = (int)(rem - (int64_t)z * half);
-        const float2* ra = reinterpret_cast<const float2*>(d + job * rs + ((size_t)z * n + 2 * yp) * (n + 2));
+    const int g0 = blockIdx.x * L;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int l = wave; l < L; l += kFFTThreads / 64) {
+        if (g0 + l >= npairs) break;
+        const float2* ra = reinterpret_cast<const float2*>(row_pair(d, g0 + l, n, rs));
         const float2* rb = ra + nh;
-        float2 a, b;
-        if (k <= half) {
-            a = ra[k];
-            b = rb[k];
-        } else {
-            a = ra[n - k];
-            b = rb[n - k];
-            a.y = -a.y;
-            b.y = -b.y;
+        for (int k = lane; k < n; k += 64) {
+            float2 a, b;
+            if (k <= half) {
+                a = ra[k];
+                b = rb[k];
+            } else {
+                a = ra[n - k];
+                b = rb[n - k];
+                a.y = -a.y;
+                b.y = -b.y;
+            }
+            if (k == 0 || k == half) {
+                a.y = 0.0f;
+                b.y = 0.0f;
+            }
+            A[l + k * L] = make_float2(a.x - b.y, a.y + b.x);
         }
-        if (k == 0 || k == half) {
-            a.y = 0.0f;
-            b.y = 0.0f;
-        }
-        A[l * n + k] = make_float2(a.x - b.y, a.y + b.x);
     }
     __syncthreads();
-    const float2* R = fft_lds<1>(A, B, tw, P, L, n, 1, false);
-    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
-        const int l = e / n, t = e - l * n;
-        const int64_t g = g0 + l;
-        const int64_t job = g / ((int64_t)n * half), rem = g - job * n * half;
-        const int z = (int)(rem / half), yp = (int)(rem - (int64_t)z * half);
-        float* ra = d + job * rs + ((size_t)z * n + 2 * yp) * (n + 2);
-        const float2 v = R[l * n + t];
-        ra[t] = v.x;
-        ra[t + n + 2] = v.y;
+    fft_lds<1>(A, tw, P, lgL, 1, L);
+    for (int l = wave; l < L; l += kFFTThreads / 64) {
+        if (g0 + l >= npairs) break;
+        float* ra = row_pair(d, g0 + l, n, rs);
+        for (int t = lane; t < n; t += 64) {
+            const float2 v = A[l + t * L];
+            ra[t] = v.x;
+            ra[t + n + 2] = v.y;
+        }
     }
 }
 
+// Tiles of C = 2^lgC consecutive kx columns x n rows along y (axis 1) or z
+// (axis 2); LDS element (row i, column c) at i*C + c.
 template <int MODE>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dreal, FFTPlan P,
-                                                          int nb, size_t rs, int axis, int C,
+                                                          size_t rs, int axis, int lgC,
                                                           const float* __restrict__ green) {
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1;
-    const int ntiles = (nh + C - 1) / C;
+    const int C = 1 << lgC;
+    const int ntiles = (nh + C - 1) >> lgC;
     float2* tw = sm;
     float2* A = sm + n;
-    float2* B = A + (size_t)C * n;
     load_twiddles(tw, P);
-    const int64_t b = blockIdx.x;
-    const int64_t job = b / ((int64_t)n * ntiles);
-    const int rem = (int)(b - job * n * ntiles);
-    const int o = rem / ntiles, tile = rem - o * ntiles;
+    const int tile = blockIdx.x % ntiles;
+    const int o = blockIdx.x / ntiles;
+    const int job = blockIdx.y;
     const int kx0 = tile * C, Cw = min(C, nh - kx0);
-    float2* d = reinterpret_cast<float2*>(dreal + job * rs);
+    float2* d = reinterpret_cast<float2*>(dreal + (size_t)job * rs);
     const size_t rowstride = axis == 1 ? (size_t)nh : (size_t)n * nh;
     const size_t obase = (axis == 1 ? (size_t)o * n * nh : (size_t)o * nh) + kx0;
-    for (int e = threadIdx.x; e < Cw * n; e += blockDim.x) {
-        const int i = e / Cw, c = e - i * Cw;
-        A[e] = d[obase + i * rowstride + c];
+    for (int e = threadIdx.x; e < n * C; e += kFFTThreads) {
+        const int i = e >> lgC, c = e & (C - 1);
+        if (c < Cw) A[e] = d[obase + i * rowstride + c];
     }
     __syncthreads();
-    float2* R;
     if (MODE == 1) {
-        R = fft_lds<1>(A, B, tw, P, Cw, 1, Cw, true);
+        fft_lds<1>(A, tw, P, lgC, 1, C);
     } else {
-        R = fft_lds<-1>(A, B, tw, P, Cw, 1, Cw, true);
+        fft_lds<-1>(A, tw, P, lgC, 1, C);
         if (MODE == 2) {
-            for (int e = threadIdx.x; e < Cw * n; e += blockDim.x) {
-                const int i = e / Cw, c = e - i * Cw;
-                const float gk = green[obase + i * rowstride + c];
-                R[e] = make_float2(R[e].x * gk, R[e].y * gk);
+            for (int e = threadIdx.x; e < n * C; e += kFFTThreads) {
+                const int i = e >> lgC, c = e & (C - 1);
+                if (c < Cw) {
+                    const float gk = green[obase + i * rowstride + c];
+                    A[e] = make_float2(A[e].x * gk, A[e].y * gk);
+                }
             }
             __syncthreads();
-            R = fft_lds<1>(R, R == A ? B : A, tw, P, Cw, 1, Cw, true);
+            fft_lds<1>(A, tw, P, lgC, 1, C);
         }
     }
-    for (int e = threadIdx.x; e < Cw * n; e += blockDim.x) {
-        const int i = e / Cw, c = e - i * Cw;
-        d[obase + i * rowstride + c] = R[e];
+    for (int e = threadIdx.x; e < n * C; e += kFFTThreads) {
+        const int i = e >> lgC, c = e & (C - 1);
+        if (c < Cw) d[obase + i * rowstride + c] = A[e];
     }
 }
 
@@ -286,7 +307,7 @@ static bool g_attr_set = false;
 
 static int set_lds_attributes() {
     if (g_attr_set) return CUBE2_OK;
-    const int bytes = (2 * kFFTElems + 1024) * (int)sizeof(float2);
+    const int bytes = (kFFTElems + 1024) * (int)sizeof(float2);
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
@@ -319,6 +340,11 @@ int fft_plan(int n, const FFTPlan** out) {
         set_error("FFT size %d has a prime factor other than 2, 3, 5", n);
         return CUBE2_ERR_CAPACITY;
     }
+    for (int st = 0, p = 1; st < P.nst; st++) {
+        P.pst[st] = p;
+        P.invp[st] = 1.0f / (float)p;
+        p *= P.rad[st];
+    }
     std::vector<float2> tw(n);
     for (int t = 0; t < n; t++) {
         const double ang = 2.0 * M_PI * (double)t / (double)n;
@@ -337,22 +363,38 @@ void fft_release() {
     g_plans.clear();
 }
 
+static int lg_floor(int v) {
+    int l = 0;
+    while ((2 << l) <= v) l++;
+    return l;
+}
+
 int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, float scale, hipStream_t s) {
     const int n = P.n;
-    const int PL = std::max(1, kFFTElems / n);
+    const int lgL = lg_floor(std::max(1, kFFTElems / n));
+    const int L = 1 << lgL;
     const int64_t npairs = (int64_t)nb * n * (n / 2);
-    const size_t lds = (size_t)(n + 2 * PL * n) * sizeof(float2);
-    k_fft_x_r2c<<<(unsigned)((npairs + PL - 1) / PL), kFFTThreads, lds, s>>>(d, P, nb, rs, PL, scale);
+    if (npairs >= (int64_t)1 << 31) {
+        set_error("FFT batch too large");
+        return CUBE2_ERR_CAPACITY;
+    }
+    const size_t lds = (size_t)(n + L * n) * sizeof(float2);
+    k_fft_x_r2c<<<(unsigned)((npairs + L - 1) / L), kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
 
 int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s) {
     const int n = P.n;
-    const int PL = std::max(1, kFFTElems / n);
+    const int lgL = lg_floor(std::max(1, kFFTElems / n));
+    const int L = 1 << lgL;
     const int64_t npairs = (int64_t)nb * n * (n / 2);
-    const size_t lds = (size_t)(n + 2 * PL * n) * sizeof(float2);
-    k_fft_x_c2r<<<(unsigned)((npairs + PL - 1) / PL), kFFTThreads, lds, s>>>(d, P, nb, rs, PL);
+    if (npairs >= (int64_t)1 << 31) {
+        set_error("FFT batch too large");
+        return CUBE2_ERR_CAPACITY;
+    }
+    const size_t lds = (size_t)(n + L * n) * sizeof(float2);
+    k_fft_x_c2r<<<(unsigned)((npairs + L - 1) / L), kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
@@ -360,13 +402,15 @@ int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s) {
 int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
              const float* green, hipStream_t s) {
     const int n = P.n, nh = n / 2 + 1;
-    const int C = std::max(1, std::min(nh, kFFTElems / n));
+    int lgC = lg_floor(std::max(1, kFFTElems / n));
+    while ((1 << lgC) > 2 * nh) lgC--;  // no wider than the half spectrum needs
+    const int C = 1 << lgC;
     const int ntiles = (nh + C - 1) / C;
-    const unsigned blocks = (unsigned)((int64_t)nb * n * ntiles);
-    const size_t lds = (size_t)(n + 2 * C * n) * sizeof(float2);
-    if (mode == 0) k_fft_cols<0><<<blocks, kFFTThreads, lds, s>>>(d, P, nb, rs, axis, C, green);
-    else if (mode == 1) k_fft_cols<1><<<blocks, kFFTThreads, lds, s>>>(d, P, nb, rs, axis, C, green);
-    else k_fft_cols<2><<<blocks, kFFTThreads, lds, s>>>(d, P, nb, rs, axis, C, green);
+    dim3 grid((unsigned)(n * ntiles), (unsigned)nb);
+    const size_t lds = (size_t)(n + C * n) * sizeof(float2);
+    if (mode == 0) k_fft_cols<0><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+    else if (mode == 1) k_fft_cols<1><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+    else k_fft_cols<2><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

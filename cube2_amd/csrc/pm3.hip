@@ -128,6 +128,89 @@ __global__ __launch_bounds__(256) void k_pm3_deposit(MeshArgs A, float* __restri
     }
 }
 
+// Brick deposit: block (brick, job) owns a kBrick^3 region of the job's
+// mesh (storage indices), sums in LDS every TSC weight that lands in it
+// from the particles of the coarse cells that can reach it, then stores the
+// brick.  Every mesh point belongs to one brick, so the mesh needs no
+// clearing and HBM sees no atomics; dense cores contend only in LDS.
+constexpr int kBrick = 16;
+
+__device__ inline int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+__global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* __restrict__ rho,
+                                                            size_t rs, int nbx,
+                                                            int* __restrict__ err) {
+    __shared__ float box[kBrick * kBrick * kBrick];
+    const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
+    const int jb = blockIdx.y;
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nbx, bz = b / (nbx * nbx);
+    const int lo[3] = {bx * kBrick, by * kBrick, bz * kBrick};
+    for (int q = threadIdx.x; q < kBrick * kBrick * kBrick; q += blockDim.x) box[q] = 0.0f;
+    const PM3Job J = A.jobs[jb];
+    const int cf = A.ratio_cs * A.rsf;  // fine cells per coarse cell
+    // A particle with c = floor(xpos) writes storage c+off .. c+off+2, so it
+    // reaches [lo, lo+kBrick) iff c in [lo-off-2, lo+kBrick-off-1]; its
+    // coarse cell (nte index) is floor((c + u*nfp)/cf) + ncb.
+    int clo[3], chi[3];
+    for (int d = 0; d < 3; d++) {
+        const int cmin = lo[d] - off - 2 + J.u[d] * A.nfp;
+        const int cmax = lo[d] + kBrick - off - 1 + J.u[d] * A.nfp;
+        const int a = floordiv(cmin, cf) + A.ncb;
+        const int e = floordiv(cmax, cf) + A.ncb;
+        const int dlo = J.u[d] * A.ntt + A.ncb - 1, dhi = (J.u[d] + 1) * A.ntt + A.ncb;
+        clo[d] = max(a, dlo);
+        chi[d] = min(e, dhi);
+    }
+    __syncthreads();
+    for (int ck = clo[2]; ck <= chi[2]; ck++)
+        for (int cj = clo[1]; cj <= chi[1]; cj++)
+            for (int ci = clo[0]; ci <= chi[0]; ci++) {
+                const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
+                const int np = A.rhoc[cell];
+                const int64_t base = A.nzero[cell];
+                if (np <= 0) continue;
+                if (base < 0 || base + np > A.n_used) {
+                    if (threadIdx.x == 0) atomicOr(err, 2);
+                    continue;
+                }
+                for (int l = threadIdx.x; l < np; l += blockDim.x) {
+                    const int16_t* p = A.xp + 3 * (base + l);
+                    int cx, cy, cz;
+                    float wx[3], wy[3], wz[3];
+                    tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
+                    tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
+                    tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+                    const int X0 = cx + off - lo[0], Y0 = cy + off - lo[1], Z0 = cz + off - lo[2];
+                    if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 || Z0 > kBrick - 1 || Z0 < -2)
+                        continue;
+#pragma unroll
+                    for (int a3 = 0; a3 < 3; a3++) {
+                        const int Z = Z0 + a3;
+                        if ((unsigned)Z >= (unsigned)kBrick) continue;
+#pragma unroll
+                        for (int a2 = 0; a2 < 3; a2++) {
+                            const int Y = Y0 + a2;
+                            if ((unsigned)Y >= (unsigned)kBrick) continue;
+#pragma unroll
+                            for (int a1 = 0; a1 < 3; a1++) {
+                                const int X = X0 + a1;
+                                if ((unsigned)X >= (unsigned)kBrick) continue;
+                                atomicAdd(&box[(Z * kBrick + Y) * kBrick + X], wx[a1] * wy[a2] * wz[a3]);
+                            }
+                        }
+                    }
+                }
+            }
+    __syncthreads();
+    float* R = rho + (size_t)jb * rs;
+    for (int q = threadIdx.x; q < kBrick * kBrick * kBrick; q += blockDim.x) {
+        const int X = lo[0] + (q % kBrick), Y = lo[1] + (q / kBrick) % kBrick,
+                  Z = lo[2] + q / (kBrick * kBrick);
+        if (X < n && Y < n && Z < n) R[((size_t)Z * n + Y) * (n + 2) + X] = box[q];
+    }
+}
+
 // Force planes over Fortran indices [0, nfp+1]^3 (kick.f90:545-568) and the
 // interior max |F|^2 (kick.f90:494).  Block = 64 x-points x 4 rows; grid =
 // (x tiles * y tiles, z, job): 32-bit index math, coalesced along x.
@@ -383,13 +466,10 @@ static int run_batch(const MeshArgs& base, const std::vector<PM3Job>& jobs, size
     const FFTPlan* P = nullptr;
     C2_TRY(fft_plan(n, &P));
     {
-        Stage st("pm3_clear", (double)nb * rs * 4);
-        C2_HIP(hipMemsetAsync(rho, 0, nb * rs * sizeof(float), s));
-    }
-    {
-        const int64_t waves = (int64_t)nb * (base.ntt + 2) * (base.ntt + 2) * (base.ntt + 2);
-        Stage st("pm3_deposit", (double)nb * rs * 4);
-        k_pm3_deposit<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, rho, rs, err);
+        const int nbx = (n + kBrick - 1) / kBrick;
+        Stage st("pm3_deposit", (double)nb * n * n * n * 4);
+        dim3 grid((unsigned)(nbx * nbx * nbx), (unsigned)nb);
+        k_pm3_deposit_bricks<<<grid, 256, 0, s>>>(A, rho, rs, nbx, err);
         C2_HIP(hipGetLastError());
     }
     if (dbg && dbg->rho) {
