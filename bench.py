@@ -98,7 +98,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pp-mode", default="ref", choices=["ref", "exact", "fast"])
+    ap.add_argument("--pp-mode", default="ref", choices=["ref", "exact", "fast", "precise"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
@@ -122,7 +122,8 @@ def main(argv=None) -> int:
     lay = build_layout(pos, vel, C1)
     del pos, vel
     log(f"inputs: {lay.n_particles} particles, {lay.n_used} slots ({time.perf_counter() - t0:.1f} s)")
-    _lib.set_pp_mode({"ref": _lib.PP_REF, "exact": _lib.PP_EXACT, "fast": _lib.PP_FAST}[args.pp_mode])
+    _lib.set_pp_mode({"ref": _lib.PP_REF, "exact": _lib.PP_EXACT, "fast": _lib.PP_FAST,
+                     "precise": _lib.PP_PRECISE}[args.pp_mode])
     k = ShortRangeKick(lay)
     levels = sorted(set(int(x) for x in k.ires))
     for L in levels:
@@ -150,7 +151,8 @@ def main(argv=None) -> int:
     _lib.profile(False)
     ms_step = 1e3 * wall / args.steps
 
-    pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq"}[args.pp_mode]
+    pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq",
+               "precise": "pp_pairs_precise"}[args.pp_mode]
     pp_ms = stats[pp_name]["ms"] / args.steps
     flops = pp_flops(res.p_cand, res.p_acc)
     pair_rate = res.p_acc / (pp_ms * 1e-3)

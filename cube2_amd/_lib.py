@@ -16,7 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libcube2_short.so"
 _LIB = None
 
 OK, ERR_ARG, ERR_GEOMETRY, ERR_CAPACITY, ERR_HIP, ERR_STATE = range(6)
-PP_EXACT, PP_FAST, PP_REF = 0, 1, 2
+PP_EXACT, PP_FAST, PP_REF, PP_PRECISE = 0, 1, 2, 3
 
 
 class Cube2Error(RuntimeError):
@@ -31,6 +31,15 @@ class Params(C.Structure):
                 ("body_centered_cubic", C.c_int), ("image_buffer", C.c_double),
                 ("tile_buffer", C.c_double), ("app", C.c_double)]
 
+
+class KickResultC(C.Structure):
+    _fields_ = [("f2max_pm3", C.c_float), ("f2max_pp", C.c_float), ("vmax", C.c_float * 3),
+                ("dt_pm3", C.c_double), ("dt_pp", C.c_double), ("dt_vmax", C.c_double),
+                ("pt_pm3", C.c_double), ("pt_pp", C.c_double), ("pair_counts", C.c_uint64 * 2),
+                ("napm", C.c_int * 7)]
+
+
+KICK_PM3, KICK_PP, KICK_KEEP_LEVELS = 1, 2, 4
 
 vp_ = C.c_void_p
 i32 = C.POINTER(C.c_int)
@@ -49,6 +58,8 @@ SIGNATURES = {
     "cube2_velocity_apply_dev": (C.c_int, [vp_, vp_, vp_, C.c_int64, vp_]),
     "cube2_green_3d_dev": (C.c_int, [vp_, C.c_int, C.c_float, C.c_float, C.c_float]),
     "cube2_select_levels": (C.c_int, [vp_] * 5),
+    "cube2_kick_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3
+                       + [vp_] * 5 + [C.POINTER(KickResultC)]),
     "cube2_pp_debug_celllist": (C.c_int, [C.c_int] + [vp_] * 6),
     "cube2_pm3_debug_job": (C.c_int, [C.c_int] + [vp_] * 6 + [C.c_int64, vp_, C.c_float] + [vp_] * 4),
     "cube2_fft3d_dev": (C.c_int, [vp_, C.c_int, C.c_int, C.c_int]),

@@ -30,49 +30,19 @@ void pm3_release();
 int green_cache_get(const float* host, size_t count, const float** dev);
 void green_cache_release();
 int selftest_div(uint64_t seed, int n, unsigned long long* mismatches);
+void heap_index(int n, const float* arr, int* indx);
+int level_of(int maxcount);
+int check_ixyz3(const Geom& g, const int* ixyz3);
+int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, const int* d_rhoc,
+         const int64_t* d_idx, const int16_t* d_xp, float* d_vp, int64_t n_used,
+         const float* const* d_green, float mass, float a_mid, float dt, int* ires, int* isort,
+         double* ptotal_pm3, double* ptotal_pp, double* ttotal_pp, cube2_kick_result_t* out);
+void kick_release();
 }  // namespace cube2
 
 using namespace cube2;
 
 static std::recursive_mutex g_mx;
-
-// Level selection of kick.f90:107-131; job order = heapsort of -npc_max
-// (kick.f90:570-614), restated.
-static void heap_index(int n, const float* arr, int* indx) {
-    for (int j = 1; j <= n; j++) indx[j - 1] = j;
-    if (n < 2) return;
-    auto A = [&](int q) { return arr[q - 1]; };
-    auto IX = [&](int q) -> int& { return indx[q - 1]; };
-    int l = n / 2 + 1, ir = n;
-    for (;;) {
-        int it;
-        float qv;
-        if (l > 1) {
-            it = IX(--l);
-            qv = A(it);
-        } else {
-            it = IX(ir);
-            qv = A(it);
-            IX(ir) = IX(1);
-            if (--ir == 1) {
-                IX(1) = it;
-                return;
-            }
-        }
-        int i = l, j = l + l;
-        while (j <= ir) {
-            if (j < ir && A(IX(j)) < A(IX(j + 1))) j++;
-            if (qv < A(IX(j))) {
-                IX(i) = IX(j);
-                i = j;
-                j += j;
-            } else {
-                j = ir + 1;
-            }
-        }
-        IX(i) = it;
-    }
-}
 
 static int64_t n_used_of(const Geom& g, const int64_t* idx_b_r) {
     return idx_b_r[(size_t)g.ntile * g.nte * g.nte - 1];
@@ -180,6 +150,7 @@ void c_pm3_fft_finalize(void) {
 void c_gpu_velocity_finalize(void) {
     std::lock_guard<std::recursive_mutex> l(g_mx);
     pp_release();
+    kick_release();
     Context& C = ctx();
     C.dv64.release();
     C.d_vp.release();
@@ -236,15 +207,11 @@ int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort
         set_error("null argument");
         return CUBE2_ERR_ARG;
     }
+    C2_TRY(check_ixyz3(g, ixyz3));
     std::vector<float> neg(g.ns3);
     int nap[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int it = 0; it < g.ns3; it++) {
         const int* x = ixyz3 + 6 * it;
-        for (int q = 0; q < 3; q++)
-            if (x[q] < 1 || x[q] > g.p.nns || x[q + 3] < 1 || x[q + 3] > g.p.nnt) {
-                set_error("invalid ixyz3 row %d", it);
-                return CUBE2_ERR_ARG;
-            }
         const int t = ((x[5] - 1) * g.p.nnt + (x[4] - 1)) * g.p.nnt + (x[3] - 1);
         const int* rc = rhoc + (size_t)t * g.nte * g.nte * g.nte;
         int mx = 0;
@@ -252,16 +219,25 @@ int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort
             for (int j = (x[1] - 1) * g.ntt; j < x[1] * g.ntt; j++)
                 for (int i = (x[0] - 1) * g.ntt; i < x[0] * g.ntt; i++)
                     mx = std::max(mx, rc[((size_t)(k + g.ncb) * g.nte + (j + g.ncb)) * g.nte + (i + g.ncb)]);
-        const float p = (float)mx / 64.0f;  // exact: thresholds are maxval < 1536/7296/20480/86400
-        int L = p < 24.0f ? 2 : p < 114.0f ? 3 : p < 320.0f ? 4 : p < 1350.0f ? 5 : 6;
+        const int L = level_of(mx);
         ires[it] = L;
         nap[L - 1]++;
-        neg[it] = -p;
+        neg[it] = -((float)mx / 64.0f);
     }
     heap_index(g.ns3, neg.data(), isort);
     if (napm)
         for (int q = 0; q < 7; q++) napm[q] = nap[q];
     return CUBE2_OK;
+}
+
+int cube2_kick_dev(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf,
+                   const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp, float* d_vp,
+                   int64_t n_used, const float* const* d_green, float mass, float a_mid, float dt,
+                   int* ires, int* isort, double* ptotal_pm3, double* ptotal_pp,
+                   double* ttotal_pp, cube2_kick_result_t* out) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return kick(flags, ixyz3, apm3, ratio_sf, d_rhoc, d_idx_b_r, d_xp, d_vp, n_used, d_green, mass,
+                a_mid, dt, ires, isort, ptotal_pm3, ptotal_pp, ttotal_pp, out);
 }
 
 int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches) {

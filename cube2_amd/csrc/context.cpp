@@ -216,7 +216,7 @@ int cube2_get_params(cube2_params_t* p) {
 }
 
 int cube2_set_pp_mode(int mode) {
-    if (mode != CUBE2_PP_EXACT && mode != CUBE2_PP_FAST && mode != CUBE2_PP_REF) {
+    if (mode < CUBE2_PP_EXACT || mode > CUBE2_PP_PRECISE) {
         set_error("unknown PP mode %d", mode);
         return CUBE2_ERR_ARG;
     }

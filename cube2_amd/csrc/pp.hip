@@ -32,7 +32,7 @@
 
 namespace cube2 {
 
-int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, bool ref, hipStream_t s);
+int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, int mode, hipStream_t s);
 
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
@@ -480,7 +480,7 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
             }
             C2_HIP(hipMemcpyAsync(&nchunks, flags + n_tot, sizeof(int), hipMemcpyDeviceToHost, s));
             C2_HIP(hipStreamSynchronize(s));
-            C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), nchunks, C.pp_mode == CUBE2_PP_REF, s));
+            C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), nchunks, C.pp_mode, s));
         }
     }
     // results

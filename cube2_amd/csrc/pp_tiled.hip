@@ -21,7 +21,15 @@
 //   midpoint, so this equals IEEE a/b.  The pair forces are bit-identical to
 //   the reference; only the order of the per-i sums differs (<= ~4e-7
 //   relative RMS).
-// !kRef: FMA/rsqrt arithmetic with F_ra in Horner form.
+// FAST: FMA/rsqrt arithmetic with F_ra in Horner form.
+// PRECISE: a cancellation-free force law and FP64 per-particle sums.  For
+//   1 < ep <= 2, 35 a^2 ep^2 (F_ra(r,a) - 1/r^2) = (ep-2)^5 (7ep^3 + 22ep^2
+//   + 10ep + 4), so 1/r^2 - F_ra = (2-ep)^5 (7ep^3+22ep^2+10ep+4)/(140 r^2)
+//   without the term-by-term cancellation near the cutoff; sums of pair
+//   terms accumulate in FP64 (full rate on MI355X) instead of Kahan-
+//   compensated FP32.
+#include <type_traits>
+
 #include "pp_kernels.h"
 #include "profile.h"
 #include "reduce.h"
@@ -79,10 +87,23 @@ __device__ inline float fra_fast2(float ep, float ir2, float c) {
     return ep > 1.0f ? hi : lo;
 }
 
-template <bool kRef>
+enum { kModeRef = 0, kModeFast = 1, kModePrecise = 2 };
+
+// 1/r^2 - F_ra(r, a) for r < a (ep = 2r/a in (0, 2]), cancellation-free.
+__device__ inline float short_term(float ep, float ir2, float c) {
+    if (ep > 1.0f) {
+        const float t = 2.0f - ep, t2 = t * t;
+        const float q = fmaf(ep, fmaf(ep, fmaf(ep, 7.0f, 22.0f), 10.0f), 4.0f);
+        return t2 * t2 * t * q * ir2 * (1.0f / 140.0f);
+    }
+    const float e2 = ep * ep;
+    return ir2 - c * ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+}
+
+template <int kMode>
 __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, float& fx, float& fy,
                                   float& fz) {
-    if (kRef) {
+    if (kMode == kModeRef) {
 #pragma clang fp contract(off)
         const float rm = sqrtf(e.w);
         const double inv_rr = recip64((double)(rm * rm));
@@ -91,7 +112,7 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
         fx = div_by(fpp * e.x, inv_rm);
         fy = div_by(fpp * e.y, inv_rm);
         fz = div_by(fpp * e.z, inv_rm);
-    } else {
+    } else if (kMode == kModeFast) {
         const float ir = __frsqrt_rn(e.w);
         const float ir2 = ir * ir;
         const float r = e.w * ir;
@@ -101,35 +122,47 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
         fx = f * e.x;
         fy = f * e.y;
         fz = f * e.z;
+    } else {
+        // pair = (1/r^2 - F_ra(r, range)) - (1/r^2 - F_ra(r, app)), the second
+        // term zero for r > app; each piece cancellation-free.
+        const float r = sqrtf(e.w);
+        const float ir = 1.0f / r;
+        const float ir2 = ir * ir;
+        float f = short_term(r * (float)(2.0 * Kr.inv_a), ir2, Kr.c);
+        if (Ka.a > 0.0f && r < Ka.a) f -= short_term(r * (float)(2.0 * Ka.inv_a), ir2, Ka.c);
+        f *= ir;
+        fx = f * e.x;
+        fy = f * e.y;
+        fz = f * e.z;
     }
 }
 
-// Orders LDS accesses of the lanes of one wavefront (no block barrier: the
-// wavefronts of a block run independent trip counts).
-__device__ inline void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// Orders LDS accesses of the lanes of one wavefront.  LDS operations of a
+// wavefront are performed in issue order, so a compiler barrier suffices (no
+// block barrier: the wavefronts of a block run independent trip counts).
+__device__ inline void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
 
+template <int kMode>
 struct WaveLDS {
+    using Acc = typename std::conditional<kMode == kModePrecise, double, float>::type;
     float4 q[kQ];
     unsigned char qi[kQ];
-    float acc[64][3];
+    Acc acc[64][3];
 };
 
-template <bool kRef>
-__device__ inline void flush(WaveLDS& L, int lane, int begin, int count, const FraK& Ka,
+template <int kMode>
+__device__ inline void flush(WaveLDS<kMode>& L, int lane, int begin, int count, const FraK& Ka,
                              const FraK& Kr) {
     if (lane < count) {
         const int s = (begin + lane) & (kQ - 1);
         const float4 e = L.q[s];
         const int ii = L.qi[s];
         float fx, fy, fz;
-        pair_force<kRef>(e, Ka, Kr, fx, fy, fz);
-        atomicAdd(&L.acc[ii][0], fx);
-        atomicAdd(&L.acc[ii][1], fy);
-        atomicAdd(&L.acc[ii][2], fz);
+        pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
+        using Acc = typename WaveLDS<kMode>::Acc;
+        atomicAdd(&L.acc[ii][0], (Acc)fx);
+        atomicAdd(&L.acc[ii][1], (Acc)fy);
+        atomicAdd(&L.acc[ii][2], (Acc)fz);
     }
     wave_lds_sync();
 }
@@ -144,12 +177,12 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
     return lo;
 }
 
-template <bool kRef>
+template <int kMode>
 __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
                                                   int nchunks) {
-    __shared__ WaveLDS lds[4];
+    __shared__ WaveLDS<kMode> lds[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveLDS& L = lds[wave];
+    WaveLDS<kMode>& L = lds[wave];
     const int ck = blockIdx.x * 4 + wave;
     float f2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f;
     unsigned long long pc = 0, pa = 0;
@@ -186,7 +219,7 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
             Ka.c = 1.0f / 35.0f / (A.app * A.app);
             Kr.c = 1.0f / 35.0f / (range * range);
         }
-        for (int q = lane; q < 64 * 3; q += 64) (&L.acc[0][0])[q] = 0.0f;
+        for (int q = lane; q < 64 * 3; q += 64) (&L.acc[0][0])[q] = 0;
         wave_lds_sync();
         const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int qhead = 0, qtail = 0;
@@ -210,7 +243,10 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
                         e.x = qj.x - p.x;
                         e.y = qj.y - p.y;
                         e.z = qj.z - p.z;
-                        e.w = e.x * e.x + e.y * e.y + e.z * e.z;
+                        if (kMode == kModePrecise)
+                            e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
+                        else
+                            e.w = e.x * e.x + e.y * e.y + e.z * e.z;
                         ok = e.w > 0.0f && e.w < pp2;
                     }
                     const uint64_t m = __ballot(ok);
@@ -223,20 +259,21 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
                         qtail += __popcll(m);
                         wave_lds_sync();
                         if (qtail - qhead >= 64) {
-                            flush<kRef>(L, lane, qhead, 64, Ka, Kr);
+                            flush<kMode>(L, lane, qhead, 64, Ka, Kr);
                             qhead += 64;
                         }
                     }
                 }
             }
         }
-        if (qtail > qhead) flush<kRef>(L, lane, qhead, qtail - qhead, Ka, Kr);
+        if (qtail > qhead) flush<kMode>(L, lane, qhead, qtail - qhead, Ka, Kr);
         pa = (lane == 0) ? (unsigned long long)qtail : 0ull;
         if (vi && jslot == 0) {
 #pragma clang fp contract(off)
             pc -= 1;  // the particle itself
             const int a = a0 + islot;
-            const float ax = L.acc[islot][0], ay = L.acc[islot][1], az = L.acc[islot][2];
+            const float ax = (float)L.acc[islot][0], ay = (float)L.acc[islot][1],
+                        az = (float)L.acc[islot][2];
             const float a0m = ax * A.mass * A.mass, a1m = ay * A.mass * A.mass,
                         a2m = az * A.mass * A.mass;
             f2 = a0m * a0m + a1m * a1m + a2m * a2m;
@@ -262,12 +299,15 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
     block_sum_to<2>(&A.red->p_cand, sm);
 }
 
-int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, bool ref, hipStream_t s) {
-    Stage st(ref ? "pp_pairs_ref" : "pp_pairs_fastq");
+int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, int mode, hipStream_t s) {
+    const char* name = mode == CUBE2_PP_REF ? "pp_pairs_ref"
+                       : mode == CUBE2_PP_FAST ? "pp_pairs_fastq" : "pp_pairs_precise";
+    Stage st(name);
     const unsigned blocks = ceil_div((size_t)nchunks, 4);
     if (blocks) {
-        if (ref) k_pp_tiled<true><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
-        else k_pp_tiled<false><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+        if (mode == CUBE2_PP_REF) k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+        else if (mode == CUBE2_PP_FAST) k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+        else k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
         C2_HIP(hipGetLastError());
     }
     return CUBE2_OK;

@@ -49,6 +49,7 @@ class KickResult:
     ttotal_pp: np.ndarray | None = None
     p_cand: int = 0
     p_acc: int = 0
+    dt_native: dict | None = None   # dt limits as the library's kick driver computed them
 
     def dt_limits(self, a_mid: float) -> dict:
         """dt_pm3, dt_pp, dt_vmax in FP64 (kick.f90:180,227,238)."""
@@ -130,18 +131,46 @@ class ShortRangeKick:
         res.ttotal_pp = tt
         res.p_cand, res.p_acc = int(cnt[0]), int(cnt[1])
 
-    def kick(self, mass: float, a_mid: float, dt: float, pm3: bool = True, pp: bool = True) -> KickResult:
-        res = KickResult()
-        if pm3:
-            self.pm3(mass, a_mid, dt, res)
-        if pp:
-            self.pp(mass, a_mid, dt, res)
-        elif pm3:
-            vm = np.zeros(3, np.float32)
-            _lib.check(_lib.lib().cube2_velocity_apply_dev(
-                self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_vp.data_ptr(),
-                self.n_used, _np_ptr(vm)))
-            res.vmax = vm
+    def _green_ptrs(self):
+        arr = (C.c_void_p * 5)()
+        for lev in sorted(set(int(x) for x in self.ires)):
+            arr[lev - 2] = self.green(lev).data_ptr()
+        return arr
+
+    def kick(self, mass: float, a_mid: float, dt: float, pm3: bool = True, pp: bool = True,
+             select: bool = False) -> KickResult:
+        """One short-range kick through the library's kick driver
+        (cube2_kick_dev).  select=True re-selects levels on the device from
+        rhoc (kick.f90:107-131); otherwise the current ires/isort are used."""
+        L = _lib.lib()
+        _lib.set_params(self.geom)
+        flags = (_lib.KICK_PM3 if pm3 else 0) | (_lib.KICK_PP if pp else 0)
+        if select:
+            self.ires = np.zeros(self.geom.ns3, np.int32)
+            self.isort = np.zeros(self.geom.ns3, np.int32)
+            # Green tables are bound after selection; select first, then kick
+            out = _lib.KickResultC()
+            _lib.check(L.cube2_kick_dev(
+                0, _np_ptr(self.ix), _np_ptr(APM3), _np_ptr(RATIO_SF), self.d_rhoc.data_ptr(),
+                self.d_idx.data_ptr(), self.d_xp.data_ptr(), self.d_vp.data_ptr(), self.n_used,
+                None, 1.0, 1.0, 0.0, _np_ptr(self.ires), _np_ptr(self.isort), None, None, None,
+                C.byref(out)))
+            self.napm = np.array(out.napm[:], np.int32)
+        ns3 = self.geom.ns3
+        pt3, ptp, ttp = np.zeros(ns3), np.zeros(ns3), np.zeros(ns3)
+        out = _lib.KickResultC()
+        _lib.check(L.cube2_kick_dev(
+            flags | _lib.KICK_KEEP_LEVELS, _np_ptr(self.ix), _np_ptr(APM3), _np_ptr(RATIO_SF),
+            self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_xp.data_ptr(),
+            self.d_vp.data_ptr(), self.n_used, self._green_ptrs(), float(mass), float(a_mid),
+            float(dt), _np_ptr(self.ires), _np_ptr(self.isort), _np_ptr(pt3), _np_ptr(ptp),
+            _np_ptr(ttp), C.byref(out)))
+        res = KickResult(f2max_pm3=out.f2max_pm3, f2max_pp=out.f2max_pp,
+                         vmax=np.array(out.vmax[:], np.float32),
+                         ptotal_pm3=pt3 if pm3 else None, ptotal_pp=ptp if pp else None,
+                         ttotal_pp=ttp if pp else None, p_cand=int(out.pair_counts[0]),
+                         p_acc=int(out.pair_counts[1]))
+        res.dt_native = dict(dt_pm3=out.dt_pm3, dt_pp=out.dt_pp, dt_vmax=out.dt_vmax)
         return res
 
     def vp_host(self) -> np.ndarray:

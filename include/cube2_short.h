@@ -38,7 +38,7 @@ enum {
     CUBE2_ERR_STATE = 5     /* call order (velocity accumulator) violated */
 };
 
-enum { CUBE2_PP_EXACT = 0, CUBE2_PP_FAST = 1, CUBE2_PP_REF = 2 };
+enum { CUBE2_PP_EXACT = 0, CUBE2_PP_FAST = 1, CUBE2_PP_REF = 2, CUBE2_PP_PRECISE = 3 };
 
 /* ---------------------------------------------------------------------
  * Drop-in entry points (host pointers; the Fortran caller owns every array)
@@ -96,7 +96,9 @@ int cube2_get_params(cube2_params_t* p);
  *   CUBE2_PP_EXACT: one lane per particle, the reference's summation order
  *     too: Delta-v bit-identical to pp_force_kernel.c (slow, for checking);
  *   CUBE2_PP_FAST: FMA/rsqrt arithmetic, work-compacted (its own rounding;
- *     accuracy stated against an FP64 direct sum). */
+ *     accuracy stated against an FP64 direct sum);
+ *   CUBE2_PP_PRECISE: cancellation-free force law near the cutoff and FP64
+ *     per-particle sums (accuracy stated against an FP64 direct sum). */
 int cube2_set_pp_mode(int mode);
 
 /* ---------------------------------------------------------------------
@@ -129,6 +131,37 @@ int cube2_green_3d_dev(float* d_G, int n, float apm_soft, float apm_range, float
 
 /* Level selection and job order of kick.f90:107-131 (host arrays). */
 int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort, int* napm);
+
+/* ---------------------------------------------------------------------
+ * Kick driver: the short-range part of kick.f90:107-238 in one call over
+ * device-resident arrays (the Fortran host's level selection, the PM3 and
+ * PP calls of kick.f90:144-146,202-205, and the dt limits of
+ * kick.f90:180,227,236).
+ * ------------------------------------------------------------------- */
+typedef struct {
+    float f2max_pm3, f2max_pp;      /* kick.f90:146,205 */
+    float vmax[3];                  /* max |vp| over physical slots after the kick */
+    double dt_pm3, dt_pp, dt_vmax;  /* FP64; +inf for a zero force / velocity or a skipped part */
+    double pt_pm3, pt_pp;           /* sum(ptotal) of PM3 and of PP */
+    uint64_t pair_counts[2];        /* PP P_cand, P_acc */
+    int napm[7];                    /* subtiles per level (kick.f90:129) */
+} cube2_kick_result_t;
+
+enum {
+    CUBE2_KICK_PM3 = 1,          /* run PM3 */
+    CUBE2_KICK_PP = 2,           /* run PP (applies PM3's pending Delta-v) */
+    CUBE2_KICK_KEEP_LEVELS = 4   /* use the caller's ires/isort instead of selecting them */
+};
+
+/* flags: CUBE2_KICK_*.  ires/isort [ns3] are written (level selection of
+ * kick.f90:107-131 from the device rhoc) unless KEEP_LEVELS.  PM3 without
+ * PP applies its Delta-v before returning.  ptotal_pm3, ptotal_pp,
+ * ttotal_pp [ns3] may be NULL. */
+int cube2_kick_dev(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf,
+                   const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp, float* d_vp,
+                   int64_t n_used, const float* const* d_green /* [5]: levels 2..6 */,
+                   float mass, float a_mid, float dt, int* ires, int* isort, double* ptotal_pm3,
+                   double* ptotal_pp, double* ttotal_pp, cube2_kick_result_t* out);
 
 /* ---------------------------------------------------------------------
  * Inspection (tests and benchmarks)

@@ -126,3 +126,27 @@ def test_pp_accepts_reference_ratio_table_without_gpu():
                                       1e-3, f2.ctypes.data, vm.ctypes.data, pt.ctypes.data, None,
                                       None)
     assert rc == _lib.ERR_HIP, _lib.lib().cube2_last_error()
+
+
+def test_kick_driver_validates_before_device_work():
+    import ctypes as C
+    from cube2_amd import _lib
+    from cube2_amd.geometry import APM3, RATIO_SF, C0, ixyz3
+    _lib.set_params(C0)
+    ix = ixyz3(C0)
+    ires = np.full(C0.ns3, 7, np.int32)
+    isort = np.arange(1, C0.ns3 + 1, dtype=np.int32)
+    out = _lib.KickResultC()
+    a = lambda x: x.ctypes.data  # noqa: E731
+    L = _lib.lib()
+    args = (a(ix), a(APM3), a(RATIO_SF), None, None, None, None, 0, None, 1.0, 0.02, 1e-3,
+            a(ires), a(isort), None, None, None, C.byref(out))
+    assert L.cube2_kick_dev(_lib.KICK_PP | _lib.KICK_KEEP_LEVELS, *args) == _lib.ERR_ARG
+    assert b"outside 2..6" in L.cube2_last_error()
+    assert L.cube2_kick_dev(64, *args) == _lib.ERR_ARG          # unknown flag
+    ires[:] = 2
+    ix2 = ix.copy()
+    ix2[0, 0] = 0
+    args2 = (a(ix2),) + args[1:]
+    assert L.cube2_kick_dev(_lib.KICK_PP | _lib.KICK_KEEP_LEVELS, *args2) == _lib.ERR_ARG
+    assert b"ixyz3" in L.cube2_last_error()

@@ -1,0 +1,107 @@
+"""The library's kick driver (cube2_kick_dev, kick.f90:107-238) on the GPU:
+device level selection equals the oracle's (integer, exact); one driver
+call equals separate PM3 and PP calls bit for bit; the dt limits equal the
+FP64 formulas of kick.f90:180,227,236."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+    assert torch.cuda.is_available()
+    from cube2_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+@pytest.fixture(scope="module")
+def mixed_layout():
+    """64^3 box in 2^3 tiles of 2^3 subtiles, uniform background plus one
+    clump; levels 2..4 drawn per subtile."""
+    from cube2_amd.geometry import Geometry
+    from cube2_amd.layout import build_layout
+    g = Geometry(ng=64, nnt=2, nns=2, ratio_cs=4, ngb=8, app=0.05)
+    rng = np.random.default_rng(23)
+    pos = np.concatenate([rng.uniform(0, 64, (20000, 3)),
+                          np.array([31.5, 30.0, 33.0]) + rng.normal(0, 0.7, (3000, 3))]) % 64
+    lay = build_layout(pos, None, g)
+    ires = rng.integers(2, 5, g.ns3).astype(np.int32)
+    return lay, ires
+
+
+@pytest.mark.parametrize("seed", [9, 10])
+def test_device_level_selection_matches_oracle(lib, oracle_lib, seed):
+    import torch
+    from cube2_amd.geometry import APM3, RATIO_SF, REFERENCE_DEFAULT, ixyz3
+    g = REFERENCE_DEFAULT.with_(nnt=2, ng=128, nns=2)
+    rng = np.random.default_rng(seed)
+    rhoc = rng.integers(0, 5, (g.ntile, g.nte, g.nte, g.nte)).astype(np.int32)
+    hot = rng.integers(0, rhoc.size, 60)            # halo and physical cells
+    rhoc.reshape(-1)[hot] = rng.integers(1000, 100000, 60)
+    rhoc.reshape(-1)[hot[:4]] = [1535, 1536, 7295, 7296]    # level thresholds
+    lib.set_params(g)
+    d_rhoc = torch.from_numpy(rhoc).cuda()
+    ix = ixyz3(g)
+    ires = np.zeros(g.ns3, np.int32)
+    isort = np.zeros(g.ns3, np.int32)
+    out = lib.KickResultC()
+    lib.check(lib.lib().cube2_kick_dev(0, ix.ctypes.data, APM3.ctypes.data, RATIO_SF.ctypes.data,
+                                       d_rhoc.data_ptr(), None, None, None, 0, None, 1.0, 1.0, 0.0,
+                                       ires.ctypes.data, isort.ctypes.data, None, None, None,
+                                       C.byref(out)))
+    o_ires, o_isort, o_napm, _ = oracle_lib.select_levels(g, rhoc, ix)
+    assert np.array_equal(ires, o_ires)
+    assert np.array_equal(isort, o_isort)
+    assert np.array_equal(np.array(out.napm[:]), o_napm)
+    assert math.isinf(out.dt_pm3) and math.isinf(out.dt_pp) and math.isinf(out.dt_vmax)
+
+
+def test_kick_driver_equals_separate_calls(lib, mixed_layout):
+    """PM3 + PP (exact mode) through one driver call and through the two
+    device entry points give identical velocities and diagnostics."""
+    from cube2_amd.kick import KickResult, ShortRangeKick
+    lay, ires = mixed_layout
+    lib.set_pp_mode(lib.PP_EXACT)
+    mass, amid, dt = 1.3, 0.05, 2e-3
+    k1 = ShortRangeKick(lay)
+    k1.set_levels(ires)
+    r1 = k1.kick(mass, amid, dt)
+    k2 = ShortRangeKick(lay, greens=k1.greens)
+    k2.set_levels(ires)
+    r2 = KickResult()
+    k2.pm3(mass, amid, dt, r2)
+    k2.pp(mass, amid, dt, r2)
+    lib.set_pp_mode(lib.PP_REF)
+    assert np.array_equal(k1.vp_host(), k2.vp_host())
+    assert r1.f2max_pm3 == r2.f2max_pm3 and r1.f2max_pp == r2.f2max_pp
+    assert np.array_equal(r1.vmax, r2.vmax)
+    assert np.array_equal(r1.ptotal_pm3, r2.ptotal_pm3)
+    assert np.array_equal(r1.ptotal_pp, r2.ptotal_pp)
+    assert (r1.p_cand, r1.p_acc) == (r2.p_cand, r2.p_acc)
+    want = r2.dt_limits(amid)
+    for key in ("dt_pm3", "dt_pp", "dt_vmax"):
+        assert r1.dt_native[key] == pytest.approx(want[key], rel=1e-15), key
+    assert r1.dt_native["dt_pm3"] == pytest.approx(
+        0.5 * math.sqrt(1.0 / (math.sqrt(r1.f2max_pm3) * amid / (6 * math.pi))), rel=1e-15)
+
+
+def test_kick_driver_selects_levels_on_device(lib, mixed_layout):
+    """select=True picks ires/isort from the device rhoc, as the host
+    restatement does from the host rhoc."""
+    from cube2_amd.kick import ShortRangeKick, select_levels
+    lay, _ = mixed_layout
+    k = ShortRangeKick(lay)
+    h_ires, h_isort, h_napm = select_levels(lay.geom, lay.rhoc)
+    res = k.kick(1.0, 0.02, 1e-3, select=True)
+    assert np.array_equal(k.ires, h_ires)
+    assert np.array_equal(k.isort, h_isort)
+    assert np.array_equal(k.napm, h_napm)
+    assert res.p_acc > 0 and res.f2max_pm3 > 0
