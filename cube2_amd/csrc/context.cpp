@@ -197,9 +197,16 @@ int cube2_set_params(const cube2_params_t* p) {
         return CUBE2_ERR_GEOMETRY;
     }
     Context& c = ctx();
+    const cube2_params_t& q = c.g.p;
+    const bool same = c.params_set && q.ng == p->ng && q.nnt == p->nnt && q.nns == p->nns &&
+                      q.ratio_cs == p->ratio_cs && q.np_nc == p->np_nc && q.ngb == p->ngb &&
+                      q.ncore == p->ncore && q.body_centered_cubic == p->body_centered_cubic &&
+                      q.image_buffer == p->image_buffer && q.tile_buffer == p->tile_buffer &&
+                      q.app == p->app;
+    if (same) return CUBE2_OK;  // a pending PM3 Delta-v stays pending
     c.g = make_geom(*p);
     c.params_set = true;
-    c.dv_state = 0;
+    c.dv_state = 0;  // a new geometry drops a pending Delta-v
     c.staged_xp_host = nullptr;
     return CUBE2_OK;
 }

@@ -86,6 +86,8 @@ typedef struct {
     double image_buffer, tile_buffer, app;
 } cube2_params_t;
 
+/* Setting the current values again is a no-op; a different geometry drops
+ * a pending PM3 Delta-v. */
 int cube2_set_params(const cube2_params_t* p);
 int cube2_get_params(cube2_params_t* p);
 
