@@ -3,8 +3,10 @@
 Workload: 256^3 particles in a 300 Mpc/h box (60% Zel'dovich lattice at
 z=49, 40% in 2000 Plummer spheres), fine mesh 512^3 as 8^3 subtiles of 64^3
 fine cells (nnt=4, nns=2, 6-coarse-cell tile buffer), synthetic and seeded.
-One step = one kick: level selection -> PM3 over all 512 subtiles -> PP over
-all 64 parent tiles with the velocity update, inputs resident in HBM.
+One step = one kick: PM3 over all 512 subtiles -> PP over all 64 parent
+tiles with the velocity update, inputs resident in HBM.  Timed twice: with
+eager dispatch (per-stage HIP-event profile) and as a recorded HIP graph
+replay; `value` is the replayed kick.
 
 Prints one JSON line: kick ms/step (value, lower is better) plus PP pair
 interactions per second, the roofline of the dominant kernel from HIP events
@@ -93,6 +95,21 @@ def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s
                         f"OpenMP threads {ncpu}"))
 
 
+def copy_probe(mib: int) -> int:
+    """Three device-to-device copies of a known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_copy": n * 4}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,8 +118,14 @@ def main(argv=None) -> int:
     ap.add_argument("--pp-mode", default="ref", choices=["ref", "exact", "fast", "precise"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager dispatch only (value = eager kick), e.g. under counters")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (calibrates HBM counters)")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path runs on a single MI355X; "
               "multi-GPU sharding is by parent tile and is not part of this build.",
@@ -132,24 +155,49 @@ def main(argv=None) -> int:
     log(f"levels {dict(zip(*np.unique(k.ires, return_counts=True)))}; Green tables built")
     mass, amid, dt = 1.0, 0.02, 1e-3
 
+    t1 = time.perf_counter()
+    res = k.kick(mass, amid, dt)
+    cold_ms = 1e3 * (time.perf_counter() - t1)
+    log(f"first kick (allocations, plans): {cold_ms:.1f} ms "
+        f"(P_cand {res.p_cand:.3e}, P_acc {res.p_acc:.3e})")
     for w in range(args.warmup):
         t1 = time.perf_counter()
         res = k.kick(mass, amid, dt)
-        log(f"warmup {w}: {1e3 * (time.perf_counter() - t1):.1f} ms "
-            f"(P_cand {res.p_cand:.3e}, P_acc {res.p_acc:.3e})")
+        log(f"warmup {w}: {1e3 * (time.perf_counter() - t1):.1f} ms")
 
+    # eager dispatch, with per-stage HIP-event timing
     _lib.profile(True)
     _lib.profile_reset()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for s in range(args.steps):
         res = k.kick(mass, amid, dt)
-        log(f"step {s} done")
+        log(f"eager step {s} done")
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
+    wall_eager = time.perf_counter() - t1
     stats = _lib.profile_stats()
     _lib.profile(False)
-    ms_step = 1e3 * wall / args.steps
+    eager_ms = 1e3 * wall_eager / args.steps
+
+    # the same kick recorded once into a HIP graph, then replayed
+    graph_ms = None
+    ms_step = eager_ms
+    if not args.no_graph:
+        t1 = time.perf_counter()
+        rg = k.kick_graph(mass, amid, dt)
+        log(f"graph recorded ({rg.recorded}) and run: {1e3 * (time.perf_counter() - t1):.1f} ms")
+        for w in range(args.warmup):
+            k.kick_graph(mass, amid, dt)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for s in range(args.steps):
+            rg = k.kick_graph(mass, amid, dt)
+            log(f"graph step {s} done")
+        torch.cuda.synchronize()
+        wall_graph = time.perf_counter() - t1
+        assert not rg.recorded and (rg.p_cand, rg.p_acc) == (res.p_cand, res.p_acc)
+        graph_ms = 1e3 * wall_graph / args.steps
+        ms_step = graph_ms
 
     pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq",
                "precise": "pp_pairs_precise"}[args.pp_mode]
@@ -198,6 +246,9 @@ def main(argv=None) -> int:
                                "ng=256 nnt=4 nns=2 ngb=24, PM3+PP kick",
                    "pp_mode": args.pp_mode, "n_used_slots": int(lay.n_used),
                    "levels": {int(a): int(b) for a, b in zip(*np.unique(k.ires, return_counts=True))}},
+        "kick_ms_graph_replay": None if graph_ms is None else round(graph_ms, 3),
+        "kick_ms_eager": round(eager_ms, 3),
+        "kick_ms_first_call": round(cold_ms, 3),
         "pp_pair_interactions_per_s": float(f"{pair_rate:.4g}"),
         "pp_ms_per_step": round(pp_ms, 3),
         "p_cand": int(res.p_cand),

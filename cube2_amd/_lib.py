@@ -60,6 +60,8 @@ SIGNATURES = {
     "cube2_select_levels": (C.c_int, [vp_] * 5),
     "cube2_kick_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3
                        + [vp_] * 5 + [C.POINTER(KickResultC)]),
+    "cube2_kick_graph_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3
+                             + [vp_] * 5 + [C.POINTER(KickResultC), vp_]),
     "cube2_pp_debug_celllist": (C.c_int, [C.c_int] + [vp_] * 6),
     "cube2_pm3_debug_job": (C.c_int, [C.c_int] + [vp_] * 6 + [C.c_int64, vp_, C.c_float] + [vp_] * 4),
     "cube2_fft3d_dev": (C.c_int, [vp_, C.c_int, C.c_int, C.c_int]),

@@ -4,41 +4,8 @@
 #include <mutex>
 
 #include "common.h"
+#include "engine.h"
 #include "profile.h"
-
-namespace cube2 {
-int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
-             const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
-             int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
-             double* ptotal, double* ttotal, uint64_t* pair_counts);
-int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf);
-int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t* slot,
-                      int32_t* cell_start);
-void pp_release();
-int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
-              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
-              const float* const* d_green, float mass, float amid, float dt, float* f2max,
-              double* ptotal);
-int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf, const int* d_rhoc,
-                  const int64_t* d_idx, const int16_t* d_xp, int64_t n_used, const float* d_green,
-                  float mass, float* rho, float* phi, float* force, float* f2max);
-int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t n_used,
-                   float* vmax);
-int green_3d(float* d_G, int n, float apm_soft, float apm_range, float gridsize);
-int fft3d(float* d, int n, int nb, int dir);
-void pm3_release();
-int green_cache_get(const float* host, size_t count, const float** dev);
-void green_cache_release();
-int selftest_div(uint64_t seed, int n, unsigned long long* mismatches);
-void heap_index(int n, const float* arr, int* indx);
-int level_of(int maxcount);
-int check_ixyz3(const Geom& g, const int* ixyz3);
-int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, const int* d_rhoc,
-         const int64_t* d_idx, const int16_t* d_xp, float* d_vp, int64_t n_used,
-         const float* const* d_green, float mass, float a_mid, float dt, int* ires, int* isort,
-         double* ptotal_pm3, double* ptotal_pp, double* ttotal_pp, cube2_kick_result_t* out);
-void kick_release();
-}  // namespace cube2
 
 using namespace cube2;
 
@@ -238,6 +205,18 @@ int cube2_kick_dev(int flags, const int* ixyz3, const float* apm3, const int* ra
     std::lock_guard<std::recursive_mutex> l(g_mx);
     return kick(flags, ixyz3, apm3, ratio_sf, d_rhoc, d_idx_b_r, d_xp, d_vp, n_used, d_green, mass,
                 a_mid, dt, ires, isort, ptotal_pm3, ptotal_pp, ttotal_pp, out);
+}
+
+int cube2_kick_graph_dev(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf,
+                         const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                         float* d_vp, int64_t n_used, const float* const* d_green, float mass,
+                         float a_mid, float dt, const int* ires, const int* isort,
+                         double* ptotal_pm3, double* ptotal_pp, double* ttotal_pp,
+                         cube2_kick_result_t* out, int* recorded) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return kick_graph(flags, ixyz3, apm3, ratio_sf, d_rhoc, d_idx_b_r, d_xp, d_vp, n_used,
+                      d_green, mass, a_mid, dt, ires, isort, ptotal_pm3, ptotal_pp, ttotal_pp,
+                      out, recorded);
 }
 
 int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches) {

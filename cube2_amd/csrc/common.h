@@ -50,6 +50,23 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Generation of the library's device/host buffers and uploaded tables:
+// bumped by every (re)allocation, release and table upload, so that a
+// recorded graph can tell that the memory or tables it refers to changed.
+uint64_t buffers_generation();
+void buffers_changed();
+
+// Grow-only pinned host buffer (hipHostMalloc): D2H targets of results, so
+// that result copies can be recorded into a HIP graph.
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
 // ------------------------------------------------------------ profiling
 // Per-stage GPU time, recorded with events on the library stream when
 // profiling is on (cube2_profile_enable).
@@ -78,9 +95,19 @@ struct Context {
     DevBuf d_xp, d_vp, d_rhoc, d_idx;
     const int16_t* staged_xp_host = nullptr;
     int64_t staged_xp_n = -1;
+    DevBuf d_scal;
+    HostBuf h_scal;
+    // true while work is being recorded into a HIP graph: no host syncs,
+    // allocations, memsets or profiler events may be issued
+    bool capturing = false;
 };
 Context& ctx();
-int ensure_init();   // params + HIP device + library stream
+int ensure_init();
+
+// Per-call scalars that kernels read from device memory, so that a recorded
+// graph replays with new values: {mass, a_mid, dt, 0}.  The upload copies
+// from a pinned host block, which a replay reads at launch time.
+int scalars_upload(float mass, float a_mid, float dt, hipStream_t s, const float** dev);   // params + HIP device + library stream
 void ensure_params();  // params only (no HIP call)
 
 // ------------------------------------------------------------- helpers

@@ -35,13 +35,22 @@ Geom make_geom(const cube2_params_t& p) {
     return g;
 }
 
+static uint64_t g_generation = 1;
+uint64_t buffers_generation() { return g_generation; }
+void buffers_changed() { ++g_generation; }
+
 int DevBuf::reserve(size_t bytes) {
     if (bytes <= cap) return CUBE2_OK;
+    if (ctx().capturing) {
+        set_error("device buffer growth (%zu bytes) while recording a graph", bytes);
+        return CUBE2_ERR_STATE;
+    }
     if (ptr) {
         C2_HIP(hipFree(ptr));
         ptr = nullptr;
         cap = 0;
     }
+    buffers_changed();
     size_t want = bytes + bytes / 8 + 256;
     C2_HIP(hipMalloc(&ptr, want));
     cap = want;
@@ -49,7 +58,33 @@ int DevBuf::reserve(size_t bytes) {
 }
 
 void DevBuf::release() {
+    buffers_changed();
     if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+int HostBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return CUBE2_OK;
+    if (ctx().capturing) {
+        set_error("host buffer growth (%zu bytes) while recording a graph", bytes);
+        return CUBE2_ERR_STATE;
+    }
+    if (ptr) {
+        C2_HIP(hipHostFree(ptr));
+        ptr = nullptr;
+        cap = 0;
+    }
+    buffers_changed();
+    size_t want = bytes + bytes / 8 + 256;
+    C2_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+    cap = want;
+    return CUBE2_OK;
+}
+
+void HostBuf::release() {
+    buffers_changed();
+    if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
     cap = 0;
 }
@@ -82,6 +117,20 @@ void ensure_params() {
         c.g = make_geom(default_params());
         c.params_set = true;
     }
+}
+
+int scalars_upload(float mass, float a_mid, float dt, hipStream_t s, const float** dev) {
+    Context& c = ctx();
+    C2_TRY(c.d_scal.reserve(4 * sizeof(float)));
+    C2_TRY(c.h_scal.reserve(4 * sizeof(float)));
+    float* h = c.h_scal.as<float>();
+    h[0] = mass;
+    h[1] = a_mid;
+    h[2] = dt;
+    h[3] = 0.0f;
+    C2_HIP(hipMemcpyAsync(c.d_scal.ptr, h, 4 * sizeof(float), hipMemcpyHostToDevice, s));
+    *dev = c.d_scal.as<float>();
+    return CUBE2_OK;
 }
 
 int ensure_init() {
@@ -129,7 +178,7 @@ static hipEvent_t get_event() {
 }
 
 Stage::Stage(const char* name, double bytes, double flops) {
-    if (!g_prof) return;
+    if (!g_prof || ctx().capturing) return;
     on_ = true;
     name_ = name;
     bytes_ = bytes;

@@ -1,0 +1,92 @@
+// Internal entry points shared by the C ABI, the kick driver and the graph
+// recorder.  Each phase split:
+//   *_prepare: validation, host tables, buffer growth and table uploads
+//              (may allocate and synchronise; never inside graph recording);
+//   *_enqueue: kernels and result copies into pinned host memory only
+//              (no host synchronisation: recordable into a HIP graph);
+//   *_finish:  after the stream has completed, host-side results and errors.
+#pragma once
+#include "common.h"
+
+namespace cube2 {
+
+// ----------------------------------------------------------------- PP
+struct PPPlan {
+    std::vector<int> iloc, owner;  // subtile location -> level / -> subtile index
+    int ntile = 0, nsub3 = 0;
+    int64_t c_tot = 0, ncells_ext = 0, n_used = 0;
+    size_t cap = 1;    // capacity of the sorted arrays (n_used)
+    int mode = 0;      // CUBE2_PP_*
+    int grid_pairs = 0;  // blocks of the persistent pair kernel
+};
+int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf);
+int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+               int64_t n_used, PPPlan& P);
+int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp,
+               float* d_vp, const float* d_scal, const double* dv64, hipStream_t s);
+int pp_finish(const PPPlan& P, float* f2max, float* vmax, double* ptotal, double* ttotal,
+              uint64_t* pair_counts, double wall_s);
+int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+             const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
+             int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
+             double* ptotal, double* ttotal, uint64_t* pair_counts);
+int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t* slot,
+                      int32_t* cell_start);
+void pp_release();
+int compute_nzero(const int* d_rhoc, const int64_t* d_idx, const int64_t** out, hipStream_t s);
+int nzero_reserve();
+
+// ---------------------------------------------------------------- PM3
+struct PM3Batch {
+    int level = 0, rsf = 0;
+    size_t job0 = 0, nb = 0;  // first job in the device job table, jobs in the batch
+    size_t slot0 = 0;         // first ptotal slot
+};
+struct PM3Plan {
+    std::vector<PM3Batch> batches;
+    std::vector<int> job_it;  // subtile of each ptotal slot
+    const float* green[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t n_used = 0;
+};
+int pm3_prepare(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+                int64_t n_used, const float* const* d_green, PM3Plan& P);
+// d_scal: {mass, a_mid, dt} on the device (scalars_upload)
+int pm3_enqueue(const PM3Plan& P, const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp,
+                const float* d_scal, int accumulate, double* dv64, hipStream_t s);
+int pm3_finish(const PM3Plan& P, float* f2max, double* ptotal);
+int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
+              const float* const* d_green, float mass, float amid, float dt, float* f2max,
+              double* ptotal);
+int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf, const int* d_rhoc,
+                  const int64_t* d_idx, const int16_t* d_xp, int64_t n_used, const float* d_green,
+                  float mass, float* rho, float* phi, float* force, float* f2max);
+int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t n_used,
+                   float* vmax);
+int green_3d(float* d_G, int n, float apm_soft, float apm_range, float gridsize);
+int fft3d(float* d, int n, int nb, int dir);
+void pm3_release();
+int green_cache_get(const float* host, size_t count, const float** dev);
+void green_cache_release();
+
+// --------------------------------------------------------------- kick
+void heap_index(int n, const float* arr, int* indx);
+int level_of(int maxcount);
+int check_ixyz3(const Geom& g, const int* ixyz3);
+int select_levels_dev(const int* ixyz3, const int* d_rhoc, int* ires, int* isort, int* napm);
+int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, const int* d_rhoc,
+         const int64_t* d_idx, const int16_t* d_xp, float* d_vp, int64_t n_used,
+         const float* const* d_green, float mass, float a_mid, float dt, int* ires, int* isort,
+         double* ptotal_pm3, double* ptotal_pp, double* ttotal_pp, cube2_kick_result_t* out);
+int kick_graph(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf,
+               const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
+               int64_t n_used, const float* const* d_green, float mass, float a_mid, float dt,
+               const int* ires, const int* isort, double* ptotal_pm3, double* ptotal_pp,
+               double* ttotal_pp, cube2_kick_result_t* out, int* recorded);
+void kick_release();
+
+// ---------------------------------------------------------------- misc
+int fill_zero(void* d, size_t bytes, hipStream_t s);  // a kernel, not a memset
+int selftest_div(uint64_t seed, int n, unsigned long long* mismatches);
+
+}  // namespace cube2

@@ -19,8 +19,9 @@ struct FFTPlan {
 int fft_plan(int n, const FFTPlan** out);
 void fft_release();
 
-// R2C along x for every row pair, scaling the real input by `scale`.
-int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, float scale, hipStream_t s);
+// R2C along x for every row pair, scaling the real input by *scale (device
+// pointer; nullptr: no scaling).
+int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale, hipStream_t s);
 // C2R along x for every row pair.
 int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s);
 // Complex transforms along y (axis 1) or z (axis 2) of the half spectrum.

@@ -179,7 +179,8 @@ __device__ inline float* row_pair(float* d, int g, int n, size_t rs) {
 // time along the row (coalesced).
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d, FFTPlan P,
                                                            int npairs, size_t rs, int lgL,
-                                                           float scale) {
+                                                           const float* __restrict__ scale_dev) {
+    const float scale = scale_dev ? *scale_dev : 1.0f;
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1;
     const int L = 1 << lgL;
@@ -359,6 +360,7 @@ int fft_plan(int n, const FFTPlan** out) {
 }
 
 void fft_release() {
+    buffers_changed();
     for (auto& kv : g_plans) (void)hipFree(kv.second.tw);
     g_plans.clear();
 }
@@ -369,7 +371,7 @@ static int lg_floor(int v) {
     return l;
 }
 
-int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, float scale, hipStream_t s) {
+int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale, hipStream_t s) {
     const int n = P.n;
     const int lgL = lg_floor(std::max(1, kFFTElems / n));
     const int L = 1 << lgL;
@@ -420,7 +422,7 @@ int fft3d_batch(float* d, int n, int nb, int dir, hipStream_t s) {
     C2_TRY(fft_plan(n, &P));
     const size_t rs = (size_t)(n + 2) * n * n;
     if (dir < 0) {
-        C2_TRY(fft_x_r2c(d, *P, nb, rs, 1.0f, s));
+        C2_TRY(fft_x_r2c(d, *P, nb, rs, nullptr, s));
         C2_TRY(fft_cols(d, *P, nb, rs, 1, 0, nullptr, s));
         C2_TRY(fft_cols(d, *P, nb, rs, 2, 0, nullptr, s));
     } else {

@@ -2,24 +2,15 @@
 // arrays: level selection from the per-subtile maximum coarse-cell count,
 // job order, PM3, PP (which applies PM3's pending Delta-v), and the FP64
 // time-step limits.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 
 #include "common.h"
+#include "engine.h"
 #include "profile.h"
 
 namespace cube2 {
-
-int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf);
-int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
-             const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
-             int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
-             double* ptotal, double* ttotal, uint64_t* pair_counts);
-int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
-              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
-              const float* const* d_green, float mass, float amid, float dt, float* f2max,
-              double* ptotal);
-int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t n_used,
-                   float* vmax);
 
 // Job order of kick.f90:131: indexed heapsort of -npc_max (kick.f90:570-614,
 // Numerical Recipes' indexx), 1-based indices, not stable.
@@ -208,7 +199,204 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
     return CUBE2_OK;
 }
 
+// ------------------------------------------------------- recorded kick
+// The kick with fixed levels, recorded once into a HIP graph and replayed.
+// Everything that shapes the launches is part of the key; the scalars
+// (mass, a_mid, dt) are read by the kernels from device memory, which the
+// graph refills from a pinned block at each launch.
+struct GraphKey {
+    int flags = 0, mode = 0;
+    int64_t n_used = 0;
+    const void* arrays[4] = {nullptr, nullptr, nullptr, nullptr};
+    const float* green[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<int> ires, isort, ixyz3;
+    float apm3[7] = {0};
+    int ratio_sf[7] = {0};
+    cube2_params_t p{};
+    bool same(const GraphKey& o) const {
+        return flags == o.flags && mode == o.mode && n_used == o.n_used &&
+               std::equal(arrays, arrays + 4, o.arrays) && std::equal(green, green + 5, o.green) &&
+               ires == o.ires && isort == o.isort && ixyz3 == o.ixyz3 &&
+               std::memcmp(apm3, o.apm3, sizeof(apm3)) == 0 &&
+               std::memcmp(ratio_sf, o.ratio_sf, sizeof(ratio_sf)) == 0 &&
+               std::memcmp(&p, &o.p, sizeof(p)) == 0;
+    }
+};
+
+struct KickGraph {
+    bool valid = false;
+    GraphKey key;
+    uint64_t generation = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    PM3Plan pm3;
+    PPPlan pp;
+};
+static KickGraph g_graph;
+
+static void graph_destroy() {
+    if (g_graph.exec) (void)hipGraphExecDestroy(g_graph.exec);
+    if (g_graph.graph) (void)hipGraphDestroy(g_graph.graph);
+    g_graph.exec = nullptr;
+    g_graph.graph = nullptr;
+    g_graph.valid = false;
+}
+
+static int graph_enqueue(const GraphKey& K, const int* d_rhoc, const int64_t* d_idx,
+                         const int16_t* d_xp, float* d_vp, float mass, float a_mid, float dt,
+                         hipStream_t s) {
+    Context& C = ctx();
+    const float* d_scal = nullptr;
+    C2_TRY(scalars_upload(mass, a_mid, dt, s, &d_scal));
+    const bool pm3 = K.flags & CUBE2_KICK_PM3;
+    if (pm3)
+        C2_TRY(pm3_enqueue(g_graph.pm3, d_rhoc, d_idx, d_xp, d_scal, 0, C.dv64.as<double>(), s));
+    C2_TRY(pp_enqueue(g_graph.pp, d_rhoc, d_idx, d_xp, d_vp, d_scal,
+                      pm3 ? C.dv64.as<double>() : nullptr, s));
+    return CUBE2_OK;
+}
+
+static int graph_record(const GraphKey& K, const int* ixyz3, const float* apm3,
+                        const int* ratio_sf, const int* d_rhoc, const int64_t* d_idx,
+                        const int16_t* d_xp, float* d_vp, const float* const* d_green, float mass,
+                        float a_mid, float dt) {
+    Context& C = ctx();
+    hipStream_t s = C.stream;
+    graph_destroy();
+    if (K.flags & CUBE2_KICK_PM3)
+        C2_TRY(pm3_prepare(K.isort.data(), K.ires.data(), ixyz3, ratio_sf, K.n_used, d_green,
+                           g_graph.pm3));
+    C2_TRY(pp_prepare(K.ires.data(), ixyz3, apm3, ratio_sf, K.n_used, g_graph.pp));
+    C2_TRY(C.dv64.reserve((size_t)std::max<int64_t>(K.n_used, 1) * 3 * sizeof(double)));
+    const float* d_scal = nullptr;
+    C2_TRY(scalars_upload(mass, a_mid, dt, s, &d_scal));  // reserves the scalar blocks
+    C2_HIP(hipStreamSynchronize(s));
+    const uint64_t gen = buffers_generation();
+    C2_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    C.capturing = true;
+    const int rc = graph_enqueue(K, d_rhoc, d_idx, d_xp, d_vp, mass, a_mid, dt, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    C.capturing = false;
+    if (rc != CUBE2_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != CUBE2_OK) return rc;
+        set_error("graph recording failed: %s", hipGetErrorString(e));
+        return CUBE2_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        set_error("graph instantiation failed: %s", hipGetErrorString(ei));
+        return CUBE2_ERR_HIP;
+    }
+    if (buffers_generation() != gen) {  // nothing may grow while recording
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        set_error("library buffers changed while recording the kick");
+        return CUBE2_ERR_STATE;
+    }
+    g_graph.graph = graph;
+    g_graph.exec = exec;
+    g_graph.key = K;
+    g_graph.generation = gen;
+    g_graph.valid = true;
+    return CUBE2_OK;
+}
+
+int kick_graph(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf,
+               const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
+               int64_t n_used, const float* const* d_green, float mass, float a_mid, float dt,
+               const int* ires, const int* isort, double* ptotal_pm3, double* ptotal_pp,
+               double* ttotal_pp, cube2_kick_result_t* out, int* recorded) {
+    if (!ixyz3 || !apm3 || !ratio_sf || !ires || !out || !d_rhoc || !d_idx || !d_xp || !d_vp ||
+        (flags != (CUBE2_KICK_PM3 | CUBE2_KICK_PP) && flags != CUBE2_KICK_PP)) {
+        set_error("recorded kick: null argument, or flags other than PM3|PP or PP");
+        return CUBE2_ERR_ARG;
+    }
+    if ((flags & CUBE2_KICK_PM3) && (!isort || !d_green)) {
+        set_error("recorded kick: PM3 needs isort and Green's tables");
+        return CUBE2_ERR_ARG;
+    }
+    ensure_params();
+    Context& C = ctx();
+    const Geom& g = C.g;
+    C2_TRY(check_ixyz3(g, ixyz3));
+    for (int it = 0; it < g.ns3; it++)
+        if (ires[it] < 2 || ires[it] > 6) {
+            set_error("recorded kick: ires[%d] = %d outside 2..6", it, ires[it]);
+            return CUBE2_ERR_ARG;
+        }
+    C2_TRY(pp_validate(ires, ixyz3, ratio_sf));
+    if (C.dv_state == 1) {
+        set_error("recorded kick: a PM3 Delta-v is pending from an earlier call");
+        return CUBE2_ERR_STATE;
+    }
+    C2_TRY(ensure_init());
+    GraphKey K;
+    K.flags = flags;
+    K.mode = C.pp_mode;
+    K.n_used = n_used;
+    K.arrays[0] = d_rhoc;
+    K.arrays[1] = d_idx;
+    K.arrays[2] = d_xp;
+    K.arrays[3] = d_vp;
+    if (flags & CUBE2_KICK_PM3) {
+        for (int q = 0; q < 5; q++) K.green[q] = d_green[q];
+        K.isort.assign(isort, isort + g.ns3);
+    }
+    K.ires.assign(ires, ires + g.ns3);
+    K.ixyz3.assign(ixyz3, ixyz3 + 6 * g.ns3);
+    std::memcpy(K.apm3, apm3, sizeof(K.apm3));
+    std::memcpy(K.ratio_sf, ratio_sf, sizeof(K.ratio_sf));
+    K.p = g.p;
+    const bool fresh = !(g_graph.valid && g_graph.key.same(K) &&
+                         g_graph.generation == buffers_generation());
+    if (fresh)
+        C2_TRY(graph_record(K, ixyz3, apm3, ratio_sf, d_rhoc, d_idx, d_xp, d_vp, d_green, mass,
+                            a_mid, dt));
+    if (recorded) *recorded = fresh ? 1 : 0;
+    hipStream_t s = C.stream;
+    float* h = C.h_scal.as<float>();  // read by the graph's first node at launch
+    h[0] = mass;
+    h[1] = a_mid;
+    h[2] = dt;
+    h[3] = 0.0f;
+    const auto t0 = std::chrono::steady_clock::now();
+    {
+        Stage st("kick_graph");
+        C2_HIP(hipGraphLaunch(g_graph.exec, s));
+    }
+    C2_TRY(sync_and_flush());
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::memset(out, 0, sizeof(*out));
+    for (int it = 0; it < g.ns3; it++) out->napm[ires[it] - 1]++;
+    std::vector<double> pt3, ptp, ttp;
+    if (!ptotal_pm3) ptotal_pm3 = (pt3.assign(g.ns3, 0.0), pt3.data());
+    if (!ptotal_pp) ptotal_pp = (ptp.assign(g.ns3, 0.0), ptp.data());
+    if (!ttotal_pp) ttotal_pp = (ttp.assign(g.ns3, 0.0), ttp.data());
+    if (flags & CUBE2_KICK_PM3) {
+        C2_TRY(pm3_finish(g_graph.pm3, &out->f2max_pm3, ptotal_pm3));
+        for (int it = 0; it < g.ns3; it++) out->pt_pm3 += ptotal_pm3[it];
+        out->dt_pm3 = dt_force(out->f2max_pm3, a_mid);
+    } else {
+        out->dt_pm3 = INFINITY;
+    }
+    uint64_t cnt[2] = {0, 0};
+    C2_TRY(pp_finish(g_graph.pp, &out->f2max_pp, out->vmax, ptotal_pp, ttotal_pp, cnt, wall));
+    out->pair_counts[0] = cnt[0];
+    out->pair_counts[1] = cnt[1];
+    for (int it = 0; it < g.ns3; it++) out->pt_pp += ptotal_pp[it];
+    out->dt_pp = dt_force(out->f2max_pp, a_mid);
+    const float vm = std::max(out->vmax[0], std::max(out->vmax[1], out->vmax[2]));
+    out->dt_vmax = vm > 0.0f ? 0.9 * 14.0 / (double)vm : INFINITY;
+    C.dv_state = 0;
+    return CUBE2_OK;
+}
+
 void kick_release() {
+    graph_destroy();
     g_kick.sub.release();
     g_kick.maxc.release();
 }

@@ -22,11 +22,10 @@
 #include "common.h"
 #include "fft.h"
 #include "profile.h"
+#include "engine.h"
 #include "reduce.h"
 
 namespace cube2 {
-
-int compute_nzero(const int* d_rhoc, const int64_t* d_idx, const int64_t** out, hipStream_t s);
 
 struct PM3Job {
     int t;      // parent tile (x-major)
@@ -36,6 +35,7 @@ struct PM3Job {
 
 struct PM3State {
     DevBuf rho, force, jobs, red, ptot;
+    HostBuf h_res;  // f2max bits, error flag, per-job particle counts (pinned)
 };
 static PM3State g_pm3;
 
@@ -257,8 +257,9 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
 // Wavefront per (job, physical coarse cell): TSC gather and FP64 Delta-v.
 __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __restrict__ force,
                                                     double* __restrict__ dv64, int accumulate,
-                                                    float amid, float dt,
+                                                    const float* __restrict__ scal,
                                                     unsigned long long* __restrict__ ptot) {
+    const float amid = scal[1], dt = scal[2];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t per = (int64_t)A.ntt * A.ntt * A.ntt;
     const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
@@ -447,19 +448,26 @@ struct DebugOut {
 };
 
 // Runs one batch of jobs of one level; f2bits and ptot are device scalars.
-static int run_batch(const MeshArgs& base, const std::vector<PM3Job>& jobs, size_t j0, size_t nb,
-                     const float* green, float mass, float amid, float dt, int accumulate,
+// Mesh and force-plane buffers of a batch of nb jobs at mesh size n.
+static int batch_reserve(int n, int nfp, size_t nb) {
+    const size_t rs = (size_t)(n + 2) * n * n;
+    const size_t fs = (size_t)(nfp + 2) * (nfp + 2) * (nfp + 2);
+    C2_TRY(g_pm3.rho.reserve(nb * rs * sizeof(float)));
+    C2_TRY(g_pm3.force.reserve(nb * 3 * fs * sizeof(float)));
+    return CUBE2_OK;
+}
+
+// d_jobs: the batch's nb jobs on the device.  Buffers must be reserved.
+static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
+                     const float* green, float mass, const float* d_scal, int accumulate,
                      double* dv64, unsigned* f2bits, unsigned long long* ptot, int* err,
                      DebugOut* dbg, hipStream_t s) {
     const int n = base.n, nfp = base.nfp;
     const size_t rs = (size_t)(n + 2) * n * n;
     const size_t fs = (size_t)(nfp + 2) * (nfp + 2) * (nfp + 2);
-    C2_TRY(g_pm3.rho.reserve(nb * rs * sizeof(float)));
-    C2_TRY(g_pm3.force.reserve(nb * 3 * fs * sizeof(float)));
-    C2_TRY(g_pm3.jobs.reserve(nb * sizeof(PM3Job)));
-    C2_HIP(hipMemcpyAsync(g_pm3.jobs.ptr, jobs.data() + j0, nb * sizeof(PM3Job), hipMemcpyHostToDevice, s));
+    C2_TRY(batch_reserve(n, nfp, nb));
     MeshArgs A = base;
-    A.jobs = g_pm3.jobs.as<PM3Job>();
+    A.jobs = d_jobs;
     A.njobs = (int)nb;
     float* rho = g_pm3.rho.as<float>();
     float* force = g_pm3.force.as<float>();
@@ -484,7 +492,7 @@ static int run_batch(const MeshArgs& base, const std::vector<PM3Job>& jobs, size
     const double mesh_bytes = (double)nb * rs * 4;
     {
         Stage st("pm3_fft_x_r2c", 2 * mesh_bytes);
-        C2_TRY(fft_x_r2c(rho, *P, (int)nb, rs, mass, s));
+        C2_TRY(fft_x_r2c(rho, *P, (int)nb, rs, d_scal, s));  // d_scal[0] = mass
     }
     {
         Stage st("pm3_fft_y", 2 * mesh_bytes);
@@ -526,7 +534,7 @@ static int run_batch(const MeshArgs& base, const std::vector<PM3Job>& jobs, size
     if (dv64) {
         const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt;
         Stage st("pm3_interp", (double)nb * 3 * fs * 4);
-        k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, amid, dt, ptot);
+        k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, d_scal, ptot);
         C2_HIP(hipGetLastError());
     }
     return CUBE2_OK;
@@ -552,14 +560,12 @@ static MeshArgs mesh_args(const Geom& g, const int* d_rhoc, const int64_t* nz, c
     return A;
 }
 
-int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
-              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
-              const float* const* d_green, float mass, float amid, float dt, float* f2max,
-              double* ptotal) {
+int pm3_prepare(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+                int64_t n_used, const float* const* d_green, PM3Plan& P) {
     ensure_params();
     Context& C = ctx();
     const Geom& g = C.g;
-    if (!ires || !ixyz3 || !ratio_sf || !f2max || !ptotal || !d_green) {
+    if (!ires || !ixyz3 || !ratio_sf || !d_green) {
         set_error("null argument");
         return CUBE2_ERR_ARG;
     }
@@ -574,54 +580,112 @@ int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ra
             set_error("no Green's table for level %d", G.L);
             return CUBE2_ERR_ARG;
         }
+    for (int q = 0; q < 5; q++) P.green[q] = d_green[q];
     C2_TRY(ensure_init());
-    hipStream_t s = C.stream;
-    if (C.dv_state == 1 && C.dv_n_used != n_used) {
-        set_error("PM3 particle count changed while a Delta-v is pending");
-        return CUBE2_ERR_STATE;
-    }
-    const int accumulate = C.dv_state == 1 ? 1 : 0;
-    C2_TRY(C.dv64.reserve((size_t)std::max<int64_t>(n_used, 1) * 3 * sizeof(double)));
-    const int64_t* nz = nullptr;
-    C2_TRY(compute_nzero(d_rhoc, d_idx, &nz, s));
-    C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
-    unsigned* f2bits = g_pm3.red.as<unsigned>();
-    int* err = reinterpret_cast<int*>(f2bits + 1);
-    C2_HIP(hipMemsetAsync(f2bits, 0, 4 * sizeof(unsigned), s));
-    size_t maxjobs = 0;
-    for (auto& G : groups) maxjobs = std::max(maxjobs, G.jobs.size());
-    C2_TRY(g_pm3.ptot.reserve(std::max<size_t>(g.ns3, 1) * sizeof(unsigned long long)));
-    unsigned long long* ptot = g_pm3.ptot.as<unsigned long long>();
-    C2_HIP(hipMemsetAsync(ptot, 0, g.ns3 * sizeof(unsigned long long), s));
-    std::vector<unsigned long long> hp(g.ns3, 0);
-    std::vector<int> job_it;  // subtile of each ptot slot
-    size_t slot_base = 0;
+    P.n_used = n_used;
+    P.batches.clear();
+    P.job_it.clear();
+    std::vector<PM3Job> all;
     for (auto& G : groups) {
         const int rsf = ratio_sf[G.L - 1];
-        MeshArgs A = mesh_args(g, d_rhoc, nz, d_xp, n_used, rsf);
+        MeshArgs A = mesh_args(g, nullptr, nullptr, nullptr, n_used, rsf);
         const size_t rs = (size_t)(A.n + 2) * A.n * A.n;
         const size_t fsz = (size_t)(A.nfp + 2) * (A.nfp + 2) * (A.nfp + 2);
         const size_t per_job = (rs + 3 * fsz) * sizeof(float);
         const size_t batch = std::max<size_t>(1, std::min(G.jobs.size(), g_budget / per_job));
+        const FFTPlan* fp = nullptr;
+        C2_TRY(fft_plan(A.n, &fp));
         for (size_t j0 = 0; j0 < G.jobs.size(); j0 += batch) {
-            const size_t nb = std::min(batch, G.jobs.size() - j0);
-            for (size_t q = 0; q < nb; q++) job_it.push_back(G.jobs[j0 + q].it);
-            C2_TRY(run_batch(A, G.jobs, j0, nb, d_green[G.L - 2], mass, amid, dt, accumulate,
-                             C.dv64.as<double>(), f2bits, ptot + slot_base, err, nullptr, s));
-            slot_base += nb;
+            PM3Batch B;
+            B.level = G.L;
+            B.rsf = rsf;
+            B.nb = std::min(batch, G.jobs.size() - j0);
+            B.job0 = all.size();
+            B.slot0 = P.job_it.size();
+            for (size_t q = 0; q < B.nb; q++) {
+                all.push_back(G.jobs[j0 + q]);
+                P.job_it.push_back(G.jobs[j0 + q].it);
+            }
+            C2_TRY(batch_reserve(A.n, A.nfp, B.nb));
+            P.batches.push_back(B);
         }
     }
-    unsigned h[4];
+    C2_TRY(nzero_reserve());
+    C2_TRY(C.dv64.reserve((size_t)std::max<int64_t>(n_used, 1) * 3 * sizeof(double)));
+    C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
+    C2_TRY(g_pm3.ptot.reserve(std::max<size_t>(g.ns3, 1) * sizeof(unsigned long long)));
+    C2_TRY(g_pm3.jobs.reserve(std::max<size_t>(all.size(), 1) * sizeof(PM3Job)));
+    C2_TRY(g_pm3.h_res.reserve(4 * sizeof(unsigned) + g.ns3 * sizeof(unsigned long long)));
+    hipStream_t s = C.stream;
+    if (!all.empty())
+        C2_HIP(hipMemcpyAsync(g_pm3.jobs.ptr, all.data(), all.size() * sizeof(PM3Job),
+                              hipMemcpyHostToDevice, s));
+    C2_HIP(hipStreamSynchronize(s));
+    buffers_changed();  // the device job table now describes this call
+    return CUBE2_OK;
+}
+
+int pm3_enqueue(const PM3Plan& P, const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp,
+                const float* d_scal, int accumulate, double* dv64, hipStream_t s) {
+    const Geom& g = ctx().g;
+    const int64_t* nz = nullptr;
+    C2_TRY(compute_nzero(d_rhoc, d_idx, &nz, s));
+    unsigned* f2bits = g_pm3.red.as<unsigned>();
+    int* err = reinterpret_cast<int*>(f2bits + 1);
+    unsigned long long* ptot = g_pm3.ptot.as<unsigned long long>();
+    C2_TRY(fill_zero(f2bits, 4 * sizeof(unsigned), s));
+    C2_TRY(fill_zero(ptot, g.ns3 * sizeof(unsigned long long), s));
+    const PM3Job* d_jobs = g_pm3.jobs.as<PM3Job>();
+    for (const PM3Batch& B : P.batches) {
+        MeshArgs A = mesh_args(g, d_rhoc, nz, d_xp, P.n_used, B.rsf);
+        C2_TRY(run_batch(A, d_jobs + B.job0, B.nb, P.green[B.level - 2], 1.0f, d_scal,
+                         accumulate, dv64, f2bits, ptot + B.slot0, err, nullptr, s));
+    }
+    char* h = g_pm3.h_res.as<char>();
     C2_HIP(hipMemcpyAsync(h, f2bits, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    C2_HIP(hipMemcpyAsync(hp.data(), ptot, g.ns3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    C2_TRY(sync_and_flush());
-    if (h[1]) {
-        set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)n_used);
+    C2_HIP(hipMemcpyAsync(h + 4 * sizeof(unsigned), ptot, g.ns3 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    return CUBE2_OK;
+}
+
+int pm3_finish(const PM3Plan& P, float* f2max, double* ptotal) {
+    const Geom& g = ctx().g;
+    const char* h = g_pm3.h_res.as<const char>();
+    unsigned r[4];
+    std::memcpy(r, h, sizeof(r));
+    const unsigned long long* hp = reinterpret_cast<const unsigned long long*>(h + sizeof(r));
+    if (r[1]) {
+        set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)P.n_used);
         return CUBE2_ERR_ARG;
     }
-    *f2max = __builtin_bit_cast(float, h[0]);
+    *f2max = __builtin_bit_cast(float, r[0]);
     for (int it = 0; it < g.ns3; it++) ptotal[it] = 0.0;
-    for (size_t q = 0; q < job_it.size(); q++) ptotal[job_it[q]] = (double)hp[q];
+    for (size_t q = 0; q < P.job_it.size(); q++) ptotal[P.job_it[q]] = (double)hp[q];
+    return CUBE2_OK;
+}
+
+int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
+              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
+              const float* const* d_green, float mass, float amid, float dt, float* f2max,
+              double* ptotal) {
+    ensure_params();
+    Context& C = ctx();
+    if (!f2max || !ptotal) {
+        set_error("null argument");
+        return CUBE2_ERR_ARG;
+    }
+    if (C.dv_state == 1 && C.dv_n_used != n_used) {
+        set_error("PM3 particle count changed while a Delta-v is pending");
+        return CUBE2_ERR_STATE;
+    }
+    PM3Plan P;
+    C2_TRY(pm3_prepare(isort, ires, ixyz3, ratio_sf, n_used, d_green, P));
+    const int accumulate = C.dv_state == 1 ? 1 : 0;
+    const float* d_scal = nullptr;
+    C2_TRY(scalars_upload(mass, amid, dt, C.stream, &d_scal));
+    C2_TRY(pm3_enqueue(P, d_rhoc, d_idx, d_xp, d_scal, accumulate, C.dv64.as<double>(), C.stream));
+    C2_TRY(sync_and_flush());
+    C2_TRY(pm3_finish(P, f2max, ptotal));
     C.dv_state = 1;
     C.dv_n_used = n_used;
     return CUBE2_OK;
@@ -652,11 +716,15 @@ int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf
     A.nzero = nz;
     C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
     unsigned* f2bits = g_pm3.red.as<unsigned>();
-    C2_HIP(hipMemsetAsync(f2bits, 0, 4 * sizeof(unsigned), s));
-    std::vector<PM3Job> one{J};
+    C2_TRY(fill_zero(f2bits, 4 * sizeof(unsigned), s));
+    C2_TRY(g_pm3.jobs.reserve(sizeof(PM3Job)));
+    C2_HIP(hipMemcpyAsync(g_pm3.jobs.ptr, &J, sizeof(PM3Job), hipMemcpyHostToDevice, s));
+    buffers_changed();
     DebugOut dbg{rho, phi, force};
-    C2_TRY(run_batch(A, one, 0, 1, d_green, mass, 0.0f, 0.0f, 0, nullptr, f2bits, nullptr,
-                     reinterpret_cast<int*>(f2bits + 1), &dbg, s));
+    const float* d_scal = nullptr;
+    C2_TRY(scalars_upload(mass, 0.0f, 0.0f, s, &d_scal));
+    C2_TRY(run_batch(A, g_pm3.jobs.as<PM3Job>(), 1, d_green, mass, d_scal, 0, nullptr, f2bits,
+                     nullptr, reinterpret_cast<int*>(f2bits + 1), &dbg, s));
     unsigned h[4];
     C2_HIP(hipMemcpyAsync(h, f2bits, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
     C2_TRY(sync_and_flush());
@@ -678,7 +746,7 @@ int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t
     C2_TRY(compute_nzero(d_rhoc, d_idx, &nz, s));
     C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
     unsigned* vb = g_pm3.red.as<unsigned>();
-    C2_HIP(hipMemsetAsync(vb, 0, 4 * sizeof(unsigned), s));
+    C2_TRY(fill_zero(vb, 4 * sizeof(unsigned), s));
     const int64_t waves = (int64_t)g.ntile * g.nt * g.nt * g.nt;
     k_velocity_apply<<<ceil_div(waves, kWaves), 256, 0, s>>>(
         d_rhoc, nz, g.nt, g.ncb, g.nte, g.ntile, n_used,
@@ -729,6 +797,7 @@ int green_cache_get(const float* host, size_t count, const float** dev) {
 }
 
 void green_cache_release() {
+    buffers_changed();
     for (auto& G : g_greens) (void)hipFree(G.dev);
     g_greens.clear();
 }
@@ -739,6 +808,7 @@ void pm3_release() {
     g_pm3.jobs.release();
     g_pm3.red.release();
     g_pm3.ptot.release();
+    g_pm3.h_res.release();
     fft_release();
 }
 

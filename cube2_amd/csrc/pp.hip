@@ -26,17 +26,21 @@
 #include <chrono>
 
 #include "common.h"
+#include "engine.h"
 #include "pp_kernels.h"
 #include "profile.h"
 #include "scan.h"
 
 namespace cube2 {
 
-int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, int mode, hipStream_t s);
+int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int grid, int mode,
+                    hipStream_t s);
+int pp_tiled_grid(int mode);
 
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
     DevBuf chunk_flags, chunks;
+    HostBuf h_res;  // PPRed, per-location counts, tile_first (pinned)
     std::vector<PPTile> h_tiles;
     int64_t c_tot = 0;
     int n_tot = 0;
@@ -161,8 +165,10 @@ __global__ __launch_bounds__(256) void k_pp_scatter(
     const int* __restrict__ rhoc, const int64_t* __restrict__ nzero, const int16_t* __restrict__ xp,
     const PPTile* __restrict__ tiles, const int* __restrict__ ires_loc,
     const float* __restrict__ apm3, int nt, int ncb, int nte, int nnt, int nns, int ntt,
-    int ratio_cs, int64_t ncells, const int* __restrict__ cell_start, float4* __restrict__ pos4,
-    uint32_t* __restrict__ slot, uint32_t* __restrict__ cellid) {
+    int ratio_cs, int64_t ncells, int64_t n_used, const int* __restrict__ cell_start,
+    const int* __restrict__ err, float4* __restrict__ pos4, uint32_t* __restrict__ slot,
+    uint32_t* __restrict__ cellid) {
+    if (*err & 4) return;  // sorted arrays would overflow: nothing is written
     __shared__ int cur[kWaves][kMaxSub];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
@@ -182,8 +188,8 @@ __global__ __launch_bounds__(256) void k_pp_scatter(
     __syncthreads();
     if (!ok) return;  // no block-level barrier below
     const int np = rhoc[c.cell];
-    if (np == 0) return;
     const int64_t base = nzero[c.cell];
+    if (np <= 0 || base < 0 || base + np > n_used) return;  // as k_pp_count: counted as empty
     const bool phys = c.ci >= 1 && c.ci <= nt && c.cj >= 1 && c.cj <= nt && c.ck >= 1 && c.ck <= nt;
     float range = 0.0f;
     if (phys) {
@@ -255,12 +261,16 @@ __global__ void k_pp_ptotal(const int* __restrict__ rhoc, int nt, int ncb, int n
 
 // Work chunks of the pair kernel: <= 64 consecutive physical particles of
 // one PP cell.  flags[a] = 1 where a chunk starts.
+// The sorted count n comes from the scan on the device; flags[a] = 0 for
+// n <= a < cap, and for every a if the call has failed.
 __global__ void k_chunk_flags(const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
-                              const int* __restrict__ cell_start, int n, int* __restrict__ flags) {
+                              const int* __restrict__ cell_start, const int* __restrict__ n_sorted,
+                              const int* __restrict__ err, int cap, int* __restrict__ flags) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
+    if (a >= cap) return;
+    const int n = (*err & 4) ? 0 : *n_sorted;
     int f = 0;
-    if (pos4[a].w > 0.0f) f = ((a - cell_start[cellid[a]]) & 63) == 0;
+    if (a < n && pos4[a].w > 0.0f) f = ((a - cell_start[cellid[a]]) & 63) == 0;
     flags[a] = f;
 }
 
@@ -302,10 +312,16 @@ static int location_maps(const Geom& g, const int* ires, const int* ixyz3, std::
 }
 
 // Cell base slots of the whole image (shared with PM3).
+int nzero_reserve() {
+    const Geom& g = ctx().g;
+    const int64_t rows = (int64_t)g.ntile * g.nte * g.nte;
+    return g_pp.nzero.reserve(std::max<int64_t>(rows * g.nte, 1) * sizeof(int64_t));
+}
+
 int compute_nzero(const int* d_rhoc, const int64_t* d_idx, const int64_t** out, hipStream_t s) {
     const Geom& g = ctx().g;
     const int64_t rows = (int64_t)g.ntile * g.nte * g.nte;
-    C2_TRY(g_pp.nzero.reserve(std::max<int64_t>(rows * g.nte, 1) * sizeof(int64_t)));
+    C2_TRY(nzero_reserve());
     if (rows > 0) {
         Stage st("nzero", (double)rows * g.nte * 12 + rows * 8);
         k_nzero<<<ceil_div(rows, 256), 256, 0, s>>>(d_rhoc, d_idx, g_pp.nzero.as<int64_t>(), g.nte, rows);
@@ -331,15 +347,18 @@ int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf) {
     return CUBE2_OK;
 }
 
-int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
-             const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
-             int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
-             double* ptotal, double* ttotal, uint64_t* pair_counts) {
+// Sorted-array total beyond capacity (idx_b_r rows overlapping): err |= 4,
+// which every later kernel of the call checks before touching the arrays.
+__global__ void k_pp_check_total(const int* __restrict__ total, int64_t cap, PPRed* red) {
+    if (threadIdx.x == 0 && (int64_t)*total > cap) red->err |= 4;
+}
+
+int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+               int64_t n_used, PPPlan& P) {
     ensure_params();
     Context& C = ctx();
     const Geom& g = C.g;
-    auto t0 = std::chrono::steady_clock::now();
-    if (!ires || !ixyz3 || !apm3 || !ratio_sf || !f2max || !vmax || !ptotal) {
+    if (!ires || !ixyz3 || !apm3 || !ratio_sf) {
         set_error("null argument");
         return CUBE2_ERR_ARG;
     }
@@ -347,171 +366,213 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
         set_error("invalid PP particle count %lld", (long long)n_used);
         return CUBE2_ERR_CAPACITY;
     }
-    std::vector<int> iloc, owner;
-    C2_TRY(location_maps(g, ires, ixyz3, iloc, owner));
+    C2_TRY(location_maps(g, ires, ixyz3, P.iloc, P.owner));
     for (int L = 2; L <= 6; L++)  // the levels ires can select (kick.f90:115-127)
         if (ratio_sf[L - 1] <= 0 || ratio_sf[L - 1] > 12) {
             set_error("ratio_sf[%d]=%d outside 1..12", L - 1, ratio_sf[L - 1]);
             return CUBE2_ERR_ARG;
         }
-    if (C.dv_state == 1 && C.dv_n_used != n_used) {
-        set_error("PM3/PP particle count changed within a kick");
-        return CUBE2_ERR_STATE;
-    }
     C2_TRY(ensure_init());
     hipStream_t s = C.stream;
     // per-tile grid (pp_force_kernel.cu:641-661)
-    const int ntile = g.ntile, nsub3 = g.p.nns * g.p.nns * g.p.nns;
+    P.ntile = g.ntile;
+    P.nsub3 = g.p.nns * g.p.nns * g.p.nns;
+    P.n_used = n_used;
+    P.mode = C.pp_mode;
     std::vector<PPTile>& T = g_pp.h_tiles;
-    T.assign(ntile, PPTile{});
+    T.assign(P.ntile, PPTile{});
     int64_t off = 0;
-    for (int t = 0; t < ntile; t++) {
+    for (int t = 0; t < P.ntile; t++) {
         int rcp = ratio_sf[1];
-        for (int q = 0; q < nsub3; q++) rcp = std::max(rcp, ratio_sf[iloc[t * nsub3 + q] - 1]);
+        for (int q = 0; q < P.nsub3; q++) rcp = std::max(rcp, ratio_sf[P.iloc[t * P.nsub3 + q] - 1]);
         T[t].rcp = rcp;
         T[t].S = g.nt * rcp + 2 * rcp;
         T[t].cell_off = off;
         off += (int64_t)T[t].S * T[t].S * T[t].S;
     }
-    const int64_t c_tot = off;
-    if (c_tot >= ((int64_t)1 << 32) - 1) {
-        set_error("PP cell grid too large (%lld cells)", (long long)c_tot);
+    P.c_tot = off;
+    if (P.c_tot >= ((int64_t)1 << 31) - 1) {
+        set_error("PP cell grid too large (%lld cells)", (long long)P.c_tot);
         return CUBE2_ERR_CAPACITY;
     }
-    const int64_t ncells_ext = (int64_t)ntile * (g.nt + 2) * (g.nt + 2) * (g.nt + 2);
-    const size_t ncap = (size_t)std::max<int64_t>(n_used, 1);
-    C2_TRY(g_pp.tiles.reserve(ntile * sizeof(PPTile)));
+    P.ncells_ext = (int64_t)P.ntile * (g.nt + 2) * (g.nt + 2) * (g.nt + 2);
+    P.cap = (size_t)std::max<int64_t>(n_used, 1);
+    C2_TRY(g_pp.tiles.reserve(P.ntile * sizeof(PPTile)));
     C2_TRY(g_pp.ires_loc.reserve(g.ns3 * sizeof(int)));
     C2_TRY(g_pp.apm3.reserve(8 * sizeof(float)));
-    C2_TRY(g_pp.counts.reserve((c_tot + 1) * sizeof(int)));
-    C2_TRY(g_pp.pos4.reserve(ncap * sizeof(float4)));
-    C2_TRY(g_pp.slot.reserve(ncap * sizeof(uint32_t)));
-    C2_TRY(g_pp.cellid.reserve(ncap * sizeof(uint32_t)));
+    C2_TRY(g_pp.counts.reserve((P.c_tot + 1) * sizeof(int)));
+    C2_TRY(g_pp.pos4.reserve(P.cap * sizeof(float4)));
+    C2_TRY(g_pp.slot.reserve(P.cap * sizeof(uint32_t)));
+    C2_TRY(g_pp.cellid.reserve(P.cap * sizeof(uint32_t)));
     C2_TRY(g_pp.red.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long)));
-    C2_TRY(g_pp.tile_first.reserve((ntile + 1) * sizeof(int)));
-    C2_HIP(hipMemcpyAsync(g_pp.tiles.ptr, T.data(), ntile * sizeof(PPTile), hipMemcpyHostToDevice, s));
-    C2_HIP(hipMemcpyAsync(g_pp.ires_loc.ptr, iloc.data(), g.ns3 * sizeof(int), hipMemcpyHostToDevice, s));
+    C2_TRY(g_pp.tile_first.reserve((P.ntile + 1) * sizeof(int)));
+    C2_TRY(g_pp.chunk_flags.reserve((P.cap + 1) * sizeof(int)));
+    C2_TRY(g_pp.chunks.reserve((P.cap + 1) * sizeof(int)));
+    C2_TRY(g_pp.scan_tmp.reserve(scan_tmp_ints(std::max<int64_t>(P.c_tot, (int64_t)P.cap)) * sizeof(int)));
+    C2_TRY(nzero_reserve());
+    C2_TRY(g_pp.h_res.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long) +
+                              (P.ntile + 1) * sizeof(int)));
+    C2_HIP(hipMemcpyAsync(g_pp.tiles.ptr, T.data(), P.ntile * sizeof(PPTile), hipMemcpyHostToDevice, s));
+    C2_HIP(hipMemcpyAsync(g_pp.ires_loc.ptr, P.iloc.data(), g.ns3 * sizeof(int), hipMemcpyHostToDevice, s));
     C2_HIP(hipMemcpyAsync(g_pp.apm3.ptr, apm3, 7 * sizeof(float), hipMemcpyHostToDevice, s));
+    C2_HIP(hipStreamSynchronize(s));  // host tables may go out of scope
+    buffers_changed();                 // the device tables now describe this call
+    P.grid_pairs = pp_tiled_grid(P.mode);
+    g_pp.valid = false;
+    return CUBE2_OK;
+}
+
+int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp,
+               float* d_vp, const float* d_scal, const double* dv64, hipStream_t s) {
+    const Geom& g = ctx().g;
     PPRed* red = g_pp.red.as<PPRed>();
     unsigned long long* ploc = reinterpret_cast<unsigned long long*>(red + 1);
-    C2_HIP(hipMemsetAsync(red, 0, sizeof(PPRed) + g.ns3 * sizeof(unsigned long long), s));
-    g_pp.valid = false;
+    int* counts = g_pp.counts.as<int>();
+    const int* n_sorted = counts + P.c_tot;
+    C2_TRY(fill_zero(red, sizeof(PPRed) + g.ns3 * sizeof(unsigned long long), s));
     const int64_t* nzp = nullptr;
     C2_TRY(compute_nzero(d_rhoc, d_idx, &nzp, s));
-    if (ncells_ext > 0) {
-        Stage st("pp_count", (double)n_used * 6 + c_tot * 4.0);
-        k_pp_count<<<ceil_div(ncells_ext, kWaves), 256, 0, s>>>(
-            d_rhoc, g_pp.nzero.as<int64_t>(), d_xp, g_pp.tiles.as<PPTile>(), g.nt, g.ncb, g.nte,
-            ncells_ext, n_used, g_pp.counts.as<int>(), &red->err);
+    if (P.ncells_ext > 0) {
+        Stage st("pp_count", (double)P.n_used * 6 + P.c_tot * 4.0);
+        k_pp_count<<<ceil_div(P.ncells_ext, kWaves), 256, 0, s>>>(
+            d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g.nt, g.ncb, g.nte, P.ncells_ext,
+            P.n_used, counts, &red->err);
         C2_HIP(hipGetLastError());
     }
     {
-        Stage st("pp_scan", (double)c_tot * 8);
-        C2_TRY(exclusive_scan(g_pp.counts.as<int>(), c_tot, g_pp.scan_tmp, s));
+        Stage st("pp_scan", (double)P.c_tot * 8);
+        C2_TRY(exclusive_scan(counts, P.c_tot, g_pp.scan_tmp, s));
+        k_pp_check_total<<<1, 64, 0, s>>>(n_sorted, (int64_t)P.cap, red);
+        C2_HIP(hipGetLastError());
     }
-    int n_tot = 0, err = 0;
-    C2_HIP(hipMemcpyAsync(&n_tot, g_pp.counts.as<int>() + c_tot, sizeof(int), hipMemcpyDeviceToHost, s));
-    C2_HIP(hipMemcpyAsync(&err, &red->err, sizeof(int), hipMemcpyDeviceToHost, s));
-    C2_HIP(hipStreamSynchronize(s));
-    if (err & 2) {
-        set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)n_used);
-        return CUBE2_ERR_ARG;
-    }
-    if (err) {
-        set_error("a particle's PP key fell outside its coarse cell (geometry too large for FP32 keys)");
-        return CUBE2_ERR_CAPACITY;
-    }
-    if (n_tot < 0 || n_tot > n_used) {
-        set_error("cell lists hold %d particles, more than n_used=%lld", n_tot, (long long)n_used);
-        return CUBE2_ERR_ARG;
-    }
-    if (ncells_ext > 0) {
-        Stage st("pp_scatter", (double)n_tot * (6 + 24));
-        k_pp_scatter<<<ceil_div(ncells_ext, kWaves), 256, 0, s>>>(
-            d_rhoc, g_pp.nzero.as<int64_t>(), d_xp, g_pp.tiles.as<PPTile>(), g_pp.ires_loc.as<int>(),
+    if (P.ncells_ext > 0) {
+        Stage st("pp_scatter", (double)P.n_used * (6 + 24));
+        k_pp_scatter<<<ceil_div(P.ncells_ext, kWaves), 256, 0, s>>>(
+            d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g_pp.ires_loc.as<int>(),
             g_pp.apm3.as<float>(), g.nt, g.ncb, g.nte, g.p.nnt, g.p.nns, g.ntt, g.p.ratio_cs,
-            ncells_ext, g_pp.counts.as<int>(), g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>(),
-            g_pp.cellid.as<uint32_t>());
+            P.ncells_ext, P.n_used, counts, &red->err, g_pp.pos4.as<float4>(),
+            g_pp.slot.as<uint32_t>(), g_pp.cellid.as<uint32_t>());
         C2_HIP(hipGetLastError());
     }
     {
-        const int64_t nphys = (int64_t)ntile * g.nt * g.nt * g.nt;
+        const int64_t nphys = (int64_t)P.ntile * g.nt * g.nt * g.nt;
         if (nphys > 0)
             k_pp_ptotal<<<ceil_div(nphys, 256), 256, 0, s>>>(d_rhoc, g.nt, g.ncb, g.nte, g.p.nnt,
                                                               g.p.nns, g.ntt, ploc);
         C2_HIP(hipGetLastError());
-        k_tile_first<<<ceil_div(ntile + 1, 64), 64, 0, s>>>(g_pp.counts.as<int>(), g_pp.tiles.as<PPTile>(),
-                                                            ntile, c_tot, g_pp.tile_first.as<int>());
+        k_tile_first<<<ceil_div(P.ntile + 1, 64), 64, 0, s>>>(counts, g_pp.tiles.as<PPTile>(),
+                                                              P.ntile, P.c_tot,
+                                                              g_pp.tile_first.as<int>());
         C2_HIP(hipGetLastError());
     }
-    const double* dv = (C.dv_state == 1) ? C.dv64.as<double>() : nullptr;
     PPArgs A;
     A.pos4 = g_pp.pos4.as<float4>();
     A.slot = g_pp.slot.as<uint32_t>();
     A.cellid = g_pp.cellid.as<uint32_t>();
-    A.cell_start = g_pp.counts.as<int>();
+    A.cell_start = counts;
     A.tiles = g_pp.tiles.as<PPTile>();
-    A.ntile = ntile;
-    A.n = n_tot;
+    A.ntile = P.ntile;
+    A.n_sorted = n_sorted;
+    A.cap = (int)P.cap;
     A.ratio_cs = g.p.ratio_cs;
     A.app = (float)g.p.app;
-    A.mass = mass;
-    A.a_mid = a_mid;
-    A.dt = dt;
-    A.dv64 = dv;
+    A.scal = d_scal;
+    A.dv64 = dv64;
     A.vp = d_vp;
     A.red = red;
-    if (n_tot > 0) {
-        if (C.pp_mode == CUBE2_PP_EXACT) {
-            C2_TRY(launch_pp_pairs(A, C.pp_mode, s));
-        } else {
-            C2_TRY(g_pp.chunk_flags.reserve(((size_t)n_tot + 1) * sizeof(int)));
-            C2_TRY(g_pp.chunks.reserve(((size_t)n_tot + 1) * sizeof(int)));
-            int* flags = g_pp.chunk_flags.as<int>();
-            int nchunks = 0;
-            {
-                Stage st("pp_chunks", (double)n_tot * 24);
-                k_chunk_flags<<<ceil_div(n_tot, 256), 256, 0, s>>>(A.pos4, A.cellid, A.cell_start,
-                                                                   n_tot, flags);
-                C2_HIP(hipGetLastError());
-                C2_TRY(exclusive_scan(flags, n_tot, g_pp.scan_tmp, s));
-                k_chunk_emit<<<ceil_div(n_tot, 256), 256, 0, s>>>(flags, n_tot, g_pp.chunks.as<int>());
-                C2_HIP(hipGetLastError());
-            }
-            C2_HIP(hipMemcpyAsync(&nchunks, flags + n_tot, sizeof(int), hipMemcpyDeviceToHost, s));
-            C2_HIP(hipStreamSynchronize(s));
-            C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), nchunks, C.pp_mode, s));
+    if (P.mode == CUBE2_PP_EXACT) {
+        C2_TRY(launch_pp_pairs(A, P.mode, s));
+    } else {
+        int* flags = g_pp.chunk_flags.as<int>();
+        {
+            Stage st("pp_chunks", (double)P.n_used * 24);
+            k_chunk_flags<<<ceil_div(P.cap, 256), 256, 0, s>>>(A.pos4, A.cellid, A.cell_start,
+                                                                n_sorted, &red->err, (int)P.cap,
+                                                                flags);
+            C2_HIP(hipGetLastError());
+            C2_TRY(exclusive_scan(flags, (int64_t)P.cap, g_pp.scan_tmp, s));
+            k_chunk_emit<<<ceil_div(P.cap, 256), 256, 0, s>>>(flags, (int)P.cap, g_pp.chunks.as<int>());
+            C2_HIP(hipGetLastError());
         }
+        C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), flags + P.cap, P.grid_pairs, P.mode, s));
     }
-    // results
-    PPRed h;
-    std::vector<unsigned long long> hp(g.ns3);
-    std::vector<int> tf(ntile + 1);
-    C2_HIP(hipMemcpyAsync(&h, red, sizeof(PPRed), hipMemcpyDeviceToHost, s));
-    C2_HIP(hipMemcpyAsync(hp.data(), ploc, g.ns3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    C2_HIP(hipMemcpyAsync(tf.data(), g_pp.tile_first.ptr, (ntile + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-    C2_TRY(sync_and_flush());
-    *f2max = __builtin_bit_cast(float, h.f2max_bits);
-    for (int d = 0; d < 3; d++) vmax[d] = __builtin_bit_cast(float, h.vmax_bits[d]);
+    // results into pinned host memory
+    char* h = g_pp.h_res.as<char>();
+    C2_HIP(hipMemcpyAsync(h, red, sizeof(PPRed) + g.ns3 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(h + sizeof(PPRed) + g.ns3 * sizeof(unsigned long long),
+                          g_pp.tile_first.ptr, (P.ntile + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    return CUBE2_OK;
+}
+
+int pp_finish(const PPPlan& P, float* f2max, float* vmax, double* ptotal, double* ttotal,
+              uint64_t* pair_counts, double wall_s) {
+    const Geom& g = ctx().g;
+    const char* h = g_pp.h_res.as<const char>();
+    PPRed r;
+    std::memcpy(&r, h, sizeof(PPRed));
+    const unsigned long long* hp = reinterpret_cast<const unsigned long long*>(h + sizeof(PPRed));
+    const int* tf = reinterpret_cast<const int*>(h + sizeof(PPRed) + g.ns3 * sizeof(unsigned long long));
+    if (r.err & 2) {
+        set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)P.n_used);
+        return CUBE2_ERR_ARG;
+    }
+    if (r.err & 4) {
+        set_error("cell lists hold more particles than n_used=%lld (overlapping idx_b_r rows)",
+                  (long long)P.n_used);
+        return CUBE2_ERR_ARG;
+    }
+    if (r.err & 1) {
+        set_error("a particle's PP key fell outside its coarse cell (geometry too large for FP32 keys)");
+        return CUBE2_ERR_CAPACITY;
+    }
+    const int n_tot = tf[P.ntile] - tf[0];
+    *f2max = __builtin_bit_cast(float, r.f2max_bits);
+    for (int d = 0; d < 3; d++) vmax[d] = __builtin_bit_cast(float, r.vmax_bits[d]);
     for (int it = 0; it < g.ns3; it++) ptotal[it] = 0.0;
-    for (int loc = 0; loc < g.ns3; loc++) ptotal[owner[loc]] += (double)hp[loc];
+    for (int loc = 0; loc < g.ns3; loc++) ptotal[P.owner[loc]] += (double)hp[loc];
     if (pair_counts) {
-        pair_counts[0] = h.p_cand;
-        pair_counts[1] = h.p_acc;
+        pair_counts[0] = r.p_cand;
+        pair_counts[1] = r.p_acc;
     }
-    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (ttotal) {
         for (int it = 0; it < g.ns3; it++) ttotal[it] = 0.0;
-        for (int t = 0; t < ntile; t++) {
-            double share = n_tot > 0 ? (double)(tf[t + 1] - tf[t]) / n_tot : 0.0;
-            ttotal[owner[t * nsub3]] = wall * share;
+        for (int t = 0; t < P.ntile; t++) {
+            const double share = n_tot > 0 ? (double)(tf[t + 1] - tf[t]) / n_tot : 0.0;
+            ttotal[P.owner[t * P.nsub3]] = wall_s * share;
         }
     }
-    g_pp.c_tot = c_tot;
+    g_pp.c_tot = P.c_tot;
     g_pp.n_tot = n_tot;
     g_pp.valid = true;
-    C.dv_state = 0;
     return CUBE2_OK;
+}
+
+int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
+             const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
+             int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
+             double* ptotal, double* ttotal, uint64_t* pair_counts) {
+    ensure_params();
+    Context& C = ctx();
+    auto t0 = std::chrono::steady_clock::now();
+    if (!f2max || !vmax || !ptotal) {
+        set_error("null argument");
+        return CUBE2_ERR_ARG;
+    }
+    if (C.dv_state == 1 && C.dv_n_used != n_used) {
+        set_error("PM3/PP particle count changed within a kick");
+        return CUBE2_ERR_STATE;
+    }
+    PPPlan P;
+    C2_TRY(pp_prepare(ires, ixyz3, apm3, ratio_sf, n_used, P));
+    const double* dv = (C.dv_state == 1) ? C.dv64.as<double>() : nullptr;
+    const float* d_scal = nullptr;
+    C2_TRY(scalars_upload(mass, a_mid, dt, C.stream, &d_scal));
+    C2_TRY(pp_enqueue(P, d_rhoc, d_idx, d_xp, d_vp, d_scal, dv, C.stream));
+    C2_TRY(sync_and_flush());
+    C.dv_state = 0;
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return pp_finish(P, f2max, vmax, ptotal, ttotal, pair_counts, wall);
 }
 
 int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t* slot,

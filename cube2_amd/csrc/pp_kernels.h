@@ -24,8 +24,10 @@ struct PPArgs {
     const uint32_t* cellid;    // sorted -> global cell (tile offset + key)
     const int* cell_start;     // c_tot + 1 exclusive starts
     const PPTile* tiles;
-    int ntile, n, ratio_cs;
-    float app, mass, a_mid, dt;
+    const int* n_sorted;       // particles in the cell lists (device, from the scan)
+    int ntile, cap, ratio_cs;  // cap: capacity of the sorted arrays
+    float app;
+    const float* scal;         // {mass, a_mid, dt} (device)
     const double* dv64;        // pending PM3 Delta-v (FP64) or nullptr
     float* vp;
     PPRed* red;

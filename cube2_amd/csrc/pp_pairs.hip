@@ -62,16 +62,17 @@ __device__ inline float fra_fast(float ep, float ir2, float c) {
 __device__ inline void pp_update(const PPArgs& A, uint32_t a, float ax, float ay, float az,
                                  float& f2, float& v0, float& v1, float& v2) {
 #pragma clang fp contract(off)
-    const float a0 = ax * A.mass * A.mass;
-    const float a1 = ay * A.mass * A.mass;
-    const float a2 = az * A.mass * A.mass;
+    const float mass = A.scal[0], a_mid = A.scal[1], dt = A.scal[2];
+    const float a0 = ax * mass * mass;
+    const float a1 = ay * mass * mass;
+    const float a2 = az * mass * mass;
     f2 = a0 * a0 + a1 * a1 + a2 * a2;
     const size_t ip = (size_t)A.slot[a] * 3;
     const float acc[3] = {a0, a1, a2};
     float nv[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const double term = (double)(acc[q] * A.a_mid * A.dt / 6.0f) / 3.14159265358979323846;
+        const double term = (double)(acc[q] * a_mid * dt / 6.0f) / 3.14159265358979323846;
         double v = (double)A.vp[ip + q];
         if (A.dv64) v += A.dv64[ip + q];
         nv[q] = (float)(v + term);
@@ -92,11 +93,13 @@ __device__ inline void pp_reduce(const PPArgs& A, float f2, float v0, float v1, 
 
 template <bool kExact>
 __global__ __launch_bounds__(256) void k_pp_pairs(PPArgs A) {
+    if (A.red->err & 4) return;  // failed call: the sorted arrays are not valid
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)*A.n_sorted;
     float f2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f;
     unsigned long long pc = 0, pa = 0;
     float4 p = make_float4(0, 0, 0, 0);
-    if (a < (uint32_t)A.n) p = A.pos4[a];
+    if (a < n) p = A.pos4[a];
     if (p.w > 0.0f) {
         const uint32_t cid = A.cellid[a];
         const PPTile T = A.tiles[find_tile(A.tiles, A.ntile, cid)];
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(256) void k_pp_pairs(PPArgs A) {
 
 int launch_pp_pairs(const PPArgs& a, int mode, hipStream_t s) {
     Stage st(mode == CUBE2_PP_EXACT ? "pp_pairs_exact" : "pp_pairs_fast");
-    const unsigned blocks = ceil_div((size_t)a.n, 256);
+    const unsigned blocks = ceil_div((size_t)a.cap, 256);  // n_sorted <= cap, read on the device
     if (mode == CUBE2_PP_EXACT) k_pp_pairs<true><<<blocks, 256, 0, s>>>(a);
     else k_pp_pairs<false><<<blocks, 256, 0, s>>>(a);
     C2_HIP(hipGetLastError());

@@ -28,6 +28,7 @@
 //   without the term-by-term cancellation near the cutoff; sums of pair
 //   terms accumulate in FP64 (full rate on MI355X) instead of Kahan-
 //   compensated FP32.
+#include <algorithm>
 #include <type_traits>
 
 #include "pp_kernels.h"
@@ -177,16 +178,18 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
     return lo;
 }
 
+// Persistent: wavefront w of the grid takes chunks w, w + W, w + 2W, ...
+// (W = waves in the grid); the chunk count is read on the device.
 template <int kMode>
 __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
-                                                  int nchunks) {
+                                                  const int* __restrict__ nchunks_dev) {
     __shared__ WaveLDS<kMode> lds[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveLDS<kMode>& L = lds[wave];
-    const int ck = blockIdx.x * 4 + wave;
+    const int nchunks = *nchunks_dev;
     float f2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f;
     unsigned long long pc = 0, pa = 0;
-    if (ck < nchunks) {
+    for (int ck = blockIdx.x * 4 + wave; ck < nchunks; ck += gridDim.x * 4) {
         const int a0 = chunks[ck];
         const uint32_t cid = A.cellid[a0];
         const int cend = A.cell_start[cid + 1];
@@ -267,31 +270,32 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
             }
         }
         if (qtail > qhead) flush<kMode>(L, lane, qhead, qtail - qhead, Ka, Kr);
-        pa = (lane == 0) ? (unsigned long long)qtail : 0ull;
+        if (lane == 0) pa += (unsigned long long)qtail;
         if (vi && jslot == 0) {
 #pragma clang fp contract(off)
             pc -= 1;  // the particle itself
             const int a = a0 + islot;
             const float ax = (float)L.acc[islot][0], ay = (float)L.acc[islot][1],
                         az = (float)L.acc[islot][2];
-            const float a0m = ax * A.mass * A.mass, a1m = ay * A.mass * A.mass,
-                        a2m = az * A.mass * A.mass;
-            f2 = a0m * a0m + a1m * a1m + a2m * a2m;
+            const float mass = A.scal[0], a_mid = A.scal[1], dt = A.scal[2];
+            const float a0m = ax * mass * mass, a1m = ay * mass * mass, a2m = az * mass * mass;
+            f2 = fmaxf(f2, a0m * a0m + a1m * a1m + a2m * a2m);
             const size_t ip = (size_t)A.slot[a] * 3;
             const float acc[3] = {a0m, a1m, a2m};
             float nv[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const double term = (double)(acc[c] * A.a_mid * A.dt / 6.0f) / 3.14159265358979323846;
+                const double term = (double)(acc[c] * a_mid * dt / 6.0f) / 3.14159265358979323846;
                 double v = (double)A.vp[ip + c];
                 if (A.dv64) v += A.dv64[ip + c];
                 nv[c] = (float)(v + term);
                 A.vp[ip + c] = nv[c];
             }
-            v0 = fabsf(nv[0]);
-            v1 = fabsf(nv[1]);
-            v2 = fabsf(nv[2]);
+            v0 = fmaxf(v0, fabsf(nv[0]));
+            v1 = fmaxf(v1, fabsf(nv[1]));
+            v2 = fmaxf(v2, fabsf(nv[2]));
         }
+        wave_lds_sync();  // acc and the queue are reused by the next chunk
     }
     const float mx[4] = {f2, v0, v1, v2};
     block_max_to<4>(&A.red->f2max_bits, mx);
@@ -299,17 +303,38 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
     block_sum_to<2>(&A.red->p_cand, sm);
 }
 
-int launch_pp_tiled(const PPArgs& a, const int* chunks, int nchunks, int mode, hipStream_t s) {
+// Blocks of the persistent pair kernel: the resident maximum over all CUs
+// (occupancy query, cached per mode).
+int pp_tiled_grid(int mode) {
+    static int cache[4] = {0, 0, 0, 0};
+    const int m = mode & 3;
+    if (cache[m]) return cache[m];
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    hipError_t e = hipErrorInvalidValue;
+    if (mode == CUBE2_PP_REF)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModeRef>, 256, 0);
+    else if (mode == CUBE2_PP_FAST)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModeFast>, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModePrecise>, 256, 0);
+    if (e != hipSuccess || per < 1) per = 2;
+    cache[m] = std::max(1, ncu) * per;
+    return cache[m];
+}
+
+int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int grid, int mode,
+                    hipStream_t s) {
     const char* name = mode == CUBE2_PP_REF ? "pp_pairs_ref"
                        : mode == CUBE2_PP_FAST ? "pp_pairs_fastq" : "pp_pairs_precise";
     Stage st(name);
-    const unsigned blocks = ceil_div((size_t)nchunks, 4);
-    if (blocks) {
-        if (mode == CUBE2_PP_REF) k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
-        else if (mode == CUBE2_PP_FAST) k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
-        else k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
-        C2_HIP(hipGetLastError());
-    }
+    const unsigned blocks = (unsigned)std::max(1, grid);
+    if (mode == CUBE2_PP_REF) k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+    else if (mode == CUBE2_PP_FAST) k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+    else k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+    C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
 

@@ -99,11 +99,23 @@ size_t scan_tmp_ints(int64_t n) {
     return t + 16;
 }
 
+__global__ void k_fill_zero(uint32_t* __restrict__ d, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = 0u;
+}
+
+// Zeroes bytes (a multiple of 4) with a kernel: recordable into HIP graphs,
+// where memset nodes are avoided.
+int fill_zero(void* d, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 4;
+    if (!n) return CUBE2_OK;
+    k_fill_zero<<<ceil_div(n, 256), 256, 0, s>>>(static_cast<uint32_t*>(d), n);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
 int exclusive_scan(int* d, int64_t n, DevBuf& tmp, hipStream_t s) {
-    if (n <= 0) {
-        C2_HIP(hipMemsetAsync(d, 0, sizeof(int), s));
-        return CUBE2_OK;
-    }
+    if (n <= 0) return fill_zero(d, sizeof(int), s);
     C2_TRY(tmp.reserve(scan_tmp_ints(n) * sizeof(int)));
     return scan_rec(d, n, tmp.as<int>(), s);
 }

@@ -50,6 +50,7 @@ class KickResult:
     p_cand: int = 0
     p_acc: int = 0
     dt_native: dict | None = None   # dt limits as the library's kick driver computed them
+    recorded: bool = False          # kick_graph: this call recorded the graph
 
     def dt_limits(self, a_mid: float) -> dict:
         """dt_pm3, dt_pp, dt_vmax in FP64 (kick.f90:180,227,238)."""
@@ -171,6 +172,31 @@ class ShortRangeKick:
                          ttotal_pp=ttp if pp else None, p_cand=int(out.pair_counts[0]),
                          p_acc=int(out.pair_counts[1]))
         res.dt_native = dict(dt_pm3=out.dt_pm3, dt_pp=out.dt_pp, dt_vmax=out.dt_vmax)
+        return res
+
+    def kick_graph(self, mass: float, a_mid: float, dt: float, pm3: bool = True) -> KickResult:
+        """The same kick through cube2_kick_graph_dev: recorded into a HIP
+        graph on the first call (res.recorded = True) and replayed while the
+        levels, arrays and sizes stay the same."""
+        L = _lib.lib()
+        _lib.set_params(self.geom)
+        flags = (_lib.KICK_PM3 if pm3 else 0) | _lib.KICK_PP
+        ns3 = self.geom.ns3
+        pt3, ptp, ttp = np.zeros(ns3), np.zeros(ns3), np.zeros(ns3)
+        out = _lib.KickResultC()
+        rec = C.c_int(0)
+        _lib.check(L.cube2_kick_graph_dev(
+            flags, _np_ptr(self.ix), _np_ptr(APM3), _np_ptr(RATIO_SF), self.d_rhoc.data_ptr(),
+            self.d_idx.data_ptr(), self.d_xp.data_ptr(), self.d_vp.data_ptr(), self.n_used,
+            self._green_ptrs() if pm3 else None, float(mass), float(a_mid), float(dt),
+            _np_ptr(self.ires), _np_ptr(self.isort), _np_ptr(pt3), _np_ptr(ptp), _np_ptr(ttp),
+            C.byref(out), C.addressof(rec)))
+        res = KickResult(f2max_pm3=out.f2max_pm3, f2max_pp=out.f2max_pp,
+                         vmax=np.array(out.vmax[:], np.float32),
+                         ptotal_pm3=pt3 if pm3 else None, ptotal_pp=ptp, ttotal_pp=ttp,
+                         p_cand=int(out.pair_counts[0]), p_acc=int(out.pair_counts[1]))
+        res.dt_native = dict(dt_pm3=out.dt_pm3, dt_pp=out.dt_pp, dt_vmax=out.dt_vmax)
+        res.recorded = bool(rec.value)
         return res
 
     def vp_host(self) -> np.ndarray:

@@ -165,6 +165,21 @@ int cube2_kick_dev(int flags, const int* ixyz3, const float* apm3, const int* ra
                    float mass, float a_mid, float dt, int* ires, int* isort, double* ptotal_pm3,
                    double* ptotal_pp, double* ttotal_pp, cube2_kick_result_t* out);
 
+/* The same kick with the caller's levels (flags CUBE2_KICK_PM3|CUBE2_KICK_PP
+ * or CUBE2_KICK_PP), recorded into a HIP graph on the first call and
+ * replayed while the levels, job order, arrays, sizes, parameters and PP
+ * mode stay the same (mass, a_mid and dt may change between calls).  Any
+ * other library call that grows buffers or uploads tables makes the next
+ * call record again.  *recorded (may be NULL) = 1 when this call recorded.
+ * Fails with CUBE2_ERR_STATE while a PM3 Delta-v from cube2_pm3_force_dev
+ * is pending. */
+int cube2_kick_graph_dev(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf,
+                         const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                         float* d_vp, int64_t n_used, const float* const* d_green, float mass,
+                         float a_mid, float dt, const int* ires, const int* isort,
+                         double* ptotal_pm3, double* ptotal_pp, double* ttotal_pp,
+                         cube2_kick_result_t* out, int* recorded);
+
 /* ---------------------------------------------------------------------
  * Inspection (tests and benchmarks)
  * ------------------------------------------------------------------- */

@@ -132,3 +132,59 @@ def test_kick_driver_selects_levels_on_device(lib, mixed_layout):
     assert np.array_equal(k.isort, h_isort)
     assert np.array_equal(k.napm, h_napm)
     assert res.p_acc > 0 and res.f2max_pm3 > 0
+
+
+def test_recorded_kick_replays_match_eager(lib, mixed_layout):
+    """cube2_kick_graph_dev: the first call records, later calls replay.
+    Every replay (with host work and synchronisation between replays, and
+    with a changed dt) equals the eager kick: PP alone bit for bit, PM3 + PP
+    within 1e-6; an eager library call in between forces a new recording."""
+    import torch
+    from cube2_amd.kick import ShortRangeKick
+    lay, ires = mixed_layout
+    phys = lay.physical_mask()
+    mass, amid = 1.3, 0.05
+    k = ShortRangeKick(lay)
+    k.set_levels(ires)
+    vp0 = k.d_vp.clone()
+
+    def reset():
+        k.d_vp.copy_(vp0)
+        torch.cuda.synchronize()
+
+    def eager(dt, pm3):
+        reset()
+        r = k.kick(mass, amid, dt, pm3=pm3, pp=True)
+        return k.vp_host(), r
+
+    for pm3 in (False, True):
+        want, rw = eager(2e-3, pm3)
+        want2, rw2 = eager(3e-3, pm3)
+        seen = []
+        for rep, dt in enumerate([2e-3, 2e-3, 3e-3, 2e-3]):
+            reset()
+            x = torch.randn(1 << 20, device="cuda")      # eager work between replays
+            float((x * x).sum())
+            r = k.kick_graph(mass, amid, dt, pm3=pm3)
+            seen.append(r.recorded)
+            got = k.vp_host()
+            ref, rref = (want, rw) if dt == 2e-3 else (want2, rw2)
+            if pm3:
+                err = float(np.sqrt((((got - ref)[phys]) ** 2).sum() / ((ref - lay.vp)[phys] ** 2).sum()))
+                print(f"pm3+pp replay {rep} (dt {dt}): Delta-v relative RMS vs eager {err:.2e}")
+                assert err <= 1e-6
+                assert r.f2max_pm3 == pytest.approx(rref.f2max_pm3, rel=1e-5)
+                assert np.array_equal(r.ptotal_pm3, rref.ptotal_pm3)
+            else:
+                assert np.array_equal(got, ref), f"PP replay {rep} differs"
+                assert r.f2max_pp == rref.f2max_pp and np.array_equal(r.vmax, rref.vmax)
+            assert (r.p_cand, r.p_acc) == (rref.p_cand, rref.p_acc)
+            assert np.array_equal(r.ptotal_pp, rref.ptotal_pp)
+            assert r.dt_native["dt_pp"] == pytest.approx(rref.dt_native["dt_pp"], rel=1e-6)
+        assert seen == [True, False, False, False]
+        eager(2e-3, pm3)                                   # an eager library call in between
+        reset()
+        r = k.kick_graph(mass, amid, 2e-3, pm3=pm3)
+        assert r.recorded
+        if not pm3:
+            assert np.array_equal(k.vp_host(), want)
