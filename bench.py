@@ -95,6 +95,18 @@ def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s
                         f"OpenMP threads {ncpu}"))
 
 
+def profiled_traffic(stage: str):
+    """HBM bytes per launch of a stage's kernel at this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (None if absent)."""
+    f = ROOT / "profiles" / f"{stage}_c1_traffic.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def copy_probe(mib: int) -> int:
     """Three device-to-device copies of a known size, for scaling the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
@@ -212,7 +224,7 @@ def main(argv=None) -> int:
         achieved = flops / (pp_ms * 1e-3) / 1e12
         roof = dict(kernel=name, bound="valu", achieved=round(achieved, 3),
                     peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None,
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=profiled_traffic(name),
                     avg_launch_ms=round(per_launch_ms, 3))
     else:
         achieved = st["bytes"] / (st["ms"] * 1e-3) / 1e9

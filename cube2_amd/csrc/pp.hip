@@ -39,7 +39,7 @@ int pp_tiled_grid(int mode);
 
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
-    DevBuf chunk_flags, chunks;
+    DevBuf chunk_flags, chunks, ilist;
     HostBuf h_res;  // PPRed, per-location counts, tile_first (pinned)
     std::vector<PPTile> h_tiles;
     int64_t c_tot = 0;
@@ -280,6 +280,82 @@ __global__ void k_chunk_emit(const int* __restrict__ scanned, int n, int* __rest
     if (scanned[a + 1] != scanned[a]) chunks[scanned[a]] = a;
 }
 
+// Spreads the low 3 bits of v to bits 0, 3, 6 (Morton interleave).
+__device__ inline int spread3(int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4); }
+
+// One wavefront per PP cell of at least 64 particles (found as its first
+// chunk): ilist[cell range] = the cell's sorted positions ordered by an
+// 8x8x8 Morton sub-cell key (counting sort in LDS), so that consecutive
+// 64-particle chunks of a dense cell are spatially compact.  The order of
+// the cell list itself (the j stream) is not changed.
+__global__ __launch_bounds__(256) void k_pp_subsort(
+    const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
+    const int* __restrict__ cell_start, const PPTile* __restrict__ tiles, int ntile,
+    const int* __restrict__ chunks, const int* __restrict__ nchunks_dev, int ratio_cs,
+    int* __restrict__ ilist) {
+    __shared__ int bins[kWaves][512];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nchunks = *nchunks_dev;
+    // grid-stride over chunks; no block-level barrier in this kernel
+    for (int ck = blockIdx.x * kWaves + wave; ck < nchunks; ck += gridDim.x * kWaves) {
+    const int a0 = chunks[ck];
+    const uint32_t cid = cellid[a0];
+    const int c0 = cell_start[cid], c1 = cell_start[cid + 1];
+    if (a0 != c0 || c1 - c0 < 64) continue;
+    int lo = 0, hi = ntile - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tiles[mid].cell_off <= (long long)cid) lo = mid;
+        else hi = mid - 1;
+    }
+    const PPTile T = tiles[lo];
+    const long long key = (long long)cid - T.cell_off;
+    const int S = T.S;
+    const int kx = (int)(key % S), ky = (int)((key / S) % S), kz = (int)(key / ((long long)S * S));
+    const float size = (float)ratio_cs / (float)T.rcp;   // cell edge in pos4 units
+    const float sx = (float)(kx - T.rcp) * size, sy = (float)(ky - T.rcp) * size,
+                sz = (float)(kz - T.rcp) * size;
+    const float inv = 8.0f / size;
+    int* b = bins[wave];
+    for (int q = lane; q < 512; q += 64) b[q] = 0;
+    __builtin_amdgcn_wave_barrier();
+    auto subkey = [&](int a) {
+        const float4 v = pos4[a];
+        const int ix = min(max((int)((v.x - sx) * inv), 0), 7);
+        const int iy = min(max((int)((v.y - sy) * inv), 0), 7);
+        const int iz = min(max((int)((v.z - sz) * inv), 0), 7);
+        return spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
+    };
+    for (int a = c0 + lane; a < c1; a += 64) atomicAdd(&b[subkey(a)], 1);
+    __builtin_amdgcn_wave_barrier();
+    // exclusive scan of the 512 bins: 8 consecutive bins per lane
+    int loc[8], s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        loc[q] = b[lane * 8 + q];
+        s += loc[q];
+    }
+    int inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    int run = inc - s;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        b[lane * 8 + q] = run;
+        run += loc[q];
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int a = c0 + lane; a < c1; a += 64) {
+        const int pos = atomicAdd(&b[subkey(a)], 1);
+        ilist[c0 + pos] = a;
+    }
+    __builtin_amdgcn_wave_barrier();  // bins are cleared for the next cell
+    }
+}
+
 // First sorted index of every tile (for per-tile counts).
 __global__ void k_tile_first(const int* __restrict__ cell_start, const PPTile* __restrict__ tiles,
                              int ntile, int64_t c_tot, int* __restrict__ out) {
@@ -408,6 +484,7 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.tile_first.reserve((P.ntile + 1) * sizeof(int)));
     C2_TRY(g_pp.chunk_flags.reserve((P.cap + 1) * sizeof(int)));
     C2_TRY(g_pp.chunks.reserve((P.cap + 1) * sizeof(int)));
+    C2_TRY(g_pp.ilist.reserve(P.cap * sizeof(int)));
     C2_TRY(g_pp.scan_tmp.reserve(scan_tmp_ints(std::max<int64_t>(P.c_tot, (int64_t)P.cap)) * sizeof(int)));
     C2_TRY(nzero_reserve());
     C2_TRY(g_pp.h_res.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long) +
@@ -470,6 +547,7 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     A.slot = g_pp.slot.as<uint32_t>();
     A.cellid = g_pp.cellid.as<uint32_t>();
     A.cell_start = counts;
+    A.ilist = g_pp.ilist.as<int>();
     A.tiles = g_pp.tiles.as<PPTile>();
     A.ntile = P.ntile;
     A.n_sorted = n_sorted;
@@ -492,6 +570,14 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             C2_HIP(hipGetLastError());
             C2_TRY(exclusive_scan(flags, (int64_t)P.cap, g_pp.scan_tmp, s));
             k_chunk_emit<<<ceil_div(P.cap, 256), 256, 0, s>>>(flags, (int)P.cap, g_pp.chunks.as<int>());
+            C2_HIP(hipGetLastError());
+        }
+        {
+            Stage st("pp_subsort", (double)P.n_used * 24);
+            // one wavefront per chunk (only first chunks of dense cells work)
+            k_pp_subsort<<<(unsigned)std::max(1, P.grid_pairs), 256, 0, s>>>(
+                A.pos4, A.cellid, A.cell_start, A.tiles, P.ntile, g_pp.chunks.as<int>(),
+                flags + P.cap, g.p.ratio_cs, g_pp.ilist.as<int>());
             C2_HIP(hipGetLastError());
         }
         C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), flags + P.cap, P.grid_pairs, P.mode, s));
@@ -610,7 +696,8 @@ int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t
 void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
-                      &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks})
+                      &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks,
+                      &p.ilist})
         b->release();
     p.valid = false;
 }

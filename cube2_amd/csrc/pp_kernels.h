@@ -23,6 +23,7 @@ struct PPArgs {
     const uint32_t* slot;      // sorted -> xp/vp slot
     const uint32_t* cellid;    // sorted -> global cell (tile offset + key)
     const int* cell_start;     // c_tot + 1 exclusive starts
+    const int* ilist;          // cells of >= 64 particles: their positions in sub-cell order
     const PPTile* tiles;
     const int* n_sorted;       // particles in the cell lists (device, from the scan)
     int ntile, cap, ratio_cs;  // cap: capacity of the sorted arrays

@@ -1,17 +1,19 @@
-// PP pair kernel, work-compacted.
+// PP pair kernel.
 //
-// One wavefront per chunk of <= 64 physical particles of one PP cell.  The
-// lanes are split into I i-slots x J j-slots (I the chunk size rounded up
-// to a power of two, I*J = 64), so a sparse cell still fills the wavefront
-// and a dense cell streams each j to all 64 i.  The j of the (2R+1)^2
-// neighbour rows are contiguous runs of the sorted particle stream.
+// Persistent wavefronts take chunks of <= 64 physical particles of one PP
+// cell.  The lanes are split into I i-slots x J j-slots (I the chunk size
+// rounded up to a power of two, I*J = 64).  The j of the (2R+1)^2
+// neighbour rows are contiguous runs of the sorted particle stream; a
+// wavefront concatenates them into one candidate list and fetches it 64 at
+// a time (one coalesced load, prefetched a batch ahead, staged in LDS).
 //
-// Phase 1 tests every candidate (dx, dy, dz, r^2 in the reference's
-// operation order).  Accepted pairs are appended to a per-wavefront LDS
-// queue (ballot + prefix count); whenever 64 are queued, phase 2 evaluates
-// 64 pair forces on a full wavefront and adds them into the i's LDS
-// accumulators.  Divergence is therefore paid on the cheap test, not on the
-// force law.
+// Full chunks (I = 64: cells of >= 64 particles) sum directly: one i per
+// lane, the force law evaluated as pairs are accepted, partial sums in
+// registers in the reference's j order.  Their i are taken in 8x8x8 Morton
+// sub-cell order (so a chunk is spatially compact), and each fetched batch
+// of j is first reduced to those within the range of the chunk's bounding
+// box.  Smaller chunks append accepted pairs to a per-wavefront LDS queue
+// (ballot + prefix count) and evaluate the force law 64 pairs at a time.
 //
 // kRef: the pair force is the reference C kernel's, operation for operation
 //   (pp_force_kernel.c:308-318, F_ra :45-74), with each correctly rounded
@@ -29,6 +31,7 @@
 //   terms accumulate in FP64 (full rate on MI355X) instead of Kahan-
 //   compensated FP32.
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "pp_kernels.h"
@@ -146,9 +149,11 @@ __device__ inline void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
 template <int kMode>
 struct WaveLDS {
     using Acc = typename std::conditional<kMode == kModePrecise, double, float>::type;
-    float4 q[kQ];
-    unsigned char qi[kQ];
-    Acc acc[64][3];
+    float4 q[kQ];           // queued chunks: accepted pairs (dx, dy, dz, r^2)
+    unsigned char qi[kQ];   // and their i slot
+    Acc acc[64][3];         // per-i force sums
+    float4 jb[64];          // current batch of j positions
+    int rs[64], rp[64];     // row group: j = v + rs[k] for rp[k-1] <= v < rp[k]
 };
 
 template <int kMode>
@@ -180,9 +185,13 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
 
 // Persistent: wavefront w of the grid takes chunks w, w + W, w + 2W, ...
 // (W = waves in the grid); the chunk count is read on the device.
+// Chunks of at least 2^direct_lg particles sum their pairs directly in
+// registers (one i per lane, j in sorted order); smaller chunks queue
+// accepted pairs in LDS and evaluate the force law 64 pairs at a time.
 template <int kMode>
 __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
-                                                  const int* __restrict__ nchunks_dev) {
+                                                  const int* __restrict__ nchunks_dev,
+                                                  int direct_lg) {
     __shared__ WaveLDS<kMode> lds[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveLDS<kMode>& L = lds[wave];
@@ -199,7 +208,11 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
         const int I = 1 << lgI, J = 64 >> lgI;
         const int islot = lane & (I - 1), jslot = lane >> lgI;
         const bool vi = islot < ni;
-        const float4 p = A.pos4[a0 + (vi ? islot : 0)];
+        // cells of >= 64 particles are visited in sub-cell order (ilist), so
+        // that a chunk is spatially compact
+        const bool grouped = cend - A.cell_start[cid] >= 64;
+        const int ia = grouped ? A.ilist[a0 + (vi ? islot : 0)] : a0 + (vi ? islot : 0);
+        const float4 p = A.pos4[ia];
         const PPTile T = A.tiles[find_tile2(A.tiles, A.ntile, cid)];
         const long long key = (long long)cid - T.cell_off;
         const int S = T.S;
@@ -226,23 +239,89 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
         wave_lds_sync();
         const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int qhead = 0, qtail = 0;
+        const bool direct = lgI >= direct_lg;
+        using Acc = typename WaveLDS<kMode>::Acc;
+        Acc dax = 0, day = 0, daz = 0;
+        unsigned npa = 0;
+        // direct chunks: bounding box of the chunk's i; a j farther than the
+        // range from the box cannot pair with any of them (the 1e-4 relative
+        // margin keeps every j the per-pair test could accept)
+        float bx0 = p.x, bx1 = p.x, by0 = p.y, by1 = p.y, bz0 = p.z, bz1 = p.z;
+        if (direct) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                bx0 = fminf(bx0, __shfl_xor(bx0, o, 64));
+                bx1 = fmaxf(bx1, __shfl_xor(bx1, o, 64));
+                by0 = fminf(by0, __shfl_xor(by0, o, 64));
+                by1 = fmaxf(by1, __shfl_xor(by1, o, 64));
+                bz0 = fminf(bz0, __shfl_xor(bz0, o, 64));
+                bz1 = fmaxf(bz1, __shfl_xor(bz1, o, 64));
+            }
+        }
+        const float pp2m = pp2 * 1.0001f;
         const int xlo = max(x - R, 0), xhi = min(x + R, S - 1);
-        for (int dz = -R; dz <= R; dz++) {
-            const int zz = z + dz;
-            if (zz < 0 || zz >= S) continue;
-            for (int dy = -R; dy <= R; dy++) {
-                const int yy = y + dy;
-                if (yy < 0 || yy >= S) continue;
-                const long long row = T.cell_off + ((long long)zz * S + yy) * S;
-                const int j0 = A.cell_start[row + xlo], j1 = A.cell_start[row + xhi + 1];
-                if (vi) pc += (unsigned long long)((j1 - j0 - jslot + J - 1) / J);
-                for (int base = j0; base < j1; base += J) {
-                    const int j = base + jslot;
+        const int W = 2 * R + 1, nrows = W * W;
+        // Neighbour rows (each a contiguous run of sorted j) in groups of 64:
+        // the group's runs form one virtual list of M candidates, fetched 64
+        // at a time with one coalesced load, prefetched a batch ahead.
+        for (int rg = 0; rg < nrows; rg += 64) {
+            int js = 0, cnt = 0;
+            const int r = rg + lane;
+            if (r < nrows) {
+                const int zz = z + r / W - R, yy = y + r % W - R;
+                if (zz >= 0 && zz < S && yy >= 0 && yy < S) {
+                    const long long row = T.cell_off + ((long long)zz * S + yy) * S;
+                    js = A.cell_start[row + xlo];
+                    cnt = A.cell_start[row + xhi + 1] - js;
+                }
+            }
+            int inc = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += t;
+            }
+            const int M = __shfl(inc, 63, 64);
+            L.rs[lane] = js - (inc - cnt);
+            L.rp[lane] = inc;
+            wave_lds_sync();
+            if (vi && jslot == 0) pc += (unsigned long long)M;
+            auto fetch = [&](int vb) {
+                const int v = vb + lane;
+                float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (v < M) {
+                    int lo = 0;  // first row with rp > v
+#pragma unroll
+                    for (int step = 32; step > 0; step >>= 1)
+                        if (L.rp[lo + step - 1] <= v) lo += step;
+                    q4 = A.pos4[v + L.rs[lo]];
+                }
+                return q4;
+            };
+            float4 qcur = fetch(0);
+            for (int vb = 0; vb < M; vb += 64) {
+                int nb;
+                if (direct) {  // keep the j near the chunk's box, compacted
+                    const float ex = fmaxf(fmaxf(bx0 - qcur.x, qcur.x - bx1), 0.0f);
+                    const float ey = fmaxf(fmaxf(by0 - qcur.y, qcur.y - by1), 0.0f);
+                    const float ez = fmaxf(fmaxf(bz0 - qcur.z, qcur.z - bz1), 0.0f);
+                    const bool keep = vb + lane < M && ex * ex + ey * ey + ez * ez < pp2m;
+                    const uint64_t km = __ballot(keep);
+                    if (keep) L.jb[__popcll(km & lt)] = qcur;
+                    nb = __popcll(km);
+                } else {
+                    L.jb[lane] = qcur;
+                    nb = min(64, M - vb);
+                }
+                wave_lds_sync();
+                if (vb + 64 < M) qcur = fetch(vb + 64);
+                for (int t = 0; t < nb; t += J) {
+                    const int jj = t + jslot;
                     bool ok = false;
                     float4 e;
-                    if (vi && j < j1) {
+                    if (vi && jj < nb) {
 #pragma clang fp contract(off)
-                        const float4 qj = A.pos4[j];
+                        const float4 qj = L.jb[jj];
                         e.x = qj.x - p.x;
                         e.y = qj.y - p.y;
                         e.z = qj.z - p.z;
@@ -251,6 +330,18 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
                         else
                             e.w = e.x * e.x + e.y * e.y + e.z * e.z;
                         ok = e.w > 0.0f && e.w < pp2;
+                    }
+                    if (direct) {
+                        if (ok) {
+#pragma clang fp contract(off)
+                            float fx, fy, fz;
+                            pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
+                            dax += (Acc)fx;
+                            day += (Acc)fy;
+                            daz += (Acc)fz;
+                            npa++;
+                        }
+                        continue;
                     }
                     const uint64_t m = __ballot(ok);
                     if (m) {
@@ -267,14 +358,24 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
                         }
                     }
                 }
+                wave_lds_sync();  // jb is rewritten by the next batch
             }
+        }
+        if (direct) {
+            if (vi) {
+                atomicAdd(&L.acc[islot][0], dax);
+                atomicAdd(&L.acc[islot][1], day);
+                atomicAdd(&L.acc[islot][2], daz);
+            }
+            pa += npa;
+            wave_lds_sync();
         }
         if (qtail > qhead) flush<kMode>(L, lane, qhead, qtail - qhead, Ka, Kr);
         if (lane == 0) pa += (unsigned long long)qtail;
         if (vi && jslot == 0) {
 #pragma clang fp contract(off)
             pc -= 1;  // the particle itself
-            const int a = a0 + islot;
+            const int a = ia;
             const float ax = (float)L.acc[islot][0], ay = (float)L.acc[islot][1],
                         az = (float)L.acc[islot][2];
             const float mass = A.scal[0], a_mid = A.scal[1], dt = A.scal[2];
@@ -321,6 +422,11 @@ int pp_tiled_grid(int mode) {
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModePrecise>, 256, 0);
     if (e != hipSuccess || per < 1) per = 2;
+    // tuning override: blocks per CU of the persistent grid
+    if (const char* env = std::getenv("CUBE2_PP_BLOCKS_PER_CU")) {
+        const int v = std::atoi(env);
+        if (v >= 1 && v <= 64) per = v;
+    }
     cache[m] = std::max(1, ncu) * per;
     return cache[m];
 }
@@ -331,9 +437,20 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
                        : mode == CUBE2_PP_FAST ? "pp_pairs_fastq" : "pp_pairs_precise";
     Stage st(name);
     const unsigned blocks = (unsigned)std::max(1, grid);
-    if (mode == CUBE2_PP_REF) k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
-    else if (mode == CUBE2_PP_FAST) k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
-    else k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks);
+    static int direct_lg = -1;
+    if (direct_lg < 0) {
+        direct_lg = 6;  // full-wavefront chunks sum directly
+        if (const char* env = std::getenv("CUBE2_PP_DIRECT_LG")) {
+            const int v = std::atoi(env);
+            if (v >= 0 && v <= 7) direct_lg = v;
+        }
+    }
+    if (mode == CUBE2_PP_REF)
+        k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg);
+    else if (mode == CUBE2_PP_FAST)
+        k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg);
+    else
+        k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

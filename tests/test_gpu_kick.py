@@ -112,11 +112,12 @@ def test_kick_driver_equals_separate_calls(lib, mixed_layout):
     assert np.array_equal(r1.ptotal_pm3, r2.ptotal_pm3)
     assert np.array_equal(r1.ptotal_pp, r2.ptotal_pp)
     assert (r1.p_cand, r1.p_acc) == (r2.p_cand, r2.p_acc)
-    want = r1.dt_limits(amid)
+    a32 = float(np.float32(amid))    # the library receives a_mid as a float
+    want = r1.dt_limits(a32)
     for key in ("dt_pm3", "dt_pp", "dt_vmax"):
         assert r1.dt_native[key] == pytest.approx(want[key], rel=1e-15), key
     assert r1.dt_native["dt_pm3"] == pytest.approx(
-        0.5 * math.sqrt(1.0 / (math.sqrt(r1.f2max_pm3) * amid / (6 * math.pi))), rel=1e-15)
+        0.5 * math.sqrt(1.0 / (math.sqrt(r1.f2max_pm3) * a32 / (6 * math.pi))), rel=1e-15)
     assert r1.dt_native["dt_vmax"] == pytest.approx(0.9 * 14 / float(np.max(r1.vmax)), rel=1e-15)
 
 
