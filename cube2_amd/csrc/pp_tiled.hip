@@ -60,24 +60,19 @@ struct FraK {
     float c;       // (1/35)/(a*a), as basic_functions.f08:103 rounds it
 };
 
-// F_ra(r, a) of basic_functions.f08:95-109 with the reference's rounding;
-// inv_rr = 1/RN(r*r) in FP64.
-__device__ inline float fra_ref(float r, const FraK& K, double inv_rr) {
+// F_ra(r, a) of basic_functions.f08:95-109 with the reference's rounding, for
+// ep = RN(2r/a) <= 2 (the caller handles a = 0 and ep > 2, where F_ra is
+// 1/(r*r)).  Evaluated without divergence: the powers are shared, both
+// polynomials are formed in the reference's operation order, and one is
+// selected.  RN(12/e2) is 12 * (FP64 reciprocal), rounded once to float.
+__device__ inline float fra_ref_poly(float ep, const FraK& K) {
 #pragma clang fp contract(off)
-    if (K.a == 0.0f) return (float)inv_rr;
-    const float ep = div_by(2.0f * r, K.inv_a);
-    if (ep > 2.0f) return (float)inv_rr;
-    if (ep > 1.0f) {
-        const float e2 = ep * ep, e3 = e2 * ep, e4 = e3 * ep, e5 = e4 * ep, e6 = e5 * ep;
-        const float t12 = (float)(12.0 * recip64((double)e2));
-        return K.c * (t12 - 224.0f + 896.0f * ep - 840.0f * e2 + 224.0f * e3 + 70.0f * e4 -
-                      48.0f * e5 + 7.0f * e6);
-    }
-    if (ep > 0.0f) {
-        const float e3 = ep * ep * ep, e4 = e3 * ep, e5 = e4 * ep, e6 = e5 * ep;
-        return K.c * (224.0f * ep - 224.0f * e3 + 70.0f * e4 + 48.0f * e5 - 21.0f * e6);
-    }
-    return 0.0f;
+    const float e2 = ep * ep, e3 = e2 * ep, e4 = e3 * ep, e5 = e4 * ep, e6 = e5 * ep;
+    const float t12 = (float)(12.0 * recip64((double)e2));
+    const float hi = K.c * (t12 - 224.0f + 896.0f * ep - 840.0f * e2 + 224.0f * e3 + 70.0f * e4 -
+                            48.0f * e5 + 7.0f * e6);
+    const float lo = K.c * (224.0f * ep - 224.0f * e3 + 70.0f * e4 + 48.0f * e5 - 21.0f * e6);
+    return ep > 1.0f ? hi : (ep > 0.0f ? lo : 0.0f);
 }
 
 __device__ inline float fra_fast2(float ep, float ir2, float c) {
@@ -95,13 +90,13 @@ enum { kModeRef = 0, kModeFast = 1, kModePrecise = 2 };
 
 // 1/r^2 - F_ra(r, a) for r < a (ep = 2r/a in (0, 2]), cancellation-free.
 __device__ inline float short_term(float ep, float ir2, float c) {
-    if (ep > 1.0f) {
-        const float t = 2.0f - ep, t2 = t * t;
-        const float q = fmaf(ep, fmaf(ep, fmaf(ep, 7.0f, 22.0f), 10.0f), 4.0f);
-        return t2 * t2 * t * q * ir2 * (1.0f / 140.0f);
-    }
+    const float t = 2.0f - ep, t2 = t * t;
+    const float q = fmaf(ep, fmaf(ep, fmaf(ep, 7.0f, 22.0f), 10.0f), 4.0f);
+    const float hi = t2 * t2 * t * q * ir2 * (1.0f / 140.0f);
     const float e2 = ep * ep;
-    return ir2 - c * ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+    const float lo =
+        ir2 - c * ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+    return ep > 1.0f ? hi : lo;
 }
 
 template <int kMode>
@@ -110,8 +105,21 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
     if (kMode == kModeRef) {
 #pragma clang fp contract(off)
         const float rm = sqrtf(e.w);
-        const double inv_rr = recip64((double)(rm * rm));
-        const float fpp = fra_ref(rm, Ka, inv_rr) - fra_ref(rm, Kr, inv_rr);
+        const float frr = (float)recip64((double)(rm * rm));  // RN(1/(r*r))
+        // range term: r < range, so ep <= 2 except by rounding at the cutoff
+        const float epr = div_by(2.0f * rm, Kr.inv_a);
+        const float fr = epr > 2.0f ? frr : fra_ref_poly(epr, Kr);
+        // softening term: 1/(r*r) unless r is within app; the polynomial runs
+        // only when some lane of the wavefront needs it
+        float fa = frr;
+        if (Ka.a != 0.0f) {
+            const float epa = div_by(2.0f * rm, Ka.inv_a);
+            if (__ballot(epa <= 2.0f)) {
+                const float pa = fra_ref_poly(epa, Ka);
+                if (epa <= 2.0f) fa = pa;
+            }
+        }
+        const float fpp = fa - fr;
         const double inv_rm = recip64((double)rm);
         fx = div_by(fpp * e.x, inv_rm);
         fy = div_by(fpp * e.y, inv_rm);
@@ -173,6 +181,12 @@ __device__ inline void flush(WaveLDS<kMode>& L, int lane, int begin, int count, 
     wave_lds_sync();
 }
 
+// Wavefront-uniform copies (scalar registers) of values every lane holds.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float uni(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32_t c) {
     int lo = 0, hi = ntile - 1;
     while (lo < hi) {
@@ -184,24 +198,28 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
 }
 
 // Persistent: wavefront w of the grid takes chunks w, w + W, w + 2W, ...
-// (W = waves in the grid); the chunk count is read on the device.
+// (W = waves in the grid); the chunk count is read on the device.  Chunk-
+// level values are kept wavefront-uniform (scalar registers), and the
+// register budget is capped for 6 wavefronts per SIMD: the candidate loop is
+// latency-bound, and occupancy 6 measured faster than 4 (REF at C1: 113 vs
+// 155 ms) despite a small spill in the REF variant.
 // Chunks of at least 2^direct_lg particles sum their pairs directly in
 // registers (one i per lane, j in sorted order); smaller chunks queue
 // accepted pairs in LDS and evaluate the force law 64 pairs at a time.
 template <int kMode>
-__global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
+__global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
                                                   const int* __restrict__ nchunks_dev,
                                                   int direct_lg) {
     __shared__ WaveLDS<kMode> lds[4];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     WaveLDS<kMode>& L = lds[wave];
-    const int nchunks = *nchunks_dev;
+    const int nchunks = uni(*nchunks_dev);
     float f2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f;
     unsigned long long pc = 0, pa = 0;
     for (int ck = blockIdx.x * 4 + wave; ck < nchunks; ck += gridDim.x * 4) {
-        const int a0 = chunks[ck];
-        const uint32_t cid = A.cellid[a0];
-        const int cend = A.cell_start[cid + 1];
+        const int a0 = uni(chunks[ck]);
+        const uint32_t cid = (uint32_t)uni((int)A.cellid[a0]);
+        const int cend = uni(A.cell_start[cid + 1]);
         const int ni = min(64, cend - a0);
         int lgI = 0;
         while ((1 << lgI) < ni) lgI++;
@@ -210,14 +228,14 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
         const bool vi = islot < ni;
         // cells of >= 64 particles are visited in sub-cell order (ilist), so
         // that a chunk is spatially compact
-        const bool grouped = cend - A.cell_start[cid] >= 64;
+        const bool grouped = cend - uni(A.cell_start[cid]) >= 64;
         const int ia = grouped ? A.ilist[a0 + (vi ? islot : 0)] : a0 + (vi ? islot : 0);
         const float4 p = A.pos4[ia];
-        const PPTile T = A.tiles[find_tile2(A.tiles, A.ntile, cid)];
+        const PPTile T = A.tiles[uni(find_tile2(A.tiles, A.ntile, cid))];
         const long long key = (long long)cid - T.cell_off;
         const int S = T.S;
         const int x = (int)(key % S), y = (int)((key / S) % S), z = (int)(key / ((long long)S * S));
-        const float range = p.w;  // one subtile per PP cell: same range for the chunk
+        const float range = uni(p.w);  // one subtile per PP cell: same range for the chunk
         int R;
         float pp2;
         {
@@ -257,6 +275,12 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
                 bz0 = fminf(bz0, __shfl_xor(bz0, o, 64));
                 bz1 = fmaxf(bz1, __shfl_xor(bz1, o, 64));
             }
+            bx0 = uni(bx0);
+            bx1 = uni(bx1);
+            by0 = uni(by0);
+            by1 = uni(by1);
+            bz0 = uni(bz0);
+            bz1 = uni(bz1);
         }
         const float pp2m = pp2 * 1.0001f;
         const int xlo = max(x - R, 0), xhi = min(x + R, S - 1);
@@ -281,7 +305,7 @@ __global__ __launch_bounds__(256) void k_pp_tiled(PPArgs A, const int* __restric
                 const int t = __shfl_up(inc, o, 64);
                 if (lane >= o) inc += t;
             }
-            const int M = __shfl(inc, 63, 64);
+            const int M = uni(__shfl(inc, 63, 64));
             L.rs[lane] = js - (inc - cnt);
             L.rp[lane] = inc;
             wave_lds_sync();
