@@ -25,9 +25,15 @@ int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale,
 // C2R along x for every row pair.
 int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s);
 // Complex transforms along y (axis 1) or z (axis 2) of the half spectrum.
-//   mode 0: forward, 1: inverse, 2: forward, multiply by green[kz][ky][kx], inverse.
+//   mode 0: forward, 1: inverse, 2: forward, multiply by the Green's table,
+//   inverse; green is the tile-ordered copy from green_tiled().
 int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
              const float* green, hipStream_t s);
+
+// Tile-ordered device copy of a Green's table G[kz][ky][kx] for the z pass,
+// made once per (G, n) and cached (G's contents must not change).
+int green_tiled(const float* G, int n, const float** out, hipStream_t s);
+void green_tiled_release();
 
 // Full batched transforms (dir -1: R2C, +1: C2R), unnormalised.
 int fft3d_batch(float* d, int n, int nb, int dir, hipStream_t s);

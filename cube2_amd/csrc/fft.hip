@@ -15,6 +15,7 @@
 //                kx=n/2 terms are ignored, as a C2R transform does.
 // Unnormalised in both directions (FFTW/cuFFT convention, kick.f90:552).
 #include <cmath>
+#include <cstdlib>
 #include <map>
 
 #include "fft.h"
@@ -23,7 +24,8 @@
 namespace cube2 {
 
 constexpr int kFFTThreads = 256;
-constexpr int kFFTElems = 4096;  // complex elements per LDS buffer
+constexpr int kFFTElems = 4096;  // complex elements per LDS buffer (x passes)
+constexpr int kColsMax = 8192;   // largest LDS buffer of the y/z passes
 
 __device__ inline float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -106,10 +108,10 @@ __device__ inline int mod_p(int j, int p, float invp) {
 // of line l at l*ls + e*es), in place: every thread first reads the inputs
 // of all its butterflies into registers, the block synchronises, then the
 // outputs are written.  Consecutive threads take consecutive lines.
-template <int R, int S>
+template <int R, int S, int CAP>
 __device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ tw, int n, int p,
                              float invp, int lgL, int ls, int es) {
-    constexpr int KMAX = (kFFTElems / R + kFFTThreads - 1) / kFFTThreads;
+    constexpr int KMAX = (CAP / R + kFFTThreads - 1) / kFFTThreads;
     const int m = n / R, ts = n / (p * R);
     const int L = 1 << lgL;
     const int tasks = L * m;
@@ -147,16 +149,17 @@ __device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ 
     __syncthreads();
 }
 
-// Transforms 2^lgL lines held in LDS buffer a in place.  Ends with a barrier.
-template <int S>
+// Transforms 2^lgL lines (at most CAP elements) held in LDS buffer a in
+// place.  Ends with a barrier.
+template <int S, int CAP = kFFTElems>
 __device__ void fft_lds(float2* a, const float2* tw, const FFTPlan& P, int lgL, int ls, int es) {
     for (int s = 0; s < P.nst; s++) {
         const int r = P.rad[s], p = P.pst[s];
         const float ip = P.invp[s];
-        if (r == 4) stage<4, S>(a, tw, P.n, p, ip, lgL, ls, es);
-        else if (r == 2) stage<2, S>(a, tw, P.n, p, ip, lgL, ls, es);
-        else if (r == 3) stage<3, S>(a, tw, P.n, p, ip, lgL, ls, es);
-        else stage<5, S>(a, tw, P.n, p, ip, lgL, ls, es);
+        if (r == 4) stage<4, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
+        else if (r == 2) stage<2, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
+        else if (r == 3) stage<3, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
+        else stage<5, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
     }
 }
 
@@ -183,26 +186,28 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d
     const float scale = scale_dev ? *scale_dev : 1.0f;
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1;
-    const int L = 1 << lgL;
+    const int L = 1 << lgL, es = L + 1;  // padded: the lanes of a row hit distinct banks
     float2* tw = sm;
     float2* A = sm + n;
     load_twiddles(tw, P);
     const int g0 = blockIdx.x * L;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll 2
     for (int l = wave; l < L; l += kFFTThreads / 64) {
-        if (g0 + l >= npairs) break;
-        const float* ra = row_pair(d, g0 + l, n, rs);
-        for (int t = lane; t < n; t += 64)
-            A[l + t * L] = make_float2(ra[t] * scale, ra[t + n + 2] * scale);
+        if (g0 + l < npairs) {
+            const float* ra = row_pair(d, g0 + l, n, rs);
+            for (int t = lane; t < n; t += 64)
+                A[l + t * es] = make_float2(ra[t] * scale, ra[t + n + 2] * scale);
+        }
     }
     __syncthreads();
-    fft_lds<-1>(A, tw, P, lgL, 1, L);
+    fft_lds<-1>(A, tw, P, lgL, 1, es);
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) break;
         float2* ra = reinterpret_cast<float2*>(row_pair(d, g0 + l, n, rs));
         float2* rb = ra + nh;
         for (int k = lane; k < nh; k += 64) {
-            const float2 zk = A[l + k * L], zn = A[l + (k == 0 ? 0 : n - k) * L];
+            const float2 zk = A[l + k * es], zn = A[l + (k == 0 ? 0 : n - k) * es];
             ra[k] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
             rb[k] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
         }
@@ -213,14 +218,15 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
                                                            int npairs, size_t rs, int lgL) {
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1, half = n / 2;
-    const int L = 1 << lgL;
+    const int L = 1 << lgL, es = L + 1;  // padded: the lanes of a row hit distinct banks
     float2* tw = sm;
     float2* A = sm + n;
     load_twiddles(tw, P);
     const int g0 = blockIdx.x * L;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll 2
     for (int l = wave; l < L; l += kFFTThreads / 64) {
-        if (g0 + l >= npairs) break;
+        if (g0 + l >= npairs) continue;
         const float2* ra = reinterpret_cast<const float2*>(row_pair(d, g0 + l, n, rs));
         const float2* rb = ra + nh;
         for (int k = lane; k < n; k += 64) {
@@ -238,16 +244,16 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
                 a.y = 0.0f;
                 b.y = 0.0f;
             }
-            A[l + k * L] = make_float2(a.x - b.y, a.y + b.x);
+            A[l + k * es] = make_float2(a.x - b.y, a.y + b.x);
         }
     }
     __syncthreads();
-    fft_lds<1>(A, tw, P, lgL, 1, L);
+    fft_lds<1>(A, tw, P, lgL, 1, es);
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) break;
         float* ra = row_pair(d, g0 + l, n, rs);
         for (int t = lane; t < n; t += 64) {
-            const float2 v = A[l + t * L];
+            const float2 v = A[l + t * es];
             ra[t] = v.x;
             ra[t + n + 2] = v.y;
         }
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
 
 // Tiles of C = 2^lgC consecutive kx columns x n rows along y (axis 1) or z
 // (axis 2); LDS element (row i, column c) at i*C + c.
-template <int MODE>
+template <int MODE, int CAP>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dreal, FFTPlan P,
                                                           size_t rs, int axis, int lgC,
                                                           const float* __restrict__ green) {
@@ -280,25 +286,45 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
     }
     __syncthreads();
     if (MODE == 1) {
-        fft_lds<1>(A, tw, P, lgC, 1, C);
+        fft_lds<1, CAP>(A, tw, P, lgC, 1, C);
     } else {
-        fft_lds<-1>(A, tw, P, lgC, 1, C);
+        fft_lds<-1, CAP>(A, tw, P, lgC, 1, C);
         if (MODE == 2) {
+            // Green's table in tile order: [y][tile][z][c], contiguous per tile
+            const float* gt = green + ((size_t)o * ntiles + tile) * n * C;
             for (int e = threadIdx.x; e < n * C; e += kFFTThreads) {
-                const int i = e >> lgC, c = e & (C - 1);
+                const int c = e & (C - 1);
                 if (c < Cw) {
-                    const float gk = green[obase + i * rowstride + c];
+                    const float gk = gt[e];
                     A[e] = make_float2(A[e].x * gk, A[e].y * gk);
                 }
             }
             __syncthreads();
-            fft_lds<1>(A, tw, P, lgC, 1, C);
+            fft_lds<1, CAP>(A, tw, P, lgC, 1, C);
         }
     }
     for (int e = threadIdx.x; e < n * C; e += kFFTThreads) {
         const int i = e >> lgC, c = e & (C - 1);
         if (c < Cw) d[obase + i * rowstride + c] = A[e];
     }
+}
+
+// Green's table G[z][y][kx] (z, y < n, kx < n/2+1) to the z pass's tile
+// order T[y][tile][z][c], kx = tile*C + c; columns past n/2 are zero.
+__global__ void k_green_tile(const float* __restrict__ G, float* __restrict__ T, int n, int lgC,
+                             int ntiles) {
+    const int nh = n / 2 + 1, C = 1 << lgC;
+    const size_t tot = (size_t)n * ntiles * n * C;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= tot) return;
+    const int c = (int)(q & (C - 1));
+    size_t r = q >> lgC;
+    const int z = (int)(r % n);
+    r /= n;
+    const int tile = (int)(r % ntiles);
+    const int y = (int)(r / ntiles);
+    const int kx = tile * C + c;
+    T[q] = kx < nh ? G[((size_t)z * n + y) * nh + kx] : 0.0f;
 }
 
 // ------------------------------------------------------------- host side
@@ -308,12 +334,17 @@ static bool g_attr_set = false;
 
 static int set_lds_attributes() {
     if (g_attr_set) return CUBE2_OK;
-    const int bytes = (kFFTElems + 1024) * (int)sizeof(float2);
+    // x passes: n twiddles + n (L+1) <= n + kFFTElems + n elements
+    const int bytes = (kFFTElems + 2048) * (int)sizeof(float2);
+    const int bytes_cols = (kColsMax + 1024) * (int)sizeof(float2);
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, 4096>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, 4096>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, 4096>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, 8192>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, 8192>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, 8192>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
     g_attr_set = true;
     return CUBE2_OK;
 }
@@ -380,7 +411,7 @@ int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale,
         set_error("FFT batch too large");
         return CUBE2_ERR_CAPACITY;
     }
-    const size_t lds = (size_t)(n + L * n) * sizeof(float2);
+    const size_t lds = (size_t)(n + (L + 1) * n) * sizeof(float2);
     k_fft_x_r2c<<<(unsigned)((npairs + L - 1) / L), kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
@@ -395,26 +426,79 @@ int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s) {
         set_error("FFT batch too large");
         return CUBE2_ERR_CAPACITY;
     }
-    const size_t lds = (size_t)(n + L * n) * sizeof(float2);
+    const size_t lds = (size_t)(n + (L + 1) * n) * sizeof(float2);
     k_fft_x_c2r<<<(unsigned)((npairs + L - 1) / L), kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
 
+// Column tile width of the y/z passes: about 4096 LDS elements per tile
+// (CUBE2_FFT_MIN_LGC raises the minimum width; 16 columns measured slower at
+// n = 320 and 480 for the lower occupancy).
+int fft_cols_lgC(int n) {
+    const int nh = n / 2 + 1;
+    static int min_lg = -1;
+    if (min_lg < 0) {
+        min_lg = 0;
+        if (const char* env = std::getenv("CUBE2_FFT_MIN_LGC")) min_lg = std::max(0, std::min(6, std::atoi(env)));
+    }
+    int lgC = std::max(lg_floor(std::max(1, kFFTElems / n)), min_lg);
+    while (lgC > 0 && (1 << lgC) > 2 * nh) lgC--;  // no wider than the half spectrum needs
+    return lgC;
+}
+
 int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
              const float* green, hipStream_t s) {
     const int n = P.n, nh = n / 2 + 1;
-    int lgC = lg_floor(std::max(1, kFFTElems / n));
-    while ((1 << lgC) > 2 * nh) lgC--;  // no wider than the half spectrum needs
+    const int lgC = fft_cols_lgC(n);
     const int C = 1 << lgC;
     const int ntiles = (nh + C - 1) / C;
     dim3 grid((unsigned)(n * ntiles), (unsigned)nb);
     const size_t lds = (size_t)(n + C * n) * sizeof(float2);
-    if (mode == 0) k_fft_cols<0><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-    else if (mode == 1) k_fft_cols<1><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-    else k_fft_cols<2><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+    if (C * n > kFFTElems) {
+        if (mode == 0) k_fft_cols<0, 8192><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else if (mode == 1) k_fft_cols<1, 8192><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else k_fft_cols<2, 8192><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+    } else {
+        if (mode == 0) k_fft_cols<0, 4096><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else if (mode == 1) k_fft_cols<1, 4096><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else k_fft_cols<2, 4096><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+    }
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
+}
+
+struct GreenTiled {
+    const float* src;
+    int n;
+    float* dev;
+};
+static std::vector<GreenTiled> g_green_tiled;
+
+int green_tiled(const float* G, int n, const float** out, hipStream_t s) {
+    for (auto& t : g_green_tiled)
+        if (t.src == G && t.n == n) {
+            *out = t.dev;
+            return CUBE2_OK;
+        }
+    const int lgC = fft_cols_lgC(n), C = 1 << lgC;
+    const int ntiles = (n / 2 + 1 + C - 1) / C;
+    const size_t tot = (size_t)n * ntiles * n * C;
+    GreenTiled t{G, n, nullptr};
+    buffers_changed();
+    C2_HIP(hipMalloc(&t.dev, tot * sizeof(float)));
+    k_green_tile<<<ceil_div(tot, 256), 256, 0, s>>>(G, t.dev, n, lgC, ntiles);
+    C2_HIP(hipGetLastError());
+    C2_HIP(hipStreamSynchronize(s));
+    g_green_tiled.push_back(t);
+    *out = t.dev;
+    return CUBE2_OK;
+}
+
+void green_tiled_release() {
+    buffers_changed();
+    for (auto& t : g_green_tiled) (void)hipFree(t.dev);
+    g_green_tiled.clear();
 }
 
 int fft3d_batch(float* d, int n, int nb, int dir, hipStream_t s) {

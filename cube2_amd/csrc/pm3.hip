@@ -580,8 +580,12 @@ int pm3_prepare(const int* isort, const int* ires, const int* ixyz3, const int* 
             set_error("no Green's table for level %d", G.L);
             return CUBE2_ERR_ARG;
         }
-    for (int q = 0; q < 5; q++) P.green[q] = d_green[q];
     C2_TRY(ensure_init());
+    for (int q = 0; q < 5; q++) P.green[q] = nullptr;
+    for (auto& G : groups) {  // tile-ordered copies for the z pass
+        const int n = (g.ntt * g.p.ratio_cs + 2 * g.p.ratio_cs) * ratio_sf[G.L - 1];
+        C2_TRY(green_tiled(d_green[G.L - 2], n, &P.green[G.L - 2], C.stream));
+    }
     P.n_used = n_used;
     P.batches.clear();
     P.job_it.clear();
@@ -721,9 +725,11 @@ int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf
     C2_HIP(hipMemcpyAsync(g_pm3.jobs.ptr, &J, sizeof(PM3Job), hipMemcpyHostToDevice, s));
     buffers_changed();
     DebugOut dbg{rho, phi, force};
+    const float* gt = nullptr;
+    C2_TRY(green_tiled(d_green, A.n, &gt, s));
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, 0.0f, 0.0f, s, &d_scal));
-    C2_TRY(run_batch(A, g_pm3.jobs.as<PM3Job>(), 1, d_green, mass, d_scal, 0, nullptr, f2bits,
+    C2_TRY(run_batch(A, g_pm3.jobs.as<PM3Job>(), 1, gt, mass, d_scal, 0, nullptr, f2bits,
                      nullptr, reinterpret_cast<int*>(f2bits + 1), &dbg, s));
     unsigned h[4];
     C2_HIP(hipMemcpyAsync(h, f2bits, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -810,6 +816,7 @@ void pm3_release() {
     g_pm3.ptot.release();
     g_pm3.h_res.release();
     fft_release();
+    green_tiled_release();
 }
 
 }  // namespace cube2

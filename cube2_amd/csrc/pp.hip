@@ -218,21 +218,19 @@ __global__ __launch_bounds__(256) void k_pp_scatter(
             b = (bz * rcp + by) * rcp + bx;
             key = ((int64_t)(c.ck * rcp + bz) * S + (c.cj * rcp + by)) * S + (c.ci * rcp + bx);
         }
-        uint64_t todo = __ballot(valid);
+        // rank among the earlier lanes with the same PP cell: compare with
+        // every lane's cell (scalar broadcasts, no dependency chain), then the
+        // last lane of each cell advances that cell's cursor
+        uint64_t same = 0;
+#pragma unroll
+        for (int k = 0; k < 64; k++)
+            same |= (uint64_t)(__builtin_amdgcn_readlane(b, k) == b) << k;
         int mypos = 0;
-        while (todo) {
-            const int leader = __ffsll((unsigned long long)todo) - 1;
-            const int bl = __shfl(b, leader, 64);
-            const uint64_t m = __ballot(valid && b == bl);
-            int old = 0;
-            if (lane == leader) {
-                old = cur[wave][bl];
-                cur[wave][bl] = old + __popcll(m);
-            }
-            old = __shfl(old, leader, 64);
-            if (valid && b == bl) mypos = old + __popcll(m & lt);
-            todo &= ~m;
-        }
+        if (valid) mypos = cur[wave][b] + __popcll(same & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && !(same & ~lt & ~(1ull << lane)))
+            cur[wave][b] += __popcll(same & lt) + 1;
+        __builtin_amdgcn_wave_barrier();
         if (valid) {
             const int dst = cell_start[off + key] + mypos;
             pos4[dst] = make_float4((float)ratio_cs * x0, (float)ratio_cs * x1,
