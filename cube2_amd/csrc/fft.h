@@ -30,7 +30,7 @@ int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s);
 int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
              const float* green, hipStream_t s);
 
-// Tile-ordered device copy of a Green's table G[kz][ky][kx] for the z pass,
+// Tile-ordered device copy of a Green's table G[kz][ky][kx] / n^3 for the z pass,
 // made once per (G, n) and cached (G's contents must not change).
 int green_tiled(const float* G, int n, const float** out, hipStream_t s);
 void green_tiled_release();

@@ -310,7 +310,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
 }
 
 // Green's table G[z][y][kx] (z, y < n, kx < n/2+1) to the z pass's tile
-// order T[y][tile][z][c], kx = tile*C + c; columns past n/2 are zero.
+// order T[y][tile][z][c], kx = tile*C + c, times 1/n^3 (the normalisation of
+// the unnormalised transform pair, kick.f90:552, applied once here instead
+// of per mesh point); columns past n/2 are zero.
 __global__ void k_green_tile(const float* __restrict__ G, float* __restrict__ T, int n, int lgC,
                              int ntiles) {
     const int nh = n / 2 + 1, C = 1 << lgC;
@@ -324,7 +326,8 @@ __global__ void k_green_tile(const float* __restrict__ G, float* __restrict__ T,
     const int tile = (int)(r % ntiles);
     const int y = (int)(r / ntiles);
     const int kx = tile * C + c;
-    T[q] = kx < nh ? G[((size_t)z * n + y) * nh + kx] : 0.0f;
+    const double inv_n3 = 1.0 / ((double)n * n * n);
+    T[q] = kx < nh ? (float)((double)G[((size_t)z * n + y) * nh + kx] * inv_n3) : 0.0f;
 }
 
 // ------------------------------------------------------------- host side

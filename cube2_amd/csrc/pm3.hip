@@ -4,12 +4,13 @@
 //
 // Per kick the subtiles are grouped by level (one mesh size n per level)
 // and each group runs as batches of jobs sized to a memory budget:
-//   deposit   one wavefront per coarse cell of a subtile plus its one-cell
-//             halo; FP64 mesh coordinate, FP32 TSC weights (kick.f90:437-445),
-//             27 hardware FP32 atomic adds into the job's padded mesh;
-//   FFT       x R2C (scaled by the particle mass), y, z+Green+z^-1, y^-1,
-//             x C2R (fft.hip);
-//   gradient  phi/n^3, 4-point difference with the real(4) weights of
+//   deposit   one block per 16^3 brick of a job's mesh sums in LDS the TSC
+//             weights (FP64 mesh coordinate, FP32 weights, kick.f90:437-445)
+//             of the particles of the coarse cells overlapping the brick,
+//             then stores the brick (no clearing, no global atomics);
+//   FFT       x R2C (scaled by the particle mass), y, z+Green/n^3+z^-1, y^-1,
+//             x C2R (fft.hip), so the potential arrives divided by n^3;
+//   gradient  4-point difference with the real(4) weights of
 //             parameters.f90:49-50, 3 force planes over [0, nfp+1]^3, max
 //             |F|^2 over the interior;
 //   interp    one wavefront per physical coarse cell, TSC gather of the 27
@@ -75,7 +76,15 @@ __device__ inline double mesh_coord(const MeshArgs& A, int c_nte, int16_t v, int
     return xc * (double)(A.ratio_cs * A.rsf) - (double)(u * A.nfp);
 }
 
+// floor(mesh_coord) exactly, in integer arithmetic: with U the unsigned
+// 16-bit position, frac * cf = (2U + 1) cf / 2^17.
+__device__ inline int fine_floor(const MeshArgs& A, int c_nte, int16_t v, int u, int cf) {
+    const int U = (int)(int16_t)((int)v - 32768) + 32768;
+    return (c_nte - A.ncb) * cf - u * A.nfp + (int)(((int64_t)(2 * U + 1) * cf) >> 17);
+}
+
 constexpr int kWaves = 4;
+constexpr int kInterpSeg = 8;  // wavefronts per coarse cell in the interpolation
 
 // Wavefront per (job, coarse cell of the subtile + 1 halo cell).
 __global__ __launch_bounds__(256) void k_pm3_deposit(MeshArgs A, float* __restrict__ rho, size_t rs,
@@ -174,16 +183,23 @@ __global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* _
                     if (threadIdx.x == 0) atomicOr(err, 2);
                     continue;
                 }
+#pragma unroll 4
                 for (int l = threadIdx.x; l < np; l += blockDim.x) {
                     const int16_t* p = A.xp + 3 * (base + l);
+                    // floor of the mesh coordinate in integer arithmetic (the
+                    // same value tsc_axis computes): particles that miss the
+                    // brick are rejected before the FP64 work
+                    const int X0 = fine_floor(A, ci, p[0], J.u[0], cf) + off - lo[0];
+                    const int Y0 = fine_floor(A, cj, p[1], J.u[1], cf) + off - lo[1];
+                    const int Z0 = fine_floor(A, ck, p[2], J.u[2], cf) + off - lo[2];
+                    if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 || Z0 > kBrick - 1 || Z0 < -2)
+                        continue;
                     int cx, cy, cz;
                     float wx[3], wy[3], wz[3];
                     tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
                     tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
                     tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
-                    const int X0 = cx + off - lo[0], Y0 = cy + off - lo[1], Z0 = cz + off - lo[2];
-                    if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 || Z0 > kBrick - 1 || Z0 < -2)
-                        continue;
+                    (void)cx, (void)cy, (void)cz;  // equal to the integer floors above
 #pragma unroll
                     for (int a3 = 0; a3 < 3; a3++) {
                         const int Z = Z0 + a3;
@@ -216,8 +232,7 @@ __global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* _
 // (x tiles * y tiles, z, job): 32-bit index math, coalesced along x.
 __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi, size_t rs,
                                                   float* __restrict__ force, int n, int nfp,
-                                                  int nfb, float n3,
-                                                  unsigned* __restrict__ f2bits) {
+                                                  int nfb, unsigned* __restrict__ f2bits) {
     const int nf = nfp + 2;
     const int xt = (nf + 63) >> 6;
     const int i0 = (int)(blockIdx.x % xt) * 64 + (threadIdx.x & 63);
@@ -237,9 +252,10 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
         float fx, fy, fz;
         {
 #pragma clang fp contract(off)
-            fx = P[c - 2] / n3 * w0 + P[c - 1] / n3 * w1 + P[c + 1] / n3 * w2 + P[c + 2] / n3 * w3;
-            fy = P[c - 2 * sr] / n3 * w0 + P[c - sr] / n3 * w1 + P[c + sr] / n3 * w2 + P[c + 2 * sr] / n3 * w3;
-            fz = P[c - 2 * sp] / n3 * w0 + P[c - sp] / n3 * w1 + P[c + sp] / n3 * w2 + P[c + 2 * sp] / n3 * w3;
+            // phi arrives divided by n^3 (folded into the Green's table)
+            fx = P[c - 2] * w0 + P[c - 1] * w1 + P[c + 1] * w2 + P[c + 2] * w3;
+            fy = P[c - 2 * sr] * w0 + P[c - sr] * w1 + P[c + sr] * w2 + P[c + 2 * sr] * w3;
+            fz = P[c - 2 * sp] * w0 + P[c - sp] * w1 + P[c + sp] * w2 + P[c + 2 * sp] * w3;
         }
         const size_t fs = (size_t)nf * nf * nf;
         const size_t u = ((size_t)k0 * nf + j0) * nf + i0;
@@ -254,7 +270,9 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
     block_max_to<1>(f2bits, fv);
 }
 
-// Wavefront per (job, physical coarse cell): TSC gather and FP64 Delta-v.
+// kInterpSeg wavefronts per (job, physical coarse cell), each taking every
+// kInterpSeg-th group of 64 particles (dense cells would otherwise leave one
+// wavefront as the tail): TSC gather and FP64 Delta-v.
 __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __restrict__ force,
                                                     double* __restrict__ dv64, int accumulate,
                                                     const float* __restrict__ scal,
@@ -262,8 +280,10 @@ __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __r
     const float amid = scal[1], dt = scal[2];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t per = (int64_t)A.ntt * A.ntt * A.ntt;
-    const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
-    if (w >= per * A.njobs) return;
+    const int64_t w0 = (int64_t)blockIdx.x * kWaves + wave;
+    if (w0 >= per * A.njobs * kInterpSeg) return;
+    const int seg = (int)(w0 % kInterpSeg);  // kInterpSeg wavefronts share a cell
+    const int64_t w = w0 / kInterpSeg;
     const int jb = (int)(w / per);
     const int r = (int)(w - (int64_t)jb * per);
     const PM3Job J = A.jobs[jb];
@@ -274,11 +294,11 @@ __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __r
     const int np = A.rhoc[cell];
     const int64_t base = A.nzero[cell];
     if (np <= 0 || base < 0 || base + np > A.n_used) return;
-    if (lane == 0) atomicAdd(&ptot[jb], (unsigned long long)np);
+    if (seg == 0 && lane == 0) atomicAdd(&ptot[jb], (unsigned long long)np);
     const int nf = A.nfp + 2;
     const size_t fs = (size_t)nf * nf * nf;
     const float* F = force + (size_t)jb * 3 * fs;
-    for (int l = lane; l < np; l += 64) {
+    for (int l = seg * 64 + lane; l < np; l += 64 * kInterpSeg) {
         const int64_t ip = base + l;
         const int16_t* p = A.xp + 3 * ip;
         int cx, cy, cz;
@@ -510,7 +530,6 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
         Stage st("pm3_fft_x_c2r", 2 * mesh_bytes);
         C2_TRY(fft_x_c2r(rho, *P, (int)nb, rs, s));
     }
-    const float n3 = (float)n * (float)n * (float)n;
     if (dbg && dbg->phi) {
         std::vector<float> h(rs);
         C2_HIP(hipMemcpyAsync(h.data(), rho, rs * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -518,13 +537,13 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
         for (int z = 0; z < n; z++)
             for (int y = 0; y < n; y++)
                 for (int x = 0; x < n; x++)
-                    dbg->phi[((size_t)z * n + y) * n + x] = h[((size_t)z * n + y) * (n + 2) + x] / n3;
+                    dbg->phi[((size_t)z * n + y) * n + x] = h[((size_t)z * n + y) * (n + 2) + x];
     }
     {
         Stage st("pm3_grad", mesh_bytes + (double)nb * 3 * fs * 4);
         const int nfq = nfp + 2;
         dim3 grid((unsigned)(((nfq + 63) / 64) * ((nfq + 3) / 4)), (unsigned)nfq, (unsigned)nb);
-        k_pm3_grad<<<grid, 256, 0, s>>>(rho, rs, force, n, nfp, base.nfb, n3, f2bits);
+        k_pm3_grad<<<grid, 256, 0, s>>>(rho, rs, force, n, nfp, base.nfb, f2bits);
         C2_HIP(hipGetLastError());
     }
     if (dbg && dbg->force) {
@@ -532,7 +551,7 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
         C2_HIP(hipStreamSynchronize(s));
     }
     if (dv64) {
-        const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt;
+        const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt * kInterpSeg;
         Stage st("pm3_interp", (double)nb * 3 * fs * 4);
         k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, d_scal, ptot);
         C2_HIP(hipGetLastError());

@@ -159,3 +159,19 @@ def test_gradient_of_a_plane_wave(oracle_lib):
     fd = sum(w[m] * np.roll(phi, -o) for m, o in enumerate((-2, -1, 1, 2)))
     want = -(8 * np.sin(k) - np.sin(2 * k)) / 6 * np.sin(k * xs)
     assert np.allclose(fd, want, atol=2e-7)
+
+
+def test_integer_fine_floor_equals_fp64_floor():
+    """The deposit rejects particles with floor(mesh coordinate) computed in
+    integers: (c - ncb) cf - u nfp + ((2U + 1) cf >> 17).  It must equal the
+    floor of the FP64 coordinate ((c - ncb) + (U + 0.5)/2^16) cf - u nfp for
+    every 16-bit position U."""
+    U = np.arange(65536, dtype=np.int64)
+    frac = (U + 0.5) / 65536.0
+    for cf, nfp, ncb in [(8, 64, 6), (16, 128, 6), (24, 192, 6), (32, 256, 6), (48, 384, 6), (4, 32, 4)]:
+        for c in (0, ncb - 1, ncb, ncb + 3, ncb + 17):
+            for u in (0, 1, 3):
+                x = ((c - ncb) + frac) * cf - u * nfp
+                want = np.floor(x).astype(np.int64)
+                got = (c - ncb) * cf - u * nfp + (((2 * U + 1) * cf) >> 17)
+                assert np.array_equal(got, want), (cf, c, u)
