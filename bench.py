@@ -130,6 +130,8 @@ def main(argv=None) -> int:
     ap.add_argument("--pp-mode", default="ref", choices=["ref", "exact", "fast", "precise"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch-shape tunables on this workload before timing")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager dispatch only (value = eager kick), e.g. under counters")
     ap.add_argument("--seed", type=int, default=1)
@@ -176,6 +178,11 @@ def main(argv=None) -> int:
         t1 = time.perf_counter()
         res = k.kick(mass, amid, dt)
         log(f"warmup {w}: {1e3 * (time.perf_counter() - t1):.1f} ms")
+    tuning = None
+    if args.autotune:
+        from cube2_amd.autotune import autotune
+        tuning = autotune(k, mass, amid, dt, log=log)
+        log(f"autotune chose {tuning['chosen']}")
 
     # eager dispatch, with per-stage HIP-event timing
     _lib.profile(True)
@@ -267,6 +274,7 @@ def main(argv=None) -> int:
         "p_acc": int(res.p_acc),
         "stage_ms_per_step": stages,
         "roofline": roof,
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -17,6 +17,8 @@ _LIB = None
 
 OK, ERR_ARG, ERR_GEOMETRY, ERR_CAPACITY, ERR_HIP, ERR_STATE = range(6)
 PP_EXACT, PP_FAST, PP_REF, PP_PRECISE = 0, 1, 2, 3
+TUNE_PP_BLOCKS_PER_CU, TUNE_PP_DIRECT_LG, TUNE_FFT_MIN_LGC, TUNE_INTERP_SEG = 0, 1, 2, 3
+TUNE_NAMES = {"pp_blocks_per_cu": 0, "pp_direct_lg": 1, "fft_min_lgc": 2, "interp_seg": 3}
 
 
 class Cube2Error(RuntimeError):
@@ -53,6 +55,8 @@ SIGNATURES = {
     "cube2_set_params": (C.c_int, [C.POINTER(Params)]),
     "cube2_get_params": (C.c_int, [C.POINTER(Params)]),
     "cube2_set_pp_mode": (C.c_int, [C.c_int]),
+    "cube2_set_tuning": (C.c_int, [C.c_int, C.c_int]),
+    "cube2_get_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     "cube2_pm3_force_dev": (C.c_int, [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3 + [vp_, vp_]),
     "cube2_pp_force_dev": (C.c_int, [vp_] * 8 + [C.c_int64] + [C.c_float] * 3 + [vp_] * 5),
     "cube2_velocity_apply_dev": (C.c_int, [vp_, vp_, vp_, C.c_int64, vp_]),
@@ -124,6 +128,16 @@ def get_params() -> Params:
 
 def set_pp_mode(mode: int) -> None:
     check(lib().cube2_set_pp_mode(int(mode)))
+
+
+def set_tuning(key: int, value: int) -> None:
+    check(lib().cube2_set_tuning(int(key), int(value)))
+
+
+def get_tuning(key: int) -> int:
+    v = C.c_int()
+    check(lib().cube2_get_tuning(int(key), C.byref(v)))
+    return v.value
 
 
 def profile(on: bool = True) -> None:

@@ -260,6 +260,33 @@ int cube2_set_params(const cube2_params_t* p) {
     return CUBE2_OK;
 }
 
+int cube2_set_tuning(int key, int value) {
+    static const int lo[CUBE2_TUNE_COUNT] = {1, 1, 1, 1}, hi[CUBE2_TUNE_COUNT] = {16, 7, 6, 32};
+    if (key < 0 || key >= CUBE2_TUNE_COUNT || (value != 0 && (value < lo[key] || value > hi[key]))) {
+        set_error("tuning key %d value %d out of range", key, value);
+        return CUBE2_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> l(g_api_mutex);
+    Context& c = ctx();
+    if (c.tune[key] != value) {
+        c.tune[key] = value;
+        buffers_changed();  // recorded launches depend on it
+    }
+    return CUBE2_OK;
+}
+
+int cube2_get_tuning(int key, int* value) {
+    if (key < 0 || key >= CUBE2_TUNE_COUNT || !value) {
+        set_error("bad tuning key %d", key);
+        return CUBE2_ERR_ARG;
+    }
+    static const int builtin[CUBE2_TUNE_COUNT] = {0, 6, 0, 8};
+    const int v = ctx().tune[key];
+    *value = v ? v : builtin[key];
+    if (key == CUBE2_TUNE_PP_BLOCKS_PER_CU && !v) *value = 0;  // occupancy-derived at launch
+    return CUBE2_OK;
+}
+
 int cube2_get_params(cube2_params_t* p) {
     std::lock_guard<std::mutex> l(g_api_mutex);
     Context& c = ctx();

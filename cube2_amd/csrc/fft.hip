@@ -15,7 +15,6 @@
 //                kx=n/2 terms are ignored, as a C2R transform does.
 // Unnormalised in both directions (FFTW/cuFFT convention, kick.f90:552).
 #include <cmath>
-#include <cstdlib>
 #include <map>
 
 #include "fft.h"
@@ -436,15 +435,11 @@ int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s) {
 }
 
 // Column tile width of the y/z passes: about 4096 LDS elements per tile
-// (CUBE2_FFT_MIN_LGC raises the minimum width; 16 columns measured slower at
-// n = 320 and 480 for the lower occupancy).
+// (the FFT_MIN_LGC tunable raises the minimum width; 16 columns measured
+// slower at n = 320 and 480 for the lower occupancy).
 int fft_cols_lgC(int n) {
     const int nh = n / 2 + 1;
-    static int min_lg = -1;
-    if (min_lg < 0) {
-        min_lg = 0;
-        if (const char* env = std::getenv("CUBE2_FFT_MIN_LGC")) min_lg = std::max(0, std::min(6, std::atoi(env)));
-    }
+    const int min_lg = ctx().tune[CUBE2_TUNE_FFT_MIN_LGC];
     int lgC = std::max(lg_floor(std::max(1, kFFTElems / n)), min_lg);
     while (lgC > 0 && (1 << lgC) > 2 * nh) lgC--;  // no wider than the half spectrum needs
     return lgC;
@@ -473,21 +468,21 @@ int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
 
 struct GreenTiled {
     const float* src;
-    int n;
+    int n, lgC;
     float* dev;
 };
 static std::vector<GreenTiled> g_green_tiled;
 
 int green_tiled(const float* G, int n, const float** out, hipStream_t s) {
+    const int lgC = fft_cols_lgC(n), C = 1 << lgC;
     for (auto& t : g_green_tiled)
-        if (t.src == G && t.n == n) {
+        if (t.src == G && t.n == n && t.lgC == lgC) {
             *out = t.dev;
             return CUBE2_OK;
         }
-    const int lgC = fft_cols_lgC(n), C = 1 << lgC;
     const int ntiles = (n / 2 + 1 + C - 1) / C;
     const size_t tot = (size_t)n * ntiles * n * C;
-    GreenTiled t{G, n, nullptr};
+    GreenTiled t{G, n, lgC, nullptr};
     buffers_changed();
     C2_HIP(hipMalloc(&t.dev, tot * sizeof(float)));
     k_green_tile<<<ceil_div(tot, 256), 256, 0, s>>>(G, t.dev, n, lgC, ntiles);

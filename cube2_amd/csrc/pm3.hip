@@ -84,7 +84,6 @@ __device__ inline int fine_floor(const MeshArgs& A, int c_nte, int16_t v, int u,
 }
 
 constexpr int kWaves = 4;
-constexpr int kInterpSeg = 8;  // wavefronts per coarse cell in the interpolation
 
 // Wavefront per (job, coarse cell of the subtile + 1 halo cell).
 __global__ __launch_bounds__(256) void k_pm3_deposit(MeshArgs A, float* __restrict__ rho, size_t rs,
@@ -270,20 +269,20 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
     block_max_to<1>(f2bits, fv);
 }
 
-// kInterpSeg wavefronts per (job, physical coarse cell), each taking every
-// kInterpSeg-th group of 64 particles (dense cells would otherwise leave one
-// wavefront as the tail): TSC gather and FP64 Delta-v.
+// nseg wavefronts per (job, physical coarse cell), each taking every nseg-th
+// group of 64 particles (dense cells would otherwise leave one wavefront as
+// the tail): TSC gather and FP64 Delta-v.
 __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __restrict__ force,
                                                     double* __restrict__ dv64, int accumulate,
-                                                    const float* __restrict__ scal,
+                                                    const float* __restrict__ scal, int nseg,
                                                     unsigned long long* __restrict__ ptot) {
     const float amid = scal[1], dt = scal[2];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t per = (int64_t)A.ntt * A.ntt * A.ntt;
     const int64_t w0 = (int64_t)blockIdx.x * kWaves + wave;
-    if (w0 >= per * A.njobs * kInterpSeg) return;
-    const int seg = (int)(w0 % kInterpSeg);  // kInterpSeg wavefronts share a cell
-    const int64_t w = w0 / kInterpSeg;
+    if (w0 >= per * A.njobs * nseg) return;
+    const int seg = (int)(w0 % nseg);  // nseg wavefronts share a cell
+    const int64_t w = w0 / nseg;
     const int jb = (int)(w / per);
     const int r = (int)(w - (int64_t)jb * per);
     const PM3Job J = A.jobs[jb];
@@ -298,7 +297,7 @@ __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __r
     const int nf = A.nfp + 2;
     const size_t fs = (size_t)nf * nf * nf;
     const float* F = force + (size_t)jb * 3 * fs;
-    for (int l = seg * 64 + lane; l < np; l += 64 * kInterpSeg) {
+    for (int l = seg * 64 + lane; l < np; l += 64 * nseg) {
         const int64_t ip = base + l;
         const int16_t* p = A.xp + 3 * ip;
         int cx, cy, cz;
@@ -551,9 +550,11 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
         C2_HIP(hipStreamSynchronize(s));
     }
     if (dv64) {
-        const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt * kInterpSeg;
+        const int nseg = ctx().tune[CUBE2_TUNE_INTERP_SEG] ? ctx().tune[CUBE2_TUNE_INTERP_SEG] : 8;
+        const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt * nseg;
         Stage st("pm3_interp", (double)nb * 3 * fs * 4);
-        k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, d_scal, ptot);
+        k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, d_scal,
+                                                              nseg, ptot);
         C2_HIP(hipGetLastError());
     }
     return CUBE2_OK;

@@ -31,7 +31,6 @@
 //   terms accumulate in FP64 (full rate on MI355X) instead of Kahan-
 //   compensated FP32.
 #include <algorithm>
-#include <cstdlib>
 #include <type_traits>
 
 #include "pp_kernels.h"
@@ -433,8 +432,15 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
 int pp_tiled_grid(int mode) {
     static int cache[4] = {0, 0, 0, 0};
     const int m = mode & 3;
-    if (cache[m]) return cache[m];
+    const int forced = ctx().tune[CUBE2_TUNE_PP_BLOCKS_PER_CU];
     int dev = 0, ncu = 0, per = 0;
+    if (forced) {
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
+        return std::max(1, ncu) * forced;
+    }
+    if (cache[m]) return cache[m];
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         ncu = 256;
@@ -446,11 +452,6 @@ int pp_tiled_grid(int mode) {
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModePrecise>, 256, 0);
     if (e != hipSuccess || per < 1) per = 2;
-    // tuning override: blocks per CU of the persistent grid
-    if (const char* env = std::getenv("CUBE2_PP_BLOCKS_PER_CU")) {
-        const int v = std::atoi(env);
-        if (v >= 1 && v <= 64) per = v;
-    }
     cache[m] = std::max(1, ncu) * per;
     return cache[m];
 }
@@ -461,14 +462,8 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
                        : mode == CUBE2_PP_FAST ? "pp_pairs_fastq" : "pp_pairs_precise";
     Stage st(name);
     const unsigned blocks = (unsigned)std::max(1, grid);
-    static int direct_lg = -1;
-    if (direct_lg < 0) {
-        direct_lg = 6;  // full-wavefront chunks sum directly
-        if (const char* env = std::getenv("CUBE2_PP_DIRECT_LG")) {
-            const int v = std::atoi(env);
-            if (v >= 0 && v <= 7) direct_lg = v;
-        }
-    }
+    // full-wavefront chunks sum directly unless tuned otherwise
+    const int direct_lg = ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] ? ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] : 6;
     if (mode == CUBE2_PP_REF)
         k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg);
     else if (mode == CUBE2_PP_FAST)

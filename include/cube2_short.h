@@ -103,6 +103,27 @@ int cube2_get_params(cube2_params_t* p);
  *     per-particle sums (accuracy stated against an FP64 direct sum). */
 int cube2_set_pp_mode(int mode);
 
+/* Launch-shape tunables (no counterpart in the reference; set by the
+ * autotuner in cube2_amd/autotune.py).  value 0 restores the built-in
+ * choice.  Valid values:
+ *   PP_BLOCKS_PER_CU  1..16, blocks per CU of the persistent pair kernel
+ *                     (built-in: the occupancy maximum);
+ *   PP_DIRECT_LG      1..7, chunks of >= 2^v particles sum directly (6);
+ *   FFT_MIN_LGC       1..6, minimum log2 column-tile width of the y/z FFT
+ *                     passes (built-in: ~4096 LDS elements per tile);
+ *   INTERP_SEG        1..32, wavefronts per coarse cell in the PM3
+ *                     interpolation (8).
+ * A change makes the next recorded kick record again. */
+enum {
+    CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
+    CUBE2_TUNE_PP_DIRECT_LG = 1,
+    CUBE2_TUNE_FFT_MIN_LGC = 2,
+    CUBE2_TUNE_INTERP_SEG = 3,
+    CUBE2_TUNE_COUNT = 4
+};
+int cube2_set_tuning(int key, int value);
+int cube2_get_tuning(int key, int* value); /* the value in use (built-in choice resolved) */
+
 /* ---------------------------------------------------------------------
  * Device-resident variants (no counterpart in the reference; they are the
  * same path with particle and cell arrays already in HBM).  rhoc, idx_b_r,

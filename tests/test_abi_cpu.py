@@ -150,3 +150,16 @@ def test_kick_driver_validates_before_device_work():
     args2 = (a(ix2),) + args[1:]
     assert L.cube2_kick_dev(_lib.KICK_PP | _lib.KICK_KEEP_LEVELS, *args2) == _lib.ERR_ARG
     assert b"ixyz3" in L.cube2_last_error()
+
+
+def test_tuning_api_validates_and_round_trips():
+    from cube2_amd import _lib
+    L = _lib.lib()
+    assert L.cube2_set_tuning(99, 1) == _lib.ERR_ARG
+    assert L.cube2_set_tuning(_lib.TUNE_PP_DIRECT_LG, 9) == _lib.ERR_ARG
+    assert L.cube2_set_tuning(_lib.TUNE_INTERP_SEG, 33) == _lib.ERR_ARG
+    _lib.set_tuning(_lib.TUNE_INTERP_SEG, 4)
+    assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 4
+    _lib.set_tuning(_lib.TUNE_INTERP_SEG, 0)
+    assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 8          # built-in choice
+    assert _lib.get_tuning(_lib.TUNE_PP_DIRECT_LG) == 6

@@ -189,3 +189,35 @@ def test_recorded_kick_replays_match_eager(lib, mixed_layout):
         assert r.recorded
         if not pm3:
             assert np.array_equal(k.vp_host(), want)
+
+
+def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
+    """Launch-shape tunables change no integer result and move Delta-v by at
+    most the summation-order level (1e-6); a change makes the recorded kick
+    record again."""
+    import torch
+    from cube2_amd.kick import ShortRangeKick
+    lay, ires = mixed_layout
+    phys = lay.physical_mask()
+    k = ShortRangeKick(lay)
+    k.set_levels(ires)
+    vp0 = k.d_vp.clone()
+    r0 = k.kick_graph(1.0, 0.02, 1e-3)
+    base = k.vp_host()
+    settings = [(lib.TUNE_PP_BLOCKS_PER_CU, 3), (lib.TUNE_PP_DIRECT_LG, 3),
+                (lib.TUNE_FFT_MIN_LGC, 4), (lib.TUNE_INTERP_SEG, 2)]
+    try:
+        for key, v in settings:
+            lib.set_tuning(key, v)
+            k.d_vp.copy_(vp0)
+            torch.cuda.synchronize()
+            r = k.kick_graph(1.0, 0.02, 1e-3)
+            assert r.recorded
+            got = k.vp_host()
+            err = float(np.sqrt((((got - base)[phys]) ** 2).sum() / ((base - lay.vp)[phys] ** 2).sum()))
+            assert err <= 1e-6, (key, v, err)
+            assert (r.p_cand, r.p_acc) == (r0.p_cand, r0.p_acc)
+            assert np.array_equal(r.ptotal_pm3, r0.ptotal_pm3)
+    finally:
+        for key, _ in settings:
+            lib.set_tuning(key, 0)
