@@ -70,10 +70,9 @@ def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s
             break
     pp_rate = f_pp / t_pp
     pp_ms = pp_flops(*p_full) / pp_rate * 1e3
-    # PM3: level-2 subtiles (n = 80), timed, extrapolated by mesh points
-    l2 = [i for i in range(g.ns3) if ires[i] == 2][:4]
-    sub_isort = np.array([i + 1 for i in l2] + [i + 1 for i in range(g.ns3) if i not in l2],
-                         np.int32)
+    # PM3: level-2 subtiles (n = 80), timed until a quarter of the budget,
+    # extrapolated by mesh points
+    l2 = [i for i in range(g.ns3) if ires[i] == 2]
     t0 = time.perf_counter()
     done = 0
     for it in l2:
@@ -86,12 +85,11 @@ def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s
     pts_done = done * g.nft(2) ** 3
     pts_all = sum(g.nft(int(L)) ** 3 for L in ires)
     pm3_ms = t_pm3 / pts_done * pts_all * 1e3
-    del sub_isort
     return dict(value=round(pp_ms + pm3_ms, 1), unit="ms/step", cores=ncpu, kind="port",
                 sample=(f"oracle PP on {len(tiles)} of {g.ntile} parent tiles "
                         f"({t_pp:.1f} s, {pp_rate / 1e9:.2f} GFLOP/s algorithmic, extrapolated "
                         f"by PP flops: {pp_ms:.0f} ms) + oracle PM3 on {done} level-2 subtiles "
-                        f"({t_pm3:.1f} s, extrapolated by mesh points: {pm3_ms:.0f} ms); "
+                        f"({t_pm3:.2f} s, extrapolated by mesh points: {pm3_ms:.0f} ms); "
                         f"OpenMP threads {ncpu}"))
 
 

@@ -23,8 +23,8 @@
 namespace cube2 {
 
 constexpr int kFFTThreads = 256;
-constexpr int kFFTElems = 4096;  // complex elements per LDS buffer (x passes)
-constexpr int kColsMax = 8192;   // largest LDS buffer of the y/z passes
+constexpr int kFFTElems = 2048;  // complex elements per LDS tile (measured best: 2048 > 4096 > 1024)
+constexpr int kColsMax = 4096;   // largest LDS tile of the y/z passes (wider minimum tiles)
 
 __device__ inline float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -341,12 +341,12 @@ static int set_lds_attributes() {
     const int bytes_cols = (kColsMax + 1024) * (int)sizeof(float2);
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, 4096>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, 4096>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, 4096>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, 8192>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, 8192>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, 8192>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kFFTElems>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kFFTElems>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kFFTElems>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kColsMax>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kColsMax>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kColsMax>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
     g_attr_set = true;
     return CUBE2_OK;
 }
@@ -434,7 +434,7 @@ int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s) {
     return CUBE2_OK;
 }
 
-// Column tile width of the y/z passes: about 4096 LDS elements per tile
+// Column tile width of the y/z passes: about kFFTElems LDS elements per tile
 // (the FFT_MIN_LGC tunable raises the minimum width; 16 columns measured
 // slower at n = 320 and 480 for the lower occupancy).
 int fft_cols_lgC(int n) {
@@ -442,6 +442,7 @@ int fft_cols_lgC(int n) {
     const int min_lg = ctx().tune[CUBE2_TUNE_FFT_MIN_LGC];
     int lgC = std::max(lg_floor(std::max(1, kFFTElems / n)), min_lg);
     while (lgC > 0 && (1 << lgC) > 2 * nh) lgC--;  // no wider than the half spectrum needs
+    while (lgC > 0 && (1 << lgC) * n > kColsMax) lgC--;  // the tile must fit the LDS buffer
     return lgC;
 }
 
@@ -454,13 +455,13 @@ int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
     dim3 grid((unsigned)(n * ntiles), (unsigned)nb);
     const size_t lds = (size_t)(n + C * n) * sizeof(float2);
     if (C * n > kFFTElems) {
-        if (mode == 0) k_fft_cols<0, 8192><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else if (mode == 1) k_fft_cols<1, 8192><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else k_fft_cols<2, 8192><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        if (mode == 0) k_fft_cols<0, kColsMax><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else if (mode == 1) k_fft_cols<1, kColsMax><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else k_fft_cols<2, kColsMax><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
     } else {
-        if (mode == 0) k_fft_cols<0, 4096><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else if (mode == 1) k_fft_cols<1, 4096><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else k_fft_cols<2, 4096><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        if (mode == 0) k_fft_cols<0, kFFTElems><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else if (mode == 1) k_fft_cols<1, kFFTElems><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else k_fft_cols<2, kFFTElems><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
     }
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
