@@ -155,7 +155,6 @@ def main(argv=None) -> int:
     t0 = time.perf_counter()
     pos, vel = config1(seed=args.seed)
     lay = build_layout(pos, vel, C1)
-    del pos, vel
     log(f"inputs: {lay.n_particles} particles, {lay.n_used} slots ({time.perf_counter() - t0:.1f} s)")
     _lib.set_pp_mode({"ref": _lib.PP_REF, "exact": _lib.PP_EXACT, "fast": _lib.PP_FAST,
                      "precise": _lib.PP_PRECISE}[args.pp_mode])
@@ -216,6 +215,22 @@ def main(argv=None) -> int:
         graph_ms = 1e3 * wall_graph / args.steps
         ms_step = graph_ms
 
+    # the same layout built on the device from the positions (not part of the kick)
+    from cube2_amd.layout import build_layout_device
+    d_pos = torch.from_numpy(pos).cuda()
+    d_vel = torch.from_numpy(np.ascontiguousarray(vel, np.float32)).cuda()
+    lay_ms = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        d_rhoc, d_idx, d_xp, d_vp, d_n = build_layout_device(d_pos, d_vel, C1)
+        torch.cuda.synchronize()
+        lay_ms.append(1e3 * (time.perf_counter() - t1))
+    assert d_n == lay.n_used and np.array_equal(d_idx.cpu().numpy(), lay.idx_b_r)
+    assert torch.equal(d_xp, k.d_xp), "device layout differs from the host layout"
+    del d_pos, d_vel, d_rhoc, d_idx, d_xp, d_vp, pos, vel
+    log(f"device layout build: {min(lay_ms):.1f} ms (host: identical arrays)")
+
     pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq",
                "precise": "pp_pairs_precise"}[args.pp_mode]
     pp_ms = stats[pp_name]["ms"] / args.steps
@@ -266,6 +281,7 @@ def main(argv=None) -> int:
         "kick_ms_graph_replay": None if graph_ms is None else round(graph_ms, 3),
         "kick_ms_eager": round(eager_ms, 3),
         "kick_ms_first_call": round(cold_ms, 3),
+        "layout_build_ms": round(min(lay_ms), 3),
         "pp_pair_interactions_per_s": float(f"{pair_rate:.4g}"),
         "pp_ms_per_step": round(pp_ms, 3),
         "p_cand": int(res.p_cand),

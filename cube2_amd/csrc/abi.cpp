@@ -118,6 +118,7 @@ void c_gpu_velocity_finalize(void) {
     std::lock_guard<std::recursive_mutex> l(g_mx);
     pp_release();
     kick_release();
+    layout_release();
     Context& C = ctx();
     C.dv64.release();
     C.d_vp.release();
@@ -217,6 +218,13 @@ int cube2_kick_graph_dev(int flags, const int* ixyz3, const float* apm3, const i
     return kick_graph(flags, ixyz3, apm3, ratio_sf, d_rhoc, d_idx_b_r, d_xp, d_vp, n_used,
                       d_green, mass, a_mid, dt, ires, isort, ptotal_pm3, ptotal_pp, ttotal_pp,
                       out, recorded);
+}
+
+int cube2_layout_build_dev(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc,
+                           int64_t* d_idx_b_r, int16_t* d_xp, float* d_vp, int64_t cap,
+                           int64_t* n_used) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return layout_build(d_pos, d_vel, n, d_rhoc, d_idx_b_r, d_xp, d_vp, cap, n_used);
 }
 
 int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches) {

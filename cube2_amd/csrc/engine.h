@@ -85,6 +85,11 @@ int kick_graph(int flags, const int* ixyz3, const float* apm3, const int* ratio_
                double* ttotal_pp, cube2_kick_result_t* out, int* recorded);
 void kick_release();
 
+// ------------------------------------------------------------- layout
+int layout_build(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc,
+                 int64_t* d_idx_b_r, int16_t* d_xp, float* d_vp, int64_t cap, int64_t* n_used);
+void layout_release();
+
 // ---------------------------------------------------------------- misc
 int fill_zero(void* d, size_t bytes, hipStream_t s);  // a kernel, not a memset
 int selftest_div(uint64_t seed, int n, unsigned long long* mismatches);

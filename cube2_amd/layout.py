@@ -65,6 +65,45 @@ class Layout:
         return np.repeat(cellphys, self.rhoc.reshape(-1).astype(np.int64))
 
 
+def build_layout_device(pos, vel, geom: Geometry):
+    """The same layout built on the device by the library
+    (cube2_layout_build_dev).  pos: float64 [n, 3] (numpy or a ROCm tensor),
+    vel: float32 [n, 3] or None.  Returns device tensors
+    (rhoc, idx_b_r, xp, vp) shaped as in ``Layout`` and n_used."""
+    import ctypes as C
+
+    import torch
+
+    from . import _lib
+    geom.validate()
+    _lib.set_params(geom)
+    dev = torch.device("cuda")
+    d_pos = torch.as_tensor(np.ascontiguousarray(pos, np.float64) if isinstance(pos, np.ndarray)
+                            else pos, dtype=torch.float64, device=dev).contiguous()
+    n = d_pos.shape[0]
+    d_vel = None
+    if vel is not None:
+        d_vel = torch.as_tensor(np.ascontiguousarray(vel, np.float32) if isinstance(vel, np.ndarray)
+                                else vel, dtype=torch.float32, device=dev).contiguous()
+    rhoc = torch.empty((geom.ntile, geom.nte, geom.nte, geom.nte), dtype=torch.int32, device=dev)
+    idx = torch.empty((geom.ntile, geom.nte, geom.nte), dtype=torch.int64, device=dev)
+    n_used = C.c_int64(0)
+    L = _lib.lib()
+    rc = L.cube2_layout_build_dev(d_pos.data_ptr(), None if d_vel is None else d_vel.data_ptr(), n,
+                                  rhoc.data_ptr(), idx.data_ptr(), None, None, 0,
+                                  C.byref(n_used))
+    if rc not in (_lib.OK, _lib.ERR_CAPACITY):
+        _lib.check(rc)
+    m = max(n_used.value, 1)
+    xp = torch.empty((m, 3), dtype=torch.int16, device=dev)
+    vp = torch.empty((m, 3), dtype=torch.float32, device=dev)
+    _lib.check(L.cube2_layout_build_dev(d_pos.data_ptr(), None if d_vel is None else d_vel.data_ptr(),
+                                        n, rhoc.data_ptr(), idx.data_ptr(), xp.data_ptr(),
+                                        vp.data_ptr(), m, C.byref(n_used)))
+    k = n_used.value
+    return rhoc, idx, xp[:k], vp[:k], k
+
+
 def encode_positions(pos: np.ndarray, geom: Geometry):
     """Global coarse cell and int16 code of each position (standard cells)."""
     ng = geom.ng

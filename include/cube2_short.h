@@ -156,6 +156,22 @@ int cube2_green_3d_dev(float* d_G, int n, float apm_soft, float apm_range, float
 int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort, int* napm);
 
 /* ---------------------------------------------------------------------
+ * Image layout on the device (no counterpart in the reference's CUDA
+ * files; it is the device form of buffer_grid / spine_image and the
+ * single-image halo copies, buffer_grid.f90:13-155, variables.f90:146-216,
+ * buffer_particle.f90:23-345).  From n positions d_pos[n][3] (FP64,
+ * standard cells, periodic in [0, ng)) and optional velocities d_vel[n][3]
+ * (NULL: zero) it writes d_rhoc [ntile][nte^3], d_idx_b_r [ntile][nte^2]
+ * and, when cap >= the slot count, d_xp [n_used][3] and d_vp [n_used][3].
+ * *n_used is always written; CUBE2_ERR_CAPACITY when cap is too small (or
+ * d_xp/d_vp NULL), so a caller can size and call again.  The particles of
+ * one cell keep their input order (a stable sort).
+ * ------------------------------------------------------------------- */
+int cube2_layout_build_dev(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc,
+                           int64_t* d_idx_b_r, int16_t* d_xp, float* d_vp, int64_t cap,
+                           int64_t* n_used);
+
+/* ---------------------------------------------------------------------
  * Kick driver: the short-range part of kick.f90:107-238 in one call over
  * device-resident arrays (the Fortran host's level selection, the PM3 and
  * PP calls of kick.f90:144-146,202-205, and the dt limits of
