@@ -133,6 +133,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager dispatch only (value = eager kick), e.g. under counters")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-pm2", action="store_true", help="skip the PM2 measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM counters)")
     args = ap.parse_args(argv)
@@ -231,6 +232,40 @@ def main(argv=None) -> int:
     del d_pos, d_vel, d_rhoc, d_idx, d_xp, d_vp, pos, vel
     log(f"device layout build: {min(lay_ms):.1f} ms (host: identical arrays)")
 
+    # PM2 (kick.f90:76-105) on the same image: the 64 parent tiles on the
+    # standard grid, mesh ngt^3 each; timed on its own, not part of the kick
+    pm2 = None
+    if not args.no_pm2:
+        from cube2_amd.kick import KickResult
+        r2 = KickResult()
+        k.green2()
+        t1 = time.perf_counter()
+        k.pm2(mass, amid, dt, r2)
+        pm2_cold = 1e3 * (time.perf_counter() - t1)
+        _lib.profile(True)
+        _lib.profile_reset()
+        pm2_ms = []
+        for s in range(args.steps):
+            t1 = time.perf_counter()
+            k.pm2(mass, amid, dt, r2)          # the pending Delta-v accumulates
+            pm2_ms.append(1e3 * (time.perf_counter() - t1))
+        st2 = _lib.profile_stats()
+        _lib.profile(False)
+        k.apply_pending()
+        torch.cuda.synchronize()
+        n2 = C1.ngt
+        pm2_stages = {n: round(v["ms"] / args.steps, 3) for n, v in
+                      sorted(st2.items(), key=lambda kv: -kv[1]["ms"]) if n.startswith("pm2_")}
+        dn, dst = max(((n, v) for n, v in st2.items() if n.startswith("pm2_")),
+                      key=lambda kv: kv[1]["ms"])
+        gbs = dst["bytes"] / (dst["ms"] * 1e-3) / 1e9
+        pm2 = {"ms": round(min(pm2_ms), 3), "first_call_ms": round(pm2_cold, 3),
+               "mesh": n2, "tiles": C1.ntile, "f2max": r2.f2max_pm2, "stage_ms": pm2_stages,
+               "roofline": dict(kernel=dn, bound="hbm", achieved=round(gbs, 1),
+                                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                                avg_launch_ms=round(dst["ms"] / dst["launches"], 3))}
+        log(f"PM2: {pm2['ms']:.1f} ms ({C1.ntile} tiles, mesh {n2}^3); stages {pm2_stages}")
+
     pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq",
                "precise": "pp_pairs_precise"}[args.pp_mode]
     pp_ms = stats[pp_name]["ms"] / args.steps
@@ -260,6 +295,18 @@ def main(argv=None) -> int:
         log("cpu baseline ...")
         cpu = cpu_baseline(lay, k.ires, k.isort, greens_host, (res.p_cand, res.p_acc), mass, amid,
                            dt, args.cpu_budget)
+        if pm2 is not None:
+            import oracle
+            g2 = k.green2().cpu().numpy()
+            t1 = time.perf_counter()
+            nt2 = 8
+            oracle.pm2_force(C1, lay.rhoc, lay.idx_b_r, lay.xp, g2, mass, amid, dt,
+                             tiles=(0, nt2))
+            t_s = time.perf_counter() - t1
+            pm2["cpu_baseline"] = dict(value=round(t_s / nt2 * C1.ntile * 1e3, 1),
+                                       unit="ms/call", cores=cpu["cores"], kind="port",
+                                       sample=f"oracle PM2 on tiles 0-{nt2 - 1} of {C1.ntile} "
+                                              f"({t_s:.2f} s), scaled by {C1.ntile}/{nt2}")
 
     out = {
         "metric": "kick ms/step + PP pair-int/s @256^3",
@@ -287,6 +334,7 @@ def main(argv=None) -> int:
         "p_cand": int(res.p_cand),
         "p_acc": int(res.p_acc),
         "stage_ms_per_step": stages,
+        "pm2": pm2,
         "roofline": roof,
         "tuning": tuning,
         "cpu_baseline": cpu,

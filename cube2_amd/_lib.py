@@ -35,13 +35,15 @@ class Params(C.Structure):
 
 
 class KickResultC(C.Structure):
-    _fields_ = [("f2max_pm3", C.c_float), ("f2max_pp", C.c_float), ("vmax", C.c_float * 3),
-                ("dt_pm3", C.c_double), ("dt_pp", C.c_double), ("dt_vmax", C.c_double),
+    _fields_ = [("f2max_pm2", C.c_float), ("f2max_pm3", C.c_float), ("f2max_pp", C.c_float),
+                ("vmax", C.c_float * 3),
+                ("dt_pm2", C.c_double), ("dt_pm3", C.c_double), ("dt_pp", C.c_double),
+                ("dt_vmax", C.c_double),
                 ("pt_pm3", C.c_double), ("pt_pp", C.c_double), ("pair_counts", C.c_uint64 * 2),
                 ("napm", C.c_int * 7)]
 
 
-KICK_PM3, KICK_PP, KICK_KEEP_LEVELS = 1, 2, 4
+KICK_PM3, KICK_PP, KICK_KEEP_LEVELS, KICK_PM2 = 1, 2, 4, 8
 
 vp_ = C.c_void_p
 i32 = C.POINTER(C.c_int)
@@ -60,6 +62,7 @@ SIGNATURES = {
     "cube2_pm3_force_dev": (C.c_int, [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3 + [vp_, vp_]),
     "cube2_pp_force_dev": (C.c_int, [vp_] * 8 + [C.c_int64] + [C.c_float] * 3 + [vp_] * 5),
     "cube2_velocity_apply_dev": (C.c_int, [vp_, vp_, vp_, C.c_int64, vp_]),
+    "cube2_pm2_force_dev": (C.c_int, [vp_] * 3 + [C.c_int64, vp_] + [C.c_float] * 3 + [vp_] * 2),
     "cube2_green_3d_dev": (C.c_int, [vp_, C.c_int, C.c_float, C.c_float, C.c_float]),
     "cube2_select_levels": (C.c_int, [vp_] * 5),
     "cube2_layout_build_dev": (C.c_int, [vp_, vp_, C.c_int64] + [vp_] * 4 + [C.c_int64, vp_]),

@@ -147,6 +147,13 @@ int cube2_pp_force_dev(const int* ires, const int* ixyz3, const float* apm3, con
                     a_mid, dt, f2max, vmax, ptotal, ttotal, pair_counts);
 }
 
+int cube2_pm2_force_dev(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                        int64_t n_used, const float* d_green2, float mass, float amid, float dt,
+                        float* f2max, double* ptotal) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return pm2_force(d_rhoc, d_idx_b_r, d_xp, n_used, d_green2, mass, amid, dt, f2max, ptotal);
+}
+
 int cube2_velocity_apply_dev(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp,
                              int64_t n_used, float* vmax) {
     std::lock_guard<std::recursive_mutex> l(g_mx);

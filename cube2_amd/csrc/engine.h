@@ -38,6 +38,7 @@ int nzero_reserve();
 
 // ---------------------------------------------------------------- PM3
 struct PM3Batch {
+    int layer = 3;            // 3: PM3 subtile jobs; 2: PM2 parent-tile jobs
     int level = 0, rsf = 0;
     size_t job0 = 0, nb = 0;  // first job in the device job table, jobs in the batch
     size_t slot0 = 0;         // first ptotal slot
@@ -46,6 +47,8 @@ struct PM3Plan {
     std::vector<PM3Batch> batches;
     std::vector<int> job_it;  // subtile of each ptotal slot
     const float* green[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const float* green2 = nullptr;  // PM2's table (tile order)
+    int nslots_out = 0;             // entries of the caller's ptotal (ns3, or ntile for PM2)
     int64_t n_used = 0;
 };
 int pm3_prepare(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
@@ -57,6 +60,12 @@ int pm3_finish(const PM3Plan& P, float* f2max, double* ptotal);
 int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
               const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
               const float* const* d_green, float mass, float amid, float dt, float* f2max,
+              double* ptotal);
+// PM2 (kick.f90:76-105): one job per parent tile on the standard grid with
+// the Green's table Gk2 (initialize.f90:135); adds to the pending Delta-v.
+int pm2_prepare(int64_t n_used, const float* d_green2, PM3Plan& P);
+int pm2_force(const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
+              const float* d_green2, float mass, float amid, float dt, float* f2max,
               double* ptotal);
 int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf, const int* d_rhoc,
                   const int64_t* d_idx, const int16_t* d_xp, int64_t n_used, const float* d_green,

@@ -15,7 +15,7 @@ struct FFTPlan {
     float2* tw = nullptr;  // exp(-2 pi i t / n), t < n (device)
 };
 
-// Plan for size n (radices 4, 2, 3, 5; cached; freed by fft_release()).
+// Plan for size n (radices 4, 2, 3, 5, 7; cached; freed by fft_release()).
 int fft_plan(int n, const FFTPlan** out);
 void fft_release();
 

@@ -1,7 +1,7 @@
 // Batched PM3 FFTs, hand-written for gfx950.
 //
 // One pass per axis; every pass stages a block of pencils in LDS and runs a
-// mixed-radix (4, 2, 3, 5) self-sorting Stockham transform there, in place
+// mixed-radix (4, 2, 3, 5, 7) self-sorting Stockham transform there, in place
 // with register staging, so HBM sees one read and one write of the mesh per
 // pass.
 //   x, forward : two real rows y, y+1 form one complex line z = a + i b;
@@ -94,6 +94,39 @@ struct DFT<5, S> {
     }
 };
 
+// Seven points: pairs a_m = x_m + x_{7-m}, b_m = x_m - x_{7-m} (m = 1..3);
+// X_k = x_0 + C_k + i S s_k and X_{7-k} = x_0 + C_k - i S s_k with
+// C_k = sum_m a_m cos(2 pi m k / 7), s_k = sum_m b_m sin(2 pi m k / 7).
+template <int S>
+struct DFT<7, S> {
+    __device__ static void run(float2* v) {
+        const float c1 = 0.62348980185873353053f, c2 = -0.22252093395631440429f,
+                    c3 = -0.90096886790241912624f;
+        const float s1 = 0.78183148246802980871f, s2 = 0.97492791218182360702f,
+                    s3 = 0.43388373911755812048f;
+        const float2 x0 = v[0];
+        const float2 a1 = cadd(v[1], v[6]), b1 = csub(v[1], v[6]);
+        const float2 a2 = cadd(v[2], v[5]), b2 = csub(v[2], v[5]);
+        const float2 a3 = cadd(v[3], v[4]), b3 = csub(v[3], v[4]);
+        auto lin = [](float p, float2 x, float q, float2 y, float r, float2 z) {
+            return make_float2(p * x.x + q * y.x + r * z.x, p * x.y + q * y.y + r * z.y);
+        };
+        const float2 C1 = cadd(x0, lin(c1, a1, c2, a2, c3, a3));
+        const float2 C2 = cadd(x0, lin(c2, a1, c3, a2, c1, a3));
+        const float2 C3 = cadd(x0, lin(c3, a1, c1, a2, c2, a3));
+        const float2 S1 = mul_si<S>(lin(s1, b1, s2, b2, s3, b3));
+        const float2 S2 = mul_si<S>(lin(s2, b1, -s3, b2, -s1, b3));
+        const float2 S3 = mul_si<S>(lin(s3, b1, -s1, b2, s2, b3));
+        v[0] = make_float2(x0.x + a1.x + a2.x + a3.x, x0.y + a1.y + a2.y + a3.y);
+        v[1] = cadd(C1, S1);
+        v[6] = csub(C1, S1);
+        v[2] = cadd(C2, S2);
+        v[5] = csub(C2, S2);
+        v[3] = cadd(C3, S3);
+        v[4] = csub(C3, S3);
+    }
+};
+
 // j mod p for 0 <= j < 2^22 via a float reciprocal and one correction.
 __device__ inline int mod_p(int j, int p, float invp) {
     int q = (int)((float)j * invp);
@@ -158,7 +191,8 @@ __device__ void fft_lds(float2* a, const float2* tw, const FFTPlan& P, int lgL, 
         if (r == 4) stage<4, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
         else if (r == 2) stage<2, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
         else if (r == 3) stage<3, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
-        else stage<5, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
+        else if (r == 5) stage<5, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
+        else stage<7, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
     }
 }
 
@@ -364,14 +398,14 @@ int fft_plan(int n, const FFTPlan** out) {
     FFTPlan P;
     P.n = n;
     int m = n;
-    for (int r : {4, 2, 3, 5})
+    for (int r : {4, 2, 3, 5, 7})
         while (m % r == 0) {
             if (P.nst == 16) break;
             P.rad[P.nst++] = r;
             m /= r;
         }
     if (m != 1) {
-        set_error("FFT size %d has a prime factor other than 2, 3, 5", n);
+        set_error("FFT size %d has a prime factor other than 2, 3, 5, 7", n);
         return CUBE2_ERR_CAPACITY;
     }
     for (int st = 0, p = 1; st < P.nst; st++) {

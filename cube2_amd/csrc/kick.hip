@@ -148,7 +148,8 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
          const float* const* d_green, float mass, float a_mid, float dt, int* ires, int* isort,
          double* ptotal_pm3, double* ptotal_pp, double* ttotal_pp, cube2_kick_result_t* out) {
     if (!ixyz3 || !apm3 || !ratio_sf || !ires || !isort || !out ||
-        (flags & ~(CUBE2_KICK_PM3 | CUBE2_KICK_PP | CUBE2_KICK_KEEP_LEVELS))) {
+        (flags & ~(CUBE2_KICK_PM3 | CUBE2_KICK_PP | CUBE2_KICK_KEEP_LEVELS | CUBE2_KICK_PM2)) ||
+        ((flags & CUBE2_KICK_PM2) && (!d_green || !d_green[5]))) {
         set_error("kick: null argument or unknown flag");
         return CUBE2_ERR_ARG;
     }
@@ -174,6 +175,13 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
     if (!ptotal_pm3) ptotal_pm3 = (pt3.assign(g.ns3, 0.0), pt3.data());
     if (!ptotal_pp) ptotal_pp = (ptp.assign(g.ns3, 0.0), ptp.data());
     if (!ttotal_pp) ttotal_pp = (ttp.assign(g.ns3, 0.0), ttp.data());
+    out->dt_pm2 = INFINITY;
+    if (flags & CUBE2_KICK_PM2) {  // kick.f90:76-105, before PM3
+        std::vector<double> pt2(g.ntile, 0.0);
+        C2_TRY(pm2_force(d_rhoc, d_idx, d_xp, n_used, d_green[5], mass, a_mid, dt,
+                         &out->f2max_pm2, pt2.data()));
+        out->dt_pm2 = dt_force(out->f2max_pm2, a_mid);
+    }
     if (flags & CUBE2_KICK_PM3) {
         C2_TRY(pm3_force(isort, ires, ixyz3, ratio_sf, d_rhoc, d_idx, d_xp, n_used, d_green, mass,
                          a_mid, dt, &out->f2max_pm3, ptotal_pm3));
@@ -188,8 +196,8 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
         out->pair_counts[1] = cnt[1];
         for (int it = 0; it < g.ns3; it++) out->pt_pp += ptotal_pp[it];
         out->dt_pp = dt_force(out->f2max_pp, a_mid);
-    } else if (flags & CUBE2_KICK_PM3) {
-        // no PP to apply the pending PM3 Delta-v: apply it here
+    } else if (flags & (CUBE2_KICK_PM3 | CUBE2_KICK_PM2)) {
+        // no PP to apply the pending mesh Delta-v: apply it here
         C2_TRY(velocity_apply(d_rhoc, d_idx, d_vp, n_used, out->vmax));
     }
     if (!(flags & CUBE2_KICK_PM3)) out->dt_pm3 = INFINITY;

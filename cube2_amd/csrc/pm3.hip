@@ -54,8 +54,10 @@ struct MeshArgs {
     const int64_t* nzero;
     const int16_t* xp;
     const PM3Job* jobs;
-    int njobs, nt, ntt, ncb, nte, nnt, ratio_cs;
+    int njobs, nt, ntt, ncb, nte, nnt, ratio_cs;  // ntt: coarse cells per job edge
     int n, nfp, nfb, rsf;
+    int nhalo;  // coarse halo cells around a job whose particles are deposited
+    int layer;  // 3: PM3 subtile jobs, 2: PM2 parent-tile jobs
     int64_t n_used;
 };
 
@@ -166,7 +168,8 @@ __global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* _
         const int cmax = lo[d] + kBrick - off - 1 + J.u[d] * A.nfp;
         const int a = floordiv(cmin, cf) + A.ncb;
         const int e = floordiv(cmax, cf) + A.ncb;
-        const int dlo = J.u[d] * A.ntt + A.ncb - 1, dhi = (J.u[d] + 1) * A.ntt + A.ncb;
+        const int dlo = J.u[d] * A.ntt + A.ncb - A.nhalo,
+                  dhi = (J.u[d] + 1) * A.ntt + A.ncb - 1 + A.nhalo;
         clo[d] = max(a, dlo);
         chi[d] = min(e, dhi);
     }
@@ -481,6 +484,7 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
                      const float* green, float mass, const float* d_scal, int accumulate,
                      double* dv64, unsigned* f2bits, unsigned long long* ptot, int* err,
                      DebugOut* dbg, hipStream_t s) {
+    const bool l2 = base.layer == 2;  // stage names of the profile
     const int n = base.n, nfp = base.nfp;
     const size_t rs = (size_t)(n + 2) * n * n;
     const size_t fs = (size_t)(nfp + 2) * (nfp + 2) * (nfp + 2);
@@ -494,7 +498,7 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     C2_TRY(fft_plan(n, &P));
     {
         const int nbx = (n + kBrick - 1) / kBrick;
-        Stage st("pm3_deposit", (double)nb * n * n * n * 4);
+        Stage st(l2 ? "pm2_deposit" : "pm3_deposit", (double)nb * n * n * n * 4);
         dim3 grid((unsigned)(nbx * nbx * nbx), (unsigned)nb);
         k_pm3_deposit_bricks<<<grid, 256, 0, s>>>(A, rho, rs, nbx, err);
         C2_HIP(hipGetLastError());
@@ -510,23 +514,23 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     }
     const double mesh_bytes = (double)nb * rs * 4;
     {
-        Stage st("pm3_fft_x_r2c", 2 * mesh_bytes);
+        Stage st(l2 ? "pm2_fft_x_r2c" : "pm3_fft_x_r2c", 2 * mesh_bytes);
         C2_TRY(fft_x_r2c(rho, *P, (int)nb, rs, d_scal, s));  // d_scal[0] = mass
     }
     {
-        Stage st("pm3_fft_y", 2 * mesh_bytes);
+        Stage st(l2 ? "pm2_fft_y" : "pm3_fft_y", 2 * mesh_bytes);
         C2_TRY(fft_cols(rho, *P, (int)nb, rs, 1, 0, nullptr, s));
     }
     {
-        Stage st("pm3_fft_z_green", 2 * mesh_bytes + (double)(n / 2 + 1) * n * n * 4);
+        Stage st(l2 ? "pm2_fft_z_green" : "pm3_fft_z_green", 2 * mesh_bytes + (double)(n / 2 + 1) * n * n * 4);
         C2_TRY(fft_cols(rho, *P, (int)nb, rs, 2, 2, green, s));
     }
     {
-        Stage st("pm3_fft_y_inv", 2 * mesh_bytes);
+        Stage st(l2 ? "pm2_fft_y_inv" : "pm3_fft_y_inv", 2 * mesh_bytes);
         C2_TRY(fft_cols(rho, *P, (int)nb, rs, 1, 1, nullptr, s));
     }
     {
-        Stage st("pm3_fft_x_c2r", 2 * mesh_bytes);
+        Stage st(l2 ? "pm2_fft_x_c2r" : "pm3_fft_x_c2r", 2 * mesh_bytes);
         C2_TRY(fft_x_c2r(rho, *P, (int)nb, rs, s));
     }
     if (dbg && dbg->phi) {
@@ -539,7 +543,7 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
                     dbg->phi[((size_t)z * n + y) * n + x] = h[((size_t)z * n + y) * (n + 2) + x];
     }
     {
-        Stage st("pm3_grad", mesh_bytes + (double)nb * 3 * fs * 4);
+        Stage st(l2 ? "pm2_grad" : "pm3_grad", mesh_bytes + (double)nb * 3 * fs * 4);
         const int nfq = nfp + 2;
         dim3 grid((unsigned)(((nfq + 63) / 64) * ((nfq + 3) / 4)), (unsigned)nfq, (unsigned)nb);
         k_pm3_grad<<<grid, 256, 0, s>>>(rho, rs, force, n, nfp, base.nfb, f2bits);
@@ -552,7 +556,7 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     if (dv64) {
         const int nseg = ctx().tune[CUBE2_TUNE_INTERP_SEG] ? ctx().tune[CUBE2_TUNE_INTERP_SEG] : 8;
         const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt * nseg;
-        Stage st("pm3_interp", (double)nb * 3 * fs * 4);
+        Stage st(l2 ? "pm2_interp" : "pm3_interp", (double)nb * 3 * fs * 4);
         k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, d_scal,
                                                               nseg, ptot);
         C2_HIP(hipGetLastError());
@@ -576,7 +580,24 @@ static MeshArgs mesh_args(const Geom& g, const int* d_rhoc, const int64_t* nz, c
     A.nfp = g.ntt * g.p.ratio_cs * rsf;
     A.nfb = g.p.ratio_cs * rsf;
     A.n = A.nfp + 2 * A.nfb;
+    A.nhalo = 1;  // nloop = 1 (kick.f90:108-130 through pm_thread)
+    A.layer = 3;
     A.n_used = n_used;
+    return A;
+}
+
+// PM2 job geometry (kick.f90:81-87): the whole parent tile (nc1..nc2 =
+// 1..nt) on the standard grid (gridsize 1), buffer ngb, all ncb halo cells
+// deposited (nloop = ncb).
+static MeshArgs mesh_args_pm2(const Geom& g, const int* d_rhoc, const int64_t* nz,
+                              const int16_t* d_xp, int64_t n_used) {
+    MeshArgs A = mesh_args(g, d_rhoc, nz, d_xp, n_used, 1);
+    A.ntt = g.nt;
+    A.nfp = g.nt * g.p.ratio_cs;
+    A.nfb = g.p.ngb;
+    A.n = A.nfp + 2 * A.nfb;
+    A.nhalo = g.ncb;
+    A.layer = 2;
     return A;
 }
 
@@ -661,9 +682,11 @@ int pm3_enqueue(const PM3Plan& P, const int* d_rhoc, const int64_t* d_idx, const
     C2_TRY(fill_zero(ptot, g.ns3 * sizeof(unsigned long long), s));
     const PM3Job* d_jobs = g_pm3.jobs.as<PM3Job>();
     for (const PM3Batch& B : P.batches) {
-        MeshArgs A = mesh_args(g, d_rhoc, nz, d_xp, P.n_used, B.rsf);
-        C2_TRY(run_batch(A, d_jobs + B.job0, B.nb, P.green[B.level - 2], 1.0f, d_scal,
-                         accumulate, dv64, f2bits, ptot + B.slot0, err, nullptr, s));
+        const MeshArgs A = B.layer == 2 ? mesh_args_pm2(g, d_rhoc, nz, d_xp, P.n_used)
+                                        : mesh_args(g, d_rhoc, nz, d_xp, P.n_used, B.rsf);
+        const float* green = B.layer == 2 ? P.green2 : P.green[B.level - 2];
+        C2_TRY(run_batch(A, d_jobs + B.job0, B.nb, green, 1.0f, d_scal, accumulate, dv64, f2bits,
+                         ptot + B.slot0, err, nullptr, s));
     }
     char* h = g_pm3.h_res.as<char>();
     C2_HIP(hipMemcpyAsync(h, f2bits, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -683,7 +706,8 @@ int pm3_finish(const PM3Plan& P, float* f2max, double* ptotal) {
         return CUBE2_ERR_ARG;
     }
     *f2max = __builtin_bit_cast(float, r[0]);
-    for (int it = 0; it < g.ns3; it++) ptotal[it] = 0.0;
+    const int nout = P.nslots_out ? P.nslots_out : g.ns3;
+    for (int it = 0; it < nout; it++) ptotal[it] = 0.0;
     for (size_t q = 0; q < P.job_it.size(); q++) ptotal[P.job_it[q]] = (double)hp[q];
     return CUBE2_OK;
 }
@@ -704,6 +728,86 @@ int pm3_force(const int* isort, const int* ires, const int* ixyz3, const int* ra
     }
     PM3Plan P;
     C2_TRY(pm3_prepare(isort, ires, ixyz3, ratio_sf, n_used, d_green, P));
+    const int accumulate = C.dv_state == 1 ? 1 : 0;
+    const float* d_scal = nullptr;
+    C2_TRY(scalars_upload(mass, amid, dt, C.stream, &d_scal));
+    C2_TRY(pm3_enqueue(P, d_rhoc, d_idx, d_xp, d_scal, accumulate, C.dv64.as<double>(), C.stream));
+    C2_TRY(sync_and_flush());
+    C2_TRY(pm3_finish(P, f2max, ptotal));
+    C.dv_state = 1;
+    C.dv_n_used = n_used;
+    return CUBE2_OK;
+}
+
+int pm2_prepare(int64_t n_used, const float* d_green2, PM3Plan& P) {
+    ensure_params();
+    Context& C = ctx();
+    const Geom& g = C.g;
+    if (!d_green2) {
+        set_error("no Green's table for PM2");
+        return CUBE2_ERR_ARG;
+    }
+    if (n_used < 0 || n_used >= ((int64_t)1 << 31)) {
+        set_error("invalid PM2 particle count %lld", (long long)n_used);
+        return CUBE2_ERR_CAPACITY;
+    }
+    C2_TRY(ensure_init());
+    const MeshArgs A = mesh_args_pm2(g, nullptr, nullptr, nullptr, n_used);
+    const FFTPlan* fp = nullptr;
+    C2_TRY(fft_plan(A.n, &fp));
+    C2_TRY(green_tiled(d_green2, A.n, &P.green2, C.stream));
+    P.n_used = n_used;
+    P.nslots_out = g.ntile;
+    P.batches.clear();
+    P.job_it.clear();
+    std::vector<PM3Job> all(g.ntile);
+    for (int t = 0; t < g.ntile; t++) {
+        all[t].t = t;
+        all[t].u[0] = all[t].u[1] = all[t].u[2] = 0;
+        all[t].it = t;
+        P.job_it.push_back(t);
+    }
+    const size_t rs = (size_t)(A.n + 2) * A.n * A.n;
+    const size_t fsz = (size_t)(A.nfp + 2) * (A.nfp + 2) * (A.nfp + 2);
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(g.ntile, g_budget / ((rs + 3 * fsz) * 4)));
+    for (size_t j0 = 0; j0 < (size_t)g.ntile; j0 += batch) {
+        PM3Batch B;
+        B.layer = 2;
+        B.rsf = 1;
+        B.nb = std::min(batch, (size_t)g.ntile - j0);
+        B.job0 = j0;
+        B.slot0 = j0;
+        C2_TRY(batch_reserve(A.n, A.nfp, B.nb));
+        P.batches.push_back(B);
+    }
+    C2_TRY(nzero_reserve());
+    C2_TRY(C.dv64.reserve((size_t)std::max<int64_t>(n_used, 1) * 3 * sizeof(double)));
+    C2_TRY(g_pm3.red.reserve(4 * sizeof(unsigned)));
+    C2_TRY(g_pm3.ptot.reserve(std::max<size_t>(g.ns3, 1) * sizeof(unsigned long long)));
+    C2_TRY(g_pm3.jobs.reserve(all.size() * sizeof(PM3Job)));
+    C2_TRY(g_pm3.h_res.reserve(4 * sizeof(unsigned) + g.ns3 * sizeof(unsigned long long)));
+    C2_HIP(hipMemcpyAsync(g_pm3.jobs.ptr, all.data(), all.size() * sizeof(PM3Job),
+                          hipMemcpyHostToDevice, C.stream));
+    C2_HIP(hipStreamSynchronize(C.stream));
+    buffers_changed();
+    return CUBE2_OK;
+}
+
+int pm2_force(const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, int64_t n_used,
+              const float* d_green2, float mass, float amid, float dt, float* f2max,
+              double* ptotal) {
+    ensure_params();
+    Context& C = ctx();
+    if (!f2max || !ptotal) {
+        set_error("null argument");
+        return CUBE2_ERR_ARG;
+    }
+    if (C.dv_state == 1 && C.dv_n_used != n_used) {
+        set_error("PM2 particle count changed while a Delta-v is pending");
+        return CUBE2_ERR_STATE;
+    }
+    PM3Plan P;
+    C2_TRY(pm2_prepare(n_used, d_green2, P));
     const int accumulate = C.dv_state == 1 ? 1 : 0;
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, amid, dt, C.stream, &d_scal));

@@ -82,6 +82,16 @@ class Geometry:
         """PM3 FFT size n = nfp + 2 nfb (parameters.f90:62-65)."""
         return self.nfp(level) + 2 * self.nfb(level)
 
+    @property
+    def ngp(self) -> int:
+        """Standard cells per tile edge (parameters.f90:54)."""
+        return self.nt * self.ratio_cs
+
+    @property
+    def ngt(self) -> int:
+        """PM2 FFT size ngp + 2 ngb (parameters.f90:58)."""
+        return self.ngp + 2 * self.ngb
+
     def validate(self) -> None:
         if self.ng % self.ratio_cs or self.nc % self.nnt or self.nt % self.nns:
             raise ValueError(f"inconsistent geometry {self}")

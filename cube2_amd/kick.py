@@ -1,10 +1,11 @@
-"""Host-side mirror of the kick driver's short-range part (kick.f90:107-236).
+"""Host-side mirror of the kick driver's short-range part (kick.f90:76-236).
 
 ``ShortRangeKick`` holds one image's layout on the device and runs
 level selection -> PM3 -> PP through the library's C ABI, the way
 kick.f90 calls c_pm3_force_kernel then c_pp_force_kernel, and derives the
 time-step limits dt_pm3, dt_pp (kick.f90:180,227) and dt_vmax
-(kick.f90:238) in FP64.
+(kick.f90:238) in FP64.  PM2 (kick.f90:76-105, a CPU loop over tiles in the
+reference) runs on the same mesh kernels when asked for.
 """
 from __future__ import annotations
 
@@ -41,9 +42,11 @@ def select_levels(geom: Geometry, rhoc: np.ndarray, ix: np.ndarray | None = None
 
 @dataclass
 class KickResult:
+    f2max_pm2: float = 0.0
     f2max_pm3: float = 0.0
     f2max_pp: float = 0.0
     vmax: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+    ptotal_pm2: np.ndarray | None = None
     ptotal_pm3: np.ndarray | None = None
     ptotal_pp: np.ndarray | None = None
     ttotal_pp: np.ndarray | None = None
@@ -53,11 +56,11 @@ class KickResult:
     recorded: bool = False          # kick_graph: this call recorded the graph
 
     def dt_limits(self, a_mid: float) -> dict:
-        """dt_pm3, dt_pp, dt_vmax in FP64 (kick.f90:180,227,238)."""
+        """dt_pm2, dt_pm3, dt_pp, dt_vmax in FP64 (kick.f90:97,180,227,238)."""
         def lim(f2):
             return 0.5 * math.sqrt(1.0 / (math.sqrt(f2) * a_mid * G_GRID)) if f2 > 0 else math.inf
         vm = float(np.max(self.vmax))
-        return dict(dt_pm3=lim(float(self.f2max_pm3)), dt_pp=lim(float(self.f2max_pp)),
+        return dict(dt_pm2=lim(float(self.f2max_pm2)), dt_pm3=lim(float(self.f2max_pm3)), dt_pp=lim(float(self.f2max_pp)),
                     dt_vmax=VBUF * 14 / vm if vm > 0 else math.inf)
 
 
@@ -97,6 +100,32 @@ class ShortRangeKick:
             self.greens[level] = t
         return self.greens[level]
 
+    def green2(self):
+        """Device Gk2 table (initialize.f90:135): Green_3D(ngt, apm2 = 3.5,
+        apm1 = 3.5 ratio_cs, gridsize 1), built once with the FP64 kernel."""
+        if "pm2" not in self.greens:
+            g = self.geom
+            n = g.ngt
+            t = self.torch.empty((n, n, n // 2 + 1), dtype=self.torch.float32, device=self.dev)
+            _lib.check(_lib.lib().cube2_green_3d_dev(t.data_ptr(), n, float(APM3F),
+                                                     float(APM3F * g.ratio_cs), 1.0))
+            self.greens["pm2"] = t
+        return self.greens["pm2"]
+
+    def pm2(self, mass: float, a_mid: float, dt: float, res: KickResult) -> None:
+        """PM2 over every tile (kick.f90:76-105); its Delta-v stays pending
+        like PM3's until PP or the velocity apply."""
+        L = _lib.lib()
+        _lib.set_params(self.geom)
+        f2 = np.zeros(1, np.float32)
+        pt = np.zeros(self.geom.ntile, np.float64)
+        _lib.check(L.cube2_pm2_force_dev(
+            self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_xp.data_ptr(), self.n_used,
+            self.green2().data_ptr(), float(mass), float(a_mid), float(dt), _np_ptr(f2),
+            _np_ptr(pt)))
+        res.f2max_pm2 = float(f2[0])
+        res.ptotal_pm2 = pt
+
     def pm3(self, mass: float, a_mid: float, dt: float, res: KickResult) -> None:
         L = _lib.lib()
         _lib.set_params(self.geom)
@@ -132,20 +161,24 @@ class ShortRangeKick:
         res.ttotal_pp = tt
         res.p_cand, res.p_acc = int(cnt[0]), int(cnt[1])
 
-    def _green_ptrs(self):
-        arr = (C.c_void_p * 5)()
+    def _green_ptrs(self, pm2: bool = False):
+        arr = (C.c_void_p * 6)()
         for lev in sorted(set(int(x) for x in self.ires)):
             arr[lev - 2] = self.green(lev).data_ptr()
+        if pm2:
+            arr[5] = self.green2().data_ptr()
         return arr
 
     def kick(self, mass: float, a_mid: float, dt: float, pm3: bool = True, pp: bool = True,
-             select: bool = False) -> KickResult:
+             select: bool = False, pm2: bool = False) -> KickResult:
         """One short-range kick through the library's kick driver
         (cube2_kick_dev).  select=True re-selects levels on the device from
-        rhoc (kick.f90:107-131); otherwise the current ires/isort are used."""
+        rhoc (kick.f90:107-131); otherwise the current ires/isort are used.
+        pm2=True runs PM2 (kick.f90:76-105) first."""
         L = _lib.lib()
         _lib.set_params(self.geom)
-        flags = (_lib.KICK_PM3 if pm3 else 0) | (_lib.KICK_PP if pp else 0)
+        flags = ((_lib.KICK_PM3 if pm3 else 0) | (_lib.KICK_PP if pp else 0) |
+                 (_lib.KICK_PM2 if pm2 else 0))
         if select:
             self.ires = np.zeros(self.geom.ns3, np.int32)
             self.isort = np.zeros(self.geom.ns3, np.int32)
@@ -163,15 +196,16 @@ class ShortRangeKick:
         _lib.check(L.cube2_kick_dev(
             flags | _lib.KICK_KEEP_LEVELS, _np_ptr(self.ix), _np_ptr(APM3), _np_ptr(RATIO_SF),
             self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_xp.data_ptr(),
-            self.d_vp.data_ptr(), self.n_used, self._green_ptrs(), float(mass), float(a_mid),
+            self.d_vp.data_ptr(), self.n_used, self._green_ptrs(pm2), float(mass), float(a_mid),
             float(dt), _np_ptr(self.ires), _np_ptr(self.isort), _np_ptr(pt3), _np_ptr(ptp),
             _np_ptr(ttp), C.byref(out)))
-        res = KickResult(f2max_pm3=out.f2max_pm3, f2max_pp=out.f2max_pp,
+        res = KickResult(f2max_pm2=out.f2max_pm2, f2max_pm3=out.f2max_pm3, f2max_pp=out.f2max_pp,
                          vmax=np.array(out.vmax[:], np.float32),
                          ptotal_pm3=pt3 if pm3 else None, ptotal_pp=ptp if pp else None,
                          ttotal_pp=ttp if pp else None, p_cand=int(out.pair_counts[0]),
                          p_acc=int(out.pair_counts[1]))
-        res.dt_native = dict(dt_pm3=out.dt_pm3, dt_pp=out.dt_pp, dt_vmax=out.dt_vmax)
+        res.dt_native = dict(dt_pm2=out.dt_pm2, dt_pm3=out.dt_pm3, dt_pp=out.dt_pp,
+                             dt_vmax=out.dt_vmax)
         return res
 
     def kick_graph(self, mass: float, a_mid: float, dt: float, pm3: bool = True) -> KickResult:
@@ -198,6 +232,15 @@ class ShortRangeKick:
         res.dt_native = dict(dt_pm3=out.dt_pm3, dt_pp=out.dt_pp, dt_vmax=out.dt_vmax)
         res.recorded = bool(rec.value)
         return res
+
+    def apply_pending(self) -> np.ndarray:
+        """Adds a pending mesh Delta-v (PM2/PM3 without PP) to the velocities
+        of the physical slots; returns max |vp| per component."""
+        vm = np.zeros(3, np.float32)
+        _lib.check(_lib.lib().cube2_velocity_apply_dev(
+            self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_vp.data_ptr(), self.n_used,
+            _np_ptr(vm)))
+        return vm
 
     def vp_host(self) -> np.ndarray:
         return self.d_vp.cpu().numpy()

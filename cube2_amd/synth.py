@@ -63,7 +63,7 @@ def zeldovich_lattice(npd: int, geom: Geometry, box_mpc: float, z: float, seed: 
     k2 = np.where(nz, kk ** 2, 1.0)
     disp = []
     for kc in (kx, ky, kz):                 # psi_k = i k / k^2 delta_k (x, y, z order)
-        psi = np.fft.irfftn(1j * kc / k2 * dk, s=(npd, npd, npd))
+        psi = np.fft.irfftn(1j * kc / k2 * dk, s=(npd, npd, npd), axes=(0, 1, 2))
         disp.append(psi / (box_mpc / ng))    # Mpc/h -> standard cells
     disp = np.stack([d.reshape(-1) for d in disp], axis=1)
     iz, iy, ix = np.meshgrid(np.arange(npd), np.arange(npd), np.arange(npd), indexing="ij")

@@ -142,6 +142,17 @@ int cube2_pp_force_dev(const int* ires, const int* ixyz3, const float* apm3, con
                        float* f2max, float* vmax, double* ptotal, double* ttotal,
                        uint64_t* pair_counts /* [2] P_cand, P_acc; may be NULL */);
 
+/* PM2 (kick.f90:76-105; the reference runs it on the CPU, pm_thread with
+ * FFTW): one job per parent tile on the standard grid, mesh n = ngp + 2 ngb
+ * (ngp = ng / nnt), the particles of all ncb halo cells deposited, the
+ * Green's table d_green2 = Gk2 [n][n][n/2+1] (initialize.f90:135:
+ * Green_3D(n, apm2 = 3.5, apm1 = 3.5 ratio_cs, gridsize 1)).  Its Delta-v
+ * joins the pending FP64 accumulator like PM3's (applied by PP or
+ * cube2_velocity_apply_dev).  ptotal [nnt^3]: physical particles per tile. */
+int cube2_pm2_force_dev(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                        int64_t n_used, const float* d_green2, float mass, float amid, float dt,
+                        float* f2max, double* ptotal);
+
 /* Applies a pending PM3 Delta-v without PP (the reference leaves it
  * pending until PP, pp_force_kernel.cu:496-498); vmax over physical slots. */
 int cube2_velocity_apply_dev(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp,
@@ -178,9 +189,10 @@ int cube2_layout_build_dev(const double* d_pos, const float* d_vel, int64_t n, i
  * kick.f90:180,227,236).
  * ------------------------------------------------------------------- */
 typedef struct {
+    float f2max_pm2;                /* kick.f90:96 (CUBE2_KICK_PM2) */
     float f2max_pm3, f2max_pp;      /* kick.f90:146,205 */
     float vmax[3];                  /* max |vp| over physical slots after the kick */
-    double dt_pm3, dt_pp, dt_vmax;  /* FP64; +inf for a zero force / velocity or a skipped part */
+    double dt_pm2, dt_pm3, dt_pp, dt_vmax;  /* FP64; +inf for a zero force / velocity or a skipped part */
     double pt_pm3, pt_pp;           /* sum(ptotal) of PM3 and of PP */
     uint64_t pair_counts[2];        /* PP P_cand, P_acc */
     int napm[7];                    /* subtiles per level (kick.f90:129) */
@@ -189,7 +201,8 @@ typedef struct {
 enum {
     CUBE2_KICK_PM3 = 1,          /* run PM3 */
     CUBE2_KICK_PP = 2,           /* run PP (applies PM3's pending Delta-v) */
-    CUBE2_KICK_KEEP_LEVELS = 4   /* use the caller's ires/isort instead of selecting them */
+    CUBE2_KICK_KEEP_LEVELS = 4,  /* use the caller's ires/isort instead of selecting them */
+    CUBE2_KICK_PM2 = 8           /* run PM2 first; d_green[5] is its table (Gk2) */
 };
 
 /* flags: CUBE2_KICK_*.  ires/isort [ns3] are written (level selection of

@@ -70,6 +70,9 @@ def lib():
         L.or_pm3_force.argtypes = [C.POINTER(OrGeom), i32p, i32p, i32p, i32p, i32p, i64p, i16p,
                                    C.POINTER(C.c_void_p), C.c_float, C.c_float, C.c_float, f64p,
                                    f32p, f64p]
+        L.or_pm2_force.restype = C.c_int
+        L.or_pm2_force.argtypes = [C.POINTER(OrGeom), i32p, i64p, i16p, f32p, C.c_float, C.c_float,
+                                   C.c_float, f64p, f32p, f64p, C.c_int, C.c_int]
         L.or_velocity_apply.argtypes = [C.POINTER(OrGeom), i32p, i64p, f32p, f64p, f32p]
         L.or_direct_pp_f64.argtypes = [f64p, C.c_int, C.c_double, i32p, C.c_int, f64p, C.c_double,
                                        f64p]
@@ -235,6 +238,23 @@ def pm3_force(geom, isort, ires, ixyz3, ratio_sf, rhoc, idx_b_r, xp, greens, mas
                         _c(xp, np.int16), arr, float(mass), float(amid), float(dt), dv, f2, pt)
     if rc:
         raise ValueError("oracle rejected the PM3 inputs")
+    return dict(dv64=dv, f2max=float(f2[0]), ptotal=pt)
+
+
+def pm2_force(geom, rhoc, idx_b_r, xp, green2, mass, amid, dt, tiles=None):
+    """green2: float32 (n,n,n/2+1), n = nt*ratio_cs + 2*ngb; tiles: (first,
+    count) or None for all.  Returns dict(dv64, f2max, ptotal)."""
+    L = lib()
+    g = geom_struct(geom)
+    n_used = int(np.asarray(idx_b_r).reshape(-1)[-1])
+    dv = np.zeros((n_used, 3), np.float64)
+    f2 = np.zeros(1, np.float32)
+    pt = np.zeros(geom.nnt ** 3, np.float64)
+    rc = L.or_pm2_force(C.byref(g), _c(rhoc, np.int32), _c(idx_b_r, np.int64), _c(xp, np.int16),
+                        _c(green2, np.float32), float(mass), float(amid), float(dt), dv, f2, pt,
+                        *((0, 0) if tiles is None else (int(tiles[0]), int(tiles[1]))))
+    if rc:
+        raise ValueError("oracle rejected the PM2 inputs")
     return dict(dv64=dv, f2max=float(f2[0]), ptotal=pt)
 
 

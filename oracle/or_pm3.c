@@ -287,8 +287,12 @@ static inline void tsc_w(double xpos, int* c, float w[3]) {
     w[1] = 1.0f - w[0] - w[2];
 }
 
+/* One mesh job: ncj coarse cells per edge starting at coarse cell u*ncj of
+ * the tile (PM3: a subtile, ncj = ntt; PM2: the whole tile, ncj = nt), the
+ * particles of nhalo further coarse cells on every side deposited
+ * (pm%nloop: 1 for PM3, ncb for PM2). */
 typedef struct {
-    int L, rsf, nfp, nfb, n, tile, u[3];
+    int L, rsf, nfp, nfb, n, tile, u[3], ncj, nhalo;
 } job_t;
 
 static int make_job(const or_geom* g, const derived* d, int it, const int* ires, const int* ixyz3,
@@ -302,7 +306,23 @@ static int make_job(const or_geom* g, const derived* d, int it, const int* ires,
     j->n = j->nfp + 2 * j->nfb;
     j->tile = ((x[5] - 1) * g->nnt + (x[4] - 1)) * g->nnt + (x[3] - 1);
     for (int c = 0; c < 3; c++) j->u[c] = x[c] - 1;
+    j->ncj = d->ntt;
+    j->nhalo = 1;
     return 0;
+}
+
+/* PM2 job of tile t (kick.f90:80-87): standard grid (gridsize 1), mesh
+ * ngp + 2 ngb with ngp = nt ratio_cs, all ncb halo cells (nloop = ncb). */
+static void make_job_pm2(const or_geom* g, const derived* d, int t, job_t* j) {
+    j->L = 0;
+    j->rsf = 1;
+    j->nfp = d->nt * g->ratio_cs;
+    j->nfb = g->ngb;
+    j->n = j->nfp + 2 * j->nfb;
+    j->tile = t;
+    for (int c = 0; c < 3; c++) j->u[c] = 0;
+    j->ncj = d->nt;
+    j->nhalo = d->ncb;
 }
 
 /* Mesh coordinate (fine cells, subtile origin) of particle slot ip in
@@ -313,25 +333,22 @@ static inline double mesh_coord(const or_geom* g, const derived* d, const job_t*
     return xc * (double)(g->ratio_cs * j->rsf) - (double)(j->u[axis] * j->nfp);
 }
 
-int or_pm3_job(const or_geom* g, int it, const int* ires, const int* ixyz3, const int* ratio_sf,
-               const int* rhoc, const int64_t* idx_b_r, const int16_t* xp, const float* green,
-               float mass, float* rho_out, double* phi_out, double* force_out,
-               double* f2max_job) {
-    derived d = derive(g);
-    job_t jb;
-    if (make_job(g, &d, it, ires, ixyz3, ratio_sf, &jb)) return -1;
+/* pm_thread's deposit, FFT solve and potential_to_force for one job
+ * (kick.f90:419-494, 545-568). */
+static void run_job(const or_geom* g, const derived* dp, job_t jb, const int* rhoc,
+                    const int64_t* idx_b_r, const int16_t* xp, const float* green, float mass,
+                    float* rho_out, double* phi_out, double* force_out, double* f2max_job) {
+    const derived d = *dp;
     const int n = jb.n, nfb = jb.nfb, nfp = jb.nfp, ne = n + 2;
     const int* rc = rhoc + (size_t)jb.tile * d.nte * d.nte * d.nte;
     const int64_t* ib = idx_b_r + (size_t)jb.tile * d.nte * d.nte;
     /* rho_th over Fortran indices [-nfb, nfp+nfb+1] (nex = nfb+1) */
     float* rt = (float*)calloc((size_t)ne * ne * ne, sizeof(float));
-    const int c_lo = jb.u[0] * d.ntt + d.ncb - 1; /* nc1-1 as 0-based nte index */
-    int lo[3], hi[3];
+    int lo[3], hi[3]; /* nc1-nloop .. nc2+nloop as 0-based nte indices */
     for (int c = 0; c < 3; c++) {
-        lo[c] = jb.u[c] * d.ntt + d.ncb - 1;
-        hi[c] = (jb.u[c] + 1) * d.ntt + d.ncb;
+        lo[c] = jb.u[c] * jb.ncj + d.ncb - jb.nhalo;
+        hi[c] = (jb.u[c] + 1) * jb.ncj + d.ncb - 1 + jb.nhalo;
     }
-    (void)c_lo;
     for (int layer = 0; layer < 3; layer++)
         for (int k = lo[2] + layer; k <= hi[2]; k += 3)
             for (int jj = lo[1]; jj <= hi[1]; jj++)
@@ -406,7 +423,60 @@ int or_pm3_job(const or_geom* g, int it, const int* ires, const int* ixyz3, cons
     free(phi);
     if (f2max_job) *f2max_job = f2;
     if (!force_out) free(F);
+}
+
+int or_pm3_job(const or_geom* g, int it, const int* ires, const int* ixyz3, const int* ratio_sf,
+               const int* rhoc, const int64_t* idx_b_r, const int16_t* xp, const float* green,
+               float mass, float* rho_out, double* phi_out, double* force_out,
+               double* f2max_job) {
+    derived d = derive(g);
+    job_t jb;
+    if (make_job(g, &d, it, ires, ixyz3, ratio_sf, &jb)) return -1;
+    run_job(g, &d, jb, rhoc, idx_b_r, xp, green, mass, rho_out, phi_out, force_out, f2max_job);
     return 0;
+}
+
+/* pm_thread's velocity update (kick.f90:497-538) of the job's physical
+ * cells into the FP64 Delta-v; returns their particle count. */
+static long interp_job(const or_geom* g, const derived* dp, const job_t* jbp, const int* rhoc,
+                       const int64_t* idx_b_r, const int16_t* xp, const double* F, double kick,
+                       double* dv64) {
+    const derived d = *dp;
+    const job_t jb = *jbp;
+    const int nf = jb.nfp + 2;
+    const int* rc = rhoc + (size_t)jb.tile * d.nte * d.nte * d.nte;
+    const int64_t* ib = idx_b_r + (size_t)jb.tile * d.nte * d.nte;
+    const size_t fs = (size_t)nf * nf * nf;
+    long cnt = 0;
+    for (int k = jb.u[2] * jb.ncj + d.ncb; k < (jb.u[2] + 1) * jb.ncj + d.ncb; k++)
+        for (int jj = jb.u[1] * jb.ncj + d.ncb; jj < (jb.u[1] + 1) * jb.ncj + d.ncb; jj++)
+            for (int i = jb.u[0] * jb.ncj + d.ncb; i < (jb.u[0] + 1) * jb.ncj + d.ncb; i++) {
+                int np = rc[((size_t)k * d.nte + jj) * d.nte + i];
+                int64_t nz = nzero_of(&d, rc, ib, k, jj, i);
+                cnt += np;
+                for (int l = 0; l < np; l++) {
+                    size_t ip = (size_t)(nz + l);
+                    const int16_t* p = xp + 3 * ip;
+                    int cx, cy, cz;
+                    float wx[3], wy[3], wz[3];
+                    tsc_w(mesh_coord(g, &d, &jb, i, p[0], 0), &cx, wx);
+                    tsc_w(mesh_coord(g, &d, &jb, jj, p[1], 1), &cy, wy);
+                    tsc_w(mesh_coord(g, &d, &jb, k, p[2], 2), &cz, wz);
+                    double dv[3] = {0, 0, 0};
+                    /* force index of the centre cell = cx+1 (Fortran) */
+                    for (int a3 = 0; a3 < 3; a3++)
+                        for (int a2 = 0; a2 < 3; a2++)
+                            for (int a1 = 0; a1 < 3; a1++) {
+                                size_t u = ((size_t)(cz + a3) * nf + (cy + a2)) * nf + (cx + a1);
+                                double wgt = (double)wx[a1] * (double)wy[a2] * (double)wz[a3];
+                                dv[0] += F[u] * wgt;
+                                dv[1] += F[fs + u] * wgt;
+                                dv[2] += F[2 * fs + u] * wgt;
+                            }
+                    for (int c = 0; c < 3; c++) dv64[3 * ip + c] += dv[c] * kick;
+                }
+            }
+    return cnt;
 }
 
 int or_pm3_force(const or_geom* g, const int* isort, const int* ires, const int* ixyz3,
@@ -424,43 +494,38 @@ int or_pm3_force(const or_geom* g, const int* isort, const int* ires, const int*
         const int nf = jb.nfp + 2;
         double* F = (double*)malloc(sizeof(double) * 3 * (size_t)nf * nf * nf);
         double fj = 0;
-        if (or_pm3_job(g, it, ires, ixyz3, ratio_sf, rhoc, idx_b_r, xp, greens[jb.L], mass, NULL,
-                       NULL, F, &fj))
-            return -1;
+        run_job(g, &d, jb, rhoc, idx_b_r, xp, greens[jb.L], mass, NULL, NULL, F, &fj);
         if (fj > f2) f2 = fj;
-        const int* rc = rhoc + (size_t)jb.tile * d.nte * d.nte * d.nte;
-        const int64_t* ib = idx_b_r + (size_t)jb.tile * d.nte * d.nte;
-        const size_t fs = (size_t)nf * nf * nf;
-        long cnt = 0;
-        for (int k = jb.u[2] * d.ntt + d.ncb; k < (jb.u[2] + 1) * d.ntt + d.ncb; k++)
-            for (int jj = jb.u[1] * d.ntt + d.ncb; jj < (jb.u[1] + 1) * d.ntt + d.ncb; jj++)
-                for (int i = jb.u[0] * d.ntt + d.ncb; i < (jb.u[0] + 1) * d.ntt + d.ncb; i++) {
-                    int np = rc[((size_t)k * d.nte + jj) * d.nte + i];
-                    int64_t nz = nzero_of(&d, rc, ib, k, jj, i);
-                    cnt += np;
-                    for (int l = 0; l < np; l++) {
-                        size_t ip = (size_t)(nz + l);
-                        const int16_t* p = xp + 3 * ip;
-                        int cx, cy, cz;
-                        float wx[3], wy[3], wz[3];
-                        tsc_w(mesh_coord(g, &d, &jb, i, p[0], 0), &cx, wx);
-                        tsc_w(mesh_coord(g, &d, &jb, jj, p[1], 1), &cy, wy);
-                        tsc_w(mesh_coord(g, &d, &jb, k, p[2], 2), &cz, wz);
-                        double dv[3] = {0, 0, 0};
-                        /* force index of the centre cell = cx+1 (Fortran) */
-                        for (int a3 = 0; a3 < 3; a3++)
-                            for (int a2 = 0; a2 < 3; a2++)
-                                for (int a1 = 0; a1 < 3; a1++) {
-                                    size_t u = ((size_t)(cz + a3) * nf + (cy + a2)) * nf + (cx + a1);
-                                    double wgt = (double)wx[a1] * (double)wy[a2] * (double)wz[a3];
-                                    dv[0] += F[u] * wgt;
-                                    dv[1] += F[fs + u] * wgt;
-                                    dv[2] += F[2 * fs + u] * wgt;
-                                }
-                        for (int c = 0; c < 3; c++) dv64[3 * ip + c] += dv[c] * kick;
-                    }
-                }
+        const long cnt = interp_job(g, &d, &jb, rhoc, idx_b_r, xp, F, kick, dv64);
         ptotal[it] = (double)cnt;
+        free(F);
+    }
+    *f2max = (float)f2;
+    return 0;
+}
+
+/* PM2 (kick.f90:76-105): pm_thread with pm_layer 2 on tiles tile0 ..
+ * tile0+ntiles-1 (ntiles <= 0: all), Green's table Gk2 [n][n][n/2+1],
+ * n = ngp + 2 ngb.  Adds to dv64; ptotal[t]: the particles of tile t's
+ * physical cells. */
+int or_pm2_force(const or_geom* g, const int* rhoc, const int64_t* idx_b_r, const int16_t* xp,
+                 const float* green2, float mass, float amid, float dt, double* dv64,
+                 float* f2max, double* ptotal, int tile0, int ntiles) {
+    derived d = derive(g);
+    double f2 = 0.0;
+    const int ntile = g->nnt * g->nnt * g->nnt;
+    const int t_end = ntiles > 0 ? tile0 + ntiles : ntile;
+    if (tile0 < 0 || t_end > ntile) return -1;
+    const double kick = (double)amid * (double)dt / (6.0 * kPi);
+    for (int t = tile0; t < t_end; t++) {
+        job_t jb;
+        make_job_pm2(g, &d, t, &jb);
+        const int nf = jb.nfp + 2;
+        double* F = (double*)malloc(sizeof(double) * 3 * (size_t)nf * nf * nf);
+        double fj = 0;
+        run_job(g, &d, jb, rhoc, idx_b_r, xp, green2, mass, NULL, NULL, F, &fj);
+        if (fj > f2) f2 = fj;
+        ptotal[t] = (double)interp_job(g, &d, &jb, rhoc, idx_b_r, xp, F, kick, dv64);
         free(F);
     }
     *f2max = (float)f2;

@@ -98,6 +98,14 @@ int or_pm3_force(const or_geom* g, const int* isort, const int* ires, const int*
                  const float* const* greens, float mass, float amid, float dt, double* dv64,
                  float* f2max, double* ptotal);
 
+/* PM2 (kick.f90:76-105) over tiles tile0 .. tile0+ntiles-1 (ntiles <= 0:
+ * all): accumulates Delta-v into dv64[3*N_used], f2max (max over tile
+ * interiors of |F|^2), ptotal[nnt^3] (particles of each tile's physical
+ * cells).  green2: Gk2 [n][n][n/2+1], n = nt ratio_cs + 2 ngb. */
+int or_pm2_force(const or_geom* g, const int* rhoc, const int64_t* idx_b_r, const int16_t* xp,
+                 const float* green2, float mass, float amid, float dt, double* dv64,
+                 float* f2max, double* ptotal, int tile0, int ntiles);
+
 /* Apply a pending Delta-v to vp for slots in physical cells, return vmax. */
 void or_velocity_apply(const or_geom* g, const int* rhoc, const int64_t* idx_b_r, float* vp,
                        const double* dv64, float* vmax);

@@ -30,7 +30,7 @@ def lib():
     return _lib
 
 
-@pytest.mark.parametrize("n", [48, 80, 144, 160, 240])
+@pytest.mark.parametrize("n", [48, 56, 80, 112, 144, 160, 240])
 def test_fft_r2c_c2r_match_numpy(lib, n):
     import torch
     nb = 2
