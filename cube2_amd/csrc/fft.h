@@ -12,6 +12,7 @@ struct FFTPlan {
     int rad[16] = {0};
     int pst[16] = {0};      // product of the radices before stage s
     float invp[16] = {0};   // 1/pst[s], for division by a runtime constant
+    bool has7 = false;      // a radix-7 stage: the kernels with that butterfly
     float2* tw = nullptr;  // exp(-2 pi i t / n), t < n (device)
 };
 

@@ -183,7 +183,9 @@ __device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ 
 
 // Transforms 2^lgL lines (at most CAP elements) held in LDS buffer a in
 // place.  Ends with a barrier.
-template <int S, int CAP = kFFTElems>
+// R7: the radix-7 butterfly is compiled in only for sizes with a factor 7
+// (its registers would lower the occupancy of every other size).
+template <int S, int CAP = kFFTElems, bool R7 = false>
 __device__ void fft_lds(float2* a, const float2* tw, const FFTPlan& P, int lgL, int ls, int es) {
     for (int s = 0; s < P.nst; s++) {
         const int r = P.rad[s], p = P.pst[s];
@@ -192,7 +194,7 @@ __device__ void fft_lds(float2* a, const float2* tw, const FFTPlan& P, int lgL, 
         else if (r == 2) stage<2, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
         else if (r == 3) stage<3, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
         else if (r == 5) stage<5, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
-        else stage<7, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
+        else if constexpr (R7) stage<7, S, CAP>(a, tw, P.n, p, ip, lgL, ls, es);
     }
 }
 
@@ -213,6 +215,7 @@ __device__ inline float* row_pair(float* d, int g, int n, size_t rs) {
 
 // Lines are row pairs; 2^lgL of them per block, one wavefront per line at a
 // time along the row (coalesced).
+template <bool R7>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d, FFTPlan P,
                                                            int npairs, size_t rs, int lgL,
                                                            const float* __restrict__ scale_dev) {
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d
         }
     }
     __syncthreads();
-    fft_lds<-1>(A, tw, P, lgL, 1, es);
+    fft_lds<-1, kFFTElems, R7>(A, tw, P, lgL, 1, es);
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) break;
         float2* ra = reinterpret_cast<float2*>(row_pair(d, g0 + l, n, rs));
@@ -247,6 +250,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d
     }
 }
 
+template <bool R7>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d, FFTPlan P,
                                                            int npairs, size_t rs, int lgL) {
     extern __shared__ float2 sm[];
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
         }
     }
     __syncthreads();
-    fft_lds<1>(A, tw, P, lgL, 1, es);
+    fft_lds<1, kFFTElems, R7>(A, tw, P, lgL, 1, es);
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) break;
         float* ra = row_pair(d, g0 + l, n, rs);
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
 
 // Tiles of C = 2^lgC consecutive kx columns x n rows along y (axis 1) or z
 // (axis 2); LDS element (row i, column c) at i*C + c.
-template <int MODE, int CAP>
+template <int MODE, int CAP, bool R7>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dreal, FFTPlan P,
                                                           size_t rs, int axis, int lgC,
                                                           const float* __restrict__ green) {
@@ -319,9 +323,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
     }
     __syncthreads();
     if (MODE == 1) {
-        fft_lds<1, CAP>(A, tw, P, lgC, 1, C);
+        fft_lds<1, CAP, R7>(A, tw, P, lgC, 1, C);
     } else {
-        fft_lds<-1, CAP>(A, tw, P, lgC, 1, C);
+        fft_lds<-1, CAP, R7>(A, tw, P, lgC, 1, C);
         if (MODE == 2) {
             // Green's table in tile order: [y][tile][z][c], contiguous per tile
             const float* gt = green + ((size_t)o * ntiles + tile) * n * C;
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
                 }
             }
             __syncthreads();
-            fft_lds<1, CAP>(A, tw, P, lgC, 1, C);
+            fft_lds<1, CAP, R7>(A, tw, P, lgC, 1, C);
         }
     }
     for (int e = threadIdx.x; e < n * C; e += kFFTThreads) {
@@ -368,19 +372,28 @@ __global__ void k_green_tile(const float* __restrict__ G, float* __restrict__ T,
 static std::map<int, FFTPlan> g_plans;
 static bool g_attr_set = false;
 
-static int set_lds_attributes() {
-    if (g_attr_set) return CUBE2_OK;
+template <bool R7>
+static int set_lds_attributes_r() {
     // x passes: n twiddles + n (L+1) <= n + kFFTElems + n elements
     const int bytes = (kFFTElems + 2048) * (int)sizeof(float2);
     const int bytes_cols = (kColsMax + 1024) * (int)sizeof(float2);
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kFFTElems>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kFFTElems>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kFFTElems>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kColsMax>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kColsMax>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kColsMax>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes_cols));
+    const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_r2c<R7>, a, bytes));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_c2r<R7>, a, bytes));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kFFTElems, R7>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kFFTElems, R7>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kFFTElems, R7>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kColsMax, R7>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kColsMax, R7>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kColsMax, R7>, a, bytes_cols));
+    return CUBE2_OK;
+}
+
+static int set_lds_attributes() {
+    if (g_attr_set) return CUBE2_OK;
+    // x passes: n twiddles + n (L+1) <= n + kFFTElems + n elements
+    C2_TRY(set_lds_attributes_r<false>());
+    C2_TRY(set_lds_attributes_r<true>());
     g_attr_set = true;
     return CUBE2_OK;
 }
@@ -402,6 +415,7 @@ int fft_plan(int n, const FFTPlan** out) {
         while (m % r == 0) {
             if (P.nst == 16) break;
             P.rad[P.nst++] = r;
+            P.has7 |= r == 7;
             m /= r;
         }
     if (m != 1) {
@@ -448,7 +462,9 @@ int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale,
         return CUBE2_ERR_CAPACITY;
     }
     const size_t lds = (size_t)(n + (L + 1) * n) * sizeof(float2);
-    k_fft_x_r2c<<<(unsigned)((npairs + L - 1) / L), kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
+    const unsigned grid = (unsigned)((npairs + L - 1) / L);
+    if (P.has7) k_fft_x_r2c<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
+    else k_fft_x_r2c<false><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
@@ -463,7 +479,9 @@ int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s) {
         return CUBE2_ERR_CAPACITY;
     }
     const size_t lds = (size_t)(n + (L + 1) * n) * sizeof(float2);
-    k_fft_x_c2r<<<(unsigned)((npairs + L - 1) / L), kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL);
+    const unsigned grid = (unsigned)((npairs + L - 1) / L);
+    if (P.has7) k_fft_x_c2r<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL);
+    else k_fft_x_c2r<false><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
@@ -480,6 +498,20 @@ int fft_cols_lgC(int n) {
     return lgC;
 }
 
+template <bool R7>
+static void launch_cols(bool wide, int mode, dim3 grid, size_t lds, hipStream_t s, float* d,
+                        const FFTPlan& P, size_t rs, int axis, int lgC, const float* green) {
+    if (wide) {
+        if (mode == 0) k_fft_cols<0, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else if (mode == 1) k_fft_cols<1, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else k_fft_cols<2, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+    } else {
+        if (mode == 0) k_fft_cols<0, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else if (mode == 1) k_fft_cols<1, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        else k_fft_cols<2, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+    }
+}
+
 int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
              const float* green, hipStream_t s) {
     const int n = P.n, nh = n / 2 + 1;
@@ -488,15 +520,8 @@ int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
     const int ntiles = (nh + C - 1) / C;
     dim3 grid((unsigned)(n * ntiles), (unsigned)nb);
     const size_t lds = (size_t)(n + C * n) * sizeof(float2);
-    if (C * n > kFFTElems) {
-        if (mode == 0) k_fft_cols<0, kColsMax><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else if (mode == 1) k_fft_cols<1, kColsMax><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else k_fft_cols<2, kColsMax><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-    } else {
-        if (mode == 0) k_fft_cols<0, kFFTElems><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else if (mode == 1) k_fft_cols<1, kFFTElems><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else k_fft_cols<2, kFFTElems><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-    }
+    if (P.has7) launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green);
+    else launch_cols<false>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

@@ -18,6 +18,7 @@
 // The Delta-v stays pending in the library's FP64 accumulator until PP
 // applies it (pp_force_kernel.cu:496-498).
 #include <algorithm>
+#include <climits>
 #include <map>
 
 #include "common.h"
@@ -25,6 +26,7 @@
 #include "profile.h"
 #include "engine.h"
 #include "reduce.h"
+#include "scan.h"
 
 namespace cube2 {
 
@@ -35,7 +37,7 @@ struct PM3Job {
 };
 
 struct PM3State {
-    DevBuf rho, force, jobs, red, ptot;
+    DevBuf rho, force, jobs, red, ptot, dep_items, dep_scan, dep_list, dep_cnts;
     HostBuf h_res;  // f2max bits, error flag, per-job particle counts (pinned)
 };
 static PM3State g_pm3;
@@ -87,76 +89,238 @@ __device__ inline int fine_floor(const MeshArgs& A, int c_nte, int16_t v, int u,
 
 constexpr int kWaves = 4;
 
-// Wavefront per (job, coarse cell of the subtile + 1 halo cell).
-__global__ __launch_bounds__(256) void k_pm3_deposit(MeshArgs A, float* __restrict__ rho, size_t rs,
-                                                     int* __restrict__ err) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ne = A.ntt + 2;
-    const int64_t per = (int64_t)ne * ne * ne;
-    const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
-    if (w >= per * A.njobs) return;
-    const int jb = (int)(w / per);
-    const int r = (int)(w - (int64_t)jb * per);
+// Cell-group deposit.  A work item is (job, group of g^3 coarse cells of
+// the job's deposit range, split, z-slab): it visits one share (every
+// nsplit-th chunk of 256 in the group's row order) of the group's particles
+// once and sums their TSC weights in LDS over the region the group's
+// particles reach, (g cf + 2)^2 fine cells x the slab's Zs planes.  A group
+// is split so that no item has more than kDepSplit + 256 particles: dense
+// clumps spread over many blocks.  LDS sums are 32-bit fixed point: LDS
+// integer atomics run several times faster than float ones and the sums do
+// not depend on the order of the adds.  An item with at most c < 2^b
+// particles scales its weights (each <= 0.75^3) by 2^(33-b), capped at
+// 2^30, so that a point's sum stays below 0.75^3 2^33 < 2^32; light items
+// keep ~30 fraction bits, dense ones (large sums, where an FP32 sum's own
+// rounding is coarser) at least 18.  When a group is not split, its region's interior
+// (2 or more cells from its faces) belongs to the item alone and is stored;
+// the shell that neighbouring groups overlap and the points of split groups
+// are added atomically.  The mesh is cleared before, so zero sums are
+// skipped.  Mesh points outside [0, n)^3 are dropped (kick.f90:463: only
+// rho_th(m1:m2) is kept).
+constexpr int kDepThreads = 256;
+constexpr int kDepLdsBytes = 80 * 1024;  // two blocks per CU
+constexpr int kDepSplit = 8192;          // particles per work item at most
+
+struct DepositShape {
+    int g, cf, E, Zs, ngx, nslab;
+    size_t lds;
+    int64_t groups;  // ngx^3 per job
+};
+
+static DepositShape deposit_shape(const MeshArgs& A) {
+    DepositShape S;
+    S.cf = A.ratio_cs * A.rsf;
+    S.g = std::max(1, std::min(4, 16 / S.cf));
+    S.E = S.g * S.cf + 2;
+    const int D = A.ntt + 2 * A.nhalo;
+    S.ngx = (D + S.g - 1) / S.g;
+    const int zmax = std::max(1, kDepLdsBytes / (int)sizeof(unsigned) / (S.E * S.E));
+    S.nslab = (S.E + zmax - 1) / zmax;
+    S.Zs = (S.E + S.nslab - 1) / S.nslab;
+    S.lds = (size_t)S.E * S.E * S.Zs * sizeof(unsigned);
+    S.groups = (int64_t)S.ngx * S.ngx * S.ngx;
+    return S;
+}
+
+// Coarse cells [c0, c1) (nte indices) of group gi of job J.
+__device__ inline void group_cells(const MeshArgs& A, const PM3Job& J, int g, int ngx, int q,
+                                   int c0[3], int c1[3]) {
+    const int gi[3] = {q % ngx, (q / ngx) % ngx, q / (ngx * ngx)};
+    const int D = A.ntt + 2 * A.nhalo;  // coarse cells per edge of the deposit range
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int base = J.u[d] * A.ntt + A.ncb - A.nhalo;
+        c0[d] = base + gi[d] * g;
+        c1[d] = min(c0[d] + g, base + D);
+    }
+}
+
+// items[jb * G + q] = work items of group q of job jb (splits x slabs);
+// cnts[jb * G + q] = its particles.
+__global__ void k_dep_count(MeshArgs A, int g, int ngx, int nslab, int* __restrict__ items,
+                            int* __restrict__ cnts) {
+    const int G = ngx * ngx * ngx;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)A.njobs * G) return;
+    const int jb = (int)(t / G), q = (int)(t - (int64_t)jb * G);
     const PM3Job J = A.jobs[jb];
-    const int ci = J.u[0] * A.ntt + A.ncb - 1 + r % ne;
-    const int cj = J.u[1] * A.ntt + A.ncb - 1 + (r / ne) % ne;
-    const int ck = J.u[2] * A.ntt + A.ncb - 1 + r / (ne * ne);
-    const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
-    const int np = A.rhoc[cell];
-    const int64_t base = A.nzero[cell];
-    if (np <= 0) return;
-    if (base < 0 || base + np > A.n_used) {
-        if (lane == 0) atomicOr(err, 2);
+    int c0[3], c1[3];
+    group_cells(A, J, g, ngx, q, c0, c1);
+    int64_t cnt = 0;
+    for (int ck = c0[2]; ck < c1[2]; ck++)
+        for (int cj = c0[1]; cj < c1[1]; cj++) {
+            const int64_t row = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte;
+            for (int ci = c0[0]; ci < c1[0]; ci++) cnt += max(A.rhoc[row + ci], 0);
+        }
+    items[t] = (int)((cnt + kDepSplit - 1) / kDepSplit) * nslab;
+    cnts[t] = (int)min(cnt, (int64_t)INT_MAX);
+}
+
+// list[offs[jq] + r] = jq for the items r of group jq (offs: exclusive scan
+// of k_dep_count's items); entries past cap are dropped (the deposit then
+// reports an error).
+__global__ void k_dep_list(const int* __restrict__ offs, int64_t ngroups, int cap,
+                           int* __restrict__ list) {
+    const int64_t jq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (jq >= ngroups) return;
+    const int o = offs[jq], c = offs[jq + 1] - o;
+    for (int r = 0; r < c && o + r < cap; r++) list[o + r] = (int)jq;
+}
+
+// Persistent: blocks take items it = blockIdx.x, + gridDim.x, ... below the
+// total offs[ngroups]; list[it] names the item's group.
+__global__ __launch_bounds__(kDepThreads) void k_mesh_deposit(MeshArgs A, float* __restrict__ rho,
+                                                              size_t rs, int g, int cf, int E,
+                                                              int Zs, int ngx, int nslab,
+                                                              const int* __restrict__ offs,
+                                                              const int* __restrict__ list, int cap,
+                                                              const int* __restrict__ cnts,
+                                                              int* __restrict__ err) {
+    extern __shared__ unsigned box[];  // [Zs][E][E], fixed point
+    const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
+    const int G = ngx * ngx * ngx;
+    const int ngroups = A.njobs * G;
+    const int total = offs[ngroups];
+    if (total > cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, 8);
         return;
     }
-    const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
-    float* R = rho + (size_t)jb * rs;
-    for (int l = lane; l < np; l += 64) {
-        const int16_t* p = A.xp + 3 * (base + l);
-        int cx, cy, cz;
-        float wx[3], wy[3], wz[3];
-        tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
-        tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
-        tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int jq = list[it];  // offs[jq] <= it < offs[jq + 1]
+        const int r = it - offs[jq];
+        const int nsplit = (offs[jq + 1] - offs[jq]) / nslab;
+        const int split = r / nslab, slab = r - split * nslab;
+        // fixed-point scale from the item's particle bound (round-robin chunks)
+        const int cnt = cnts[jq];
+        const int nch = (cnt + (int)blockDim.x - 1) / (int)blockDim.x;  // chunks of the group
+        const int bound = max(1, min(cnt, (nch + nsplit - 1) / nsplit * (int)blockDim.x));
+        const int fbits = min(30, 33 - (32 - __clz(bound)));
+        const float fix = ldexpf(1.0f, fbits), inv_fix = ldexpf(1.0f, -fbits);
+        const int jb = jq / G, q = jq - jb * G;
+        const PM3Job J = A.jobs[jb];
+        int c0[3], c1[3], X0[3], Ee[3];
+        group_cells(A, J, g, ngx, q, c0, c1);
 #pragma unroll
-        for (int a3 = 0; a3 < 3; a3++) {
-            const int Z = cz + a3 + off;
-            if ((unsigned)Z >= (unsigned)n) continue;
+        for (int d = 0; d < 3; d++) {
+            X0[d] = (c0[d] - A.ncb) * cf - J.u[d] * A.nfp + off;  // storage index of local 0
+            Ee[d] = (c1[d] - c0[d]) * cf + 2;
+        }
+        const int zlo = slab * Zs, nz = min(Zs, Ee[2] - zlo);  // region-local planes of the slab
+        if (nz <= 0) continue;  // uniform over the block
+        __syncthreads();        // the previous item's write-out has read the LDS
+        for (int e = threadIdx.x; e < E * E * nz; e += blockDim.x) box[e] = 0u;
+        __syncthreads();
+        const int gx = c1[0] - c0[0];  // 1..4 cells along x
+        const int T = (int)blockDim.x;
+        int64_t gbase = 0;  // group-order index of the row's first particle
+        for (int ck = c0[2]; ck < c1[2]; ck++) {
+            const int zc = (ck - c0[2]) * cf;  // the cell reaches local planes [zc, zc + cf + 2)
+            const bool reach = !(zc + cf + 2 <= zlo || zc >= zlo + nz);
+            for (int cj = c0[1]; cj < c1[1]; cj++) {
+                // the particles of cells c0..c1-1 of a row are contiguous slots
+                const int64_t cell0 = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + c0[0];
+                const int64_t s0 = A.nzero[cell0];
+                const int64_t e0 = s0 + A.rhoc[cell0];
+                const int64_t e1 = gx > 1 ? e0 + A.rhoc[cell0 + 1] : e0;
+                const int64_t e2 = gx > 2 ? e1 + A.rhoc[cell0 + 2] : e1;
+                const int64_t e3 = gx > 3 ? e2 + A.rhoc[cell0 + 3] : e2;
+                if (e0 == s0 && e1 == e0 && e2 == e1 && e3 == e2) continue;  // no particles
+                if (s0 < 0 || e0 < s0 || e1 < e0 || e2 < e1 || e3 < e2 || e3 > A.n_used) {
+                    if (threadIdx.x == 0 && split == 0 && slab == 0) atomicOr(err, 2);
+                    continue;
+                }
+                const int64_t rlo = gbase, rhi = gbase + (e3 - s0);
+                gbase = rhi;
+                if (!reach) continue;
+                // chunks of T particles in group order go round-robin to the splits
+                int64_t ch = rlo / T;
+                ch += ((split - ch % nsplit) + nsplit) % nsplit;
+                for (; ch * T < rhi; ch += nsplit) {
+                    const int64_t gi = ch * T + threadIdx.x;
+                    if (gi < rlo || gi >= rhi) continue;
+                    const int64_t l = s0 + (gi - rlo);
+                    const int ci = c0[0] + (l >= e0) + (l >= e1) + (l >= e2);
+                    const int16_t* p = A.xp + 3 * l;
+                    // floor of the mesh coordinate in integer arithmetic (the
+                    // value tsc_axis computes); x, y land in [0, Ee - 3]
+                    const int Xl = fine_floor(A, ci, p[0], J.u[0], cf) + off - X0[0];
+                    const int Yl = fine_floor(A, cj, p[1], J.u[1], cf) + off - X0[1];
+                    const int Zl = fine_floor(A, ck, p[2], J.u[2], cf) + off - X0[2] - zlo;
+                    if (Zl > nz - 1 || Zl < -2) continue;
+                    int cx, cy, cz;
+                    float wx[3], wy[3], wz[3];
+                    tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
+                    tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
+                    tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+                    (void)cx, (void)cy, (void)cz;  // equal to the integer floors above
 #pragma unroll
-            for (int a2 = 0; a2 < 3; a2++) {
-                const int Y = cy + a2 + off;
-                if ((unsigned)Y >= (unsigned)n) continue;
-                float* row = R + ((size_t)Z * n + Y) * (n + 2);
+                    for (int a3 = 0; a3 < 3; a3++) {
+                        const int Z = Zl + a3;
+                        if ((unsigned)Z >= (unsigned)nz) continue;
 #pragma unroll
-                for (int a1 = 0; a1 < 3; a1++) {
-                    const int X = cx + a1 + off;
-                    if ((unsigned)X >= (unsigned)n) continue;
-                    unsafeAtomicAdd(row + X, wx[a1] * wy[a2] * wz[a3]);
+                        for (int a2 = 0; a2 < 3; a2++) {
+                            unsigned* row = box + (Z * E + Yl + a2) * E + Xl;
+#pragma unroll
+                            for (int a1 = 0; a1 < 3; a1++) {
+                                const float w = wx[a1] * wy[a2] * wz[a3];
+                                atomicAdd(row + a1, (unsigned)rintf(w * fix));
+                            }
+                        }
+                    }
                 }
             }
+        }
+        __syncthreads();
+        float* R = rho + (size_t)jb * rs;
+        const int ex = Ee[0], ey = Ee[1], ez = Ee[2];
+        for (int e = threadIdx.x; e < ex * ey * nz; e += blockDim.x) {
+            const int x = e % ex, y = (e / ex) % ey, zl = e / (ex * ey), z = zl + zlo;
+            const int X = X0[0] + x, Y = X0[1] + y, Z = X0[2] + z;
+            if ((unsigned)X >= (unsigned)n || (unsigned)Y >= (unsigned)n ||
+                (unsigned)Z >= (unsigned)n)
+                continue;
+            const unsigned u = box[(zl * E + y) * E + x];
+            if (u == 0u) continue;  // the mesh is cleared
+            const float v = (float)u * inv_fix;  // one FP32 rounding of the sum
+            float* dst = R + ((size_t)Z * n + Y) * (n + 2) + X;
+            if (nsplit == 1 && x >= 2 && x < ex - 2 && y >= 2 && y < ey - 2 && z >= 2 && z < ez - 2)
+                *dst = v;
+            else
+                unsafeAtomicAdd(dst, v);
         }
     }
 }
 
 // Brick deposit: block (brick, job) owns a kBrick^3 region of the job's
-// mesh (storage indices), sums in LDS every TSC weight that lands in it
-// from the particles of the coarse cells that can reach it, then stores the
-// brick.  Every mesh point belongs to one brick, so the mesh needs no
-// clearing and HBM sees no atomics; dense cores contend only in LDS.
+// mesh (storage indices) and sums in LDS every TSC weight that lands in it
+// from the particles of the coarse cells that can reach it (64-bit fixed
+// point, each weight rounded to 2^-32: integer LDS atomics), then stores
+// the brick.  Every mesh point belongs to one brick: the mesh needs no
+// clearing and HBM sees no atomics, at the price of visiting a particle
+// once per brick its cell overlaps.
 constexpr int kBrick = 16;
 
 __device__ inline int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
-__global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* __restrict__ rho,
-                                                            size_t rs, int nbx,
-                                                            int* __restrict__ err) {
-    __shared__ float box[kBrick * kBrick * kBrick];
+__global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __restrict__ rho,
+                                                       size_t rs, int nbx, int* __restrict__ err) {
+    __shared__ unsigned long long box[kBrick * kBrick * kBrick];
     const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
     const int jb = blockIdx.y;
     const int b = blockIdx.x;
     const int bx = b % nbx, by = (b / nbx) % nbx, bz = b / (nbx * nbx);
     const int lo[3] = {bx * kBrick, by * kBrick, bz * kBrick};
-    for (int q = threadIdx.x; q < kBrick * kBrick * kBrick; q += blockDim.x) box[q] = 0.0f;
+    for (int q = threadIdx.x; q < kBrick * kBrick * kBrick; q += blockDim.x) box[q] = 0ull;
     const PM3Job J = A.jobs[jb];
     const int cf = A.ratio_cs * A.rsf;  // fine cells per coarse cell
     // A particle with c = floor(xpos) writes storage c+off .. c+off+2, so it
@@ -185,7 +349,6 @@ __global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* _
                     if (threadIdx.x == 0) atomicOr(err, 2);
                     continue;
                 }
-#pragma unroll 4
                 for (int l = threadIdx.x; l < np; l += blockDim.x) {
                     const int16_t* p = A.xp + 3 * (base + l);
                     // floor of the mesh coordinate in integer arithmetic (the
@@ -194,7 +357,8 @@ __global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* _
                     const int X0 = fine_floor(A, ci, p[0], J.u[0], cf) + off - lo[0];
                     const int Y0 = fine_floor(A, cj, p[1], J.u[1], cf) + off - lo[1];
                     const int Z0 = fine_floor(A, ck, p[2], J.u[2], cf) + off - lo[2];
-                    if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 || Z0 > kBrick - 1 || Z0 < -2)
+                    if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 ||
+                        Z0 > kBrick - 1 || Z0 < -2)
                         continue;
                     int cx, cy, cz;
                     float wx[3], wy[3], wz[3];
@@ -214,7 +378,9 @@ __global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* _
                             for (int a1 = 0; a1 < 3; a1++) {
                                 const int X = X0 + a1;
                                 if ((unsigned)X >= (unsigned)kBrick) continue;
-                                atomicAdd(&box[(Z * kBrick + Y) * kBrick + X], wx[a1] * wy[a2] * wz[a3]);
+                                const float w = wx[a1] * wy[a2] * wz[a3];  // <= 0.75^3
+                                atomicAdd(&box[(Z * kBrick + Y) * kBrick + X],
+                                          (unsigned long long)(unsigned)rintf(w * 4294967296.0f));
                             }
                         }
                     }
@@ -225,8 +391,31 @@ __global__ __launch_bounds__(256) void k_pm3_deposit_bricks(MeshArgs A, float* _
     for (int q = threadIdx.x; q < kBrick * kBrick * kBrick; q += blockDim.x) {
         const int X = lo[0] + (q % kBrick), Y = lo[1] + (q / kBrick) % kBrick,
                   Z = lo[2] + q / (kBrick * kBrick);
-        if (X < n && Y < n && Z < n) R[((size_t)Z * n + Y) * (n + 2) + X] = box[q];
+        if (X < n && Y < n && Z < n)
+            R[((size_t)Z * n + Y) * (n + 2) + X] = (float)((double)box[q] * (1.0 / 4294967296.0));
     }
+}
+
+// Which deposit a mesh gets: bricks on the fine PM3 meshes, cell-group work
+// items on PM2's (dense clumps would leave a few bricks with most of the
+// work).
+static bool use_bricks(const MeshArgs& A) { return A.layer == 3; }
+
+// Blocks of the persistent deposit: what fits the CUs at once.
+static int deposit_grid(size_t lds) {
+    static std::map<size_t, int> cache;
+    auto it = cache.find(lds);
+    if (it != cache.end()) return it->second;
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mesh_deposit, kDepThreads, lds) !=
+            hipSuccess ||
+        per < 1)
+        per = 2;
+    cache[lds] = std::max(1, ncu) * per;
+    return cache[lds];
 }
 
 // Force planes over Fortran indices [0, nfp+1]^3 (kick.f90:545-568) and the
@@ -471,11 +660,29 @@ struct DebugOut {
 
 // Runs one batch of jobs of one level; f2bits and ptot are device scalars.
 // Mesh and force-plane buffers of a batch of nb jobs at mesh size n.
-static int batch_reserve(int n, int nfp, size_t nb) {
-    const size_t rs = (size_t)(n + 2) * n * n;
-    const size_t fs = (size_t)(nfp + 2) * (nfp + 2) * (nfp + 2);
+// Work items of a batch at most: a group of c particles has
+// ceil(c / kDepSplit) <= c / kDepSplit + 1 splits per slab, and a slot lies
+// in the deposit ranges of at most 8 jobs.
+static int deposit_cap(const MeshArgs& A, size_t nb) {
+    const DepositShape S = deposit_shape(A);
+    const int64_t cap = ((int64_t)nb * S.groups + 8 * A.n_used / kDepSplit + 1) * S.nslab;
+    return (int)std::min<int64_t>(cap, (int64_t)1 << 30);
+}
+
+static int batch_reserve(const MeshArgs& A, size_t nb) {
+    const size_t rs = (size_t)(A.n + 2) * A.n * A.n;
+    const size_t fs = (size_t)(A.nfp + 2) * (A.nfp + 2) * (A.nfp + 2);
     C2_TRY(g_pm3.rho.reserve(nb * rs * sizeof(float)));
     C2_TRY(g_pm3.force.reserve(nb * 3 * fs * sizeof(float)));
+    const int64_t ng = (int64_t)nb * deposit_shape(A).groups;
+    if (ng >= ((int64_t)1 << 30)) {
+        set_error("deposit: %lld cell groups in one batch", (long long)ng);
+        return CUBE2_ERR_CAPACITY;
+    }
+    C2_TRY(g_pm3.dep_items.reserve((size_t)(ng + 1) * sizeof(int)));
+    C2_TRY(g_pm3.dep_scan.reserve(scan_tmp_ints(ng) * sizeof(int)));
+    C2_TRY(g_pm3.dep_list.reserve((size_t)deposit_cap(A, nb) * sizeof(int)));
+    C2_TRY(g_pm3.dep_cnts.reserve((size_t)std::max<int64_t>(ng, 1) * sizeof(int)));
     return CUBE2_OK;
 }
 
@@ -488,7 +695,7 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     const int n = base.n, nfp = base.nfp;
     const size_t rs = (size_t)(n + 2) * n * n;
     const size_t fs = (size_t)(nfp + 2) * (nfp + 2) * (nfp + 2);
-    C2_TRY(batch_reserve(n, nfp, nb));
+    C2_TRY(batch_reserve(base, nb));
     MeshArgs A = base;
     A.jobs = d_jobs;
     A.njobs = (int)nb;
@@ -496,11 +703,34 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     float* force = g_pm3.force.as<float>();
     const FFTPlan* P = nullptr;
     C2_TRY(fft_plan(n, &P));
-    {
+    if (use_bricks(A)) {
         const int nbx = (n + kBrick - 1) / kBrick;
-        Stage st(l2 ? "pm2_deposit" : "pm3_deposit", (double)nb * n * n * n * 4);
+        Stage st(l2 ? "pm2_deposit" : "pm3_deposit", (double)nb * rs * 4);
         dim3 grid((unsigned)(nbx * nbx * nbx), (unsigned)nb);
-        k_pm3_deposit_bricks<<<grid, 256, 0, s>>>(A, rho, rs, nbx, err);
+        k_brick_deposit<<<grid, 256, 0, s>>>(A, rho, rs, nbx, err);
+        C2_HIP(hipGetLastError());
+    } else {
+        const DepositShape S = deposit_shape(A);
+        Stage st(l2 ? "pm2_deposit" : "pm3_deposit", (double)nb * rs * 4 * 2);
+        C2_TRY(fill_zero(rho, nb * rs * sizeof(float), s));
+        static bool attr = false;
+        if (!attr) {
+            C2_HIP(hipFuncSetAttribute((const void*)k_mesh_deposit,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kDepLdsBytes));
+            attr = true;
+        }
+        const int64_t ng = (int64_t)nb * S.groups;
+        int* items = g_pm3.dep_items.as<int>();
+        int* cnts = g_pm3.dep_cnts.as<int>();
+        k_dep_count<<<ceil_div(ng, 256), 256, 0, s>>>(A, S.g, S.ngx, S.nslab, items, cnts);
+        C2_HIP(hipGetLastError());
+        C2_TRY(exclusive_scan(items, ng, g_pm3.dep_scan, s));  // items -> offsets, total at [ng]
+        const int cap = deposit_cap(A, nb);
+        int* list = g_pm3.dep_list.as<int>();
+        k_dep_list<<<ceil_div(ng, 256), 256, 0, s>>>(items, ng, cap, list);
+        C2_HIP(hipGetLastError());
+        k_mesh_deposit<<<deposit_grid(S.lds), kDepThreads, S.lds, s>>>(
+            A, rho, rs, S.g, S.cf, S.E, S.Zs, S.ngx, S.nslab, items, list, cap, cnts, err);
         C2_HIP(hipGetLastError());
     }
     if (dbg && dbg->rho) {
@@ -651,7 +881,7 @@ int pm3_prepare(const int* isort, const int* ires, const int* ixyz3, const int* 
                 all.push_back(G.jobs[j0 + q]);
                 P.job_it.push_back(G.jobs[j0 + q].it);
             }
-            C2_TRY(batch_reserve(A.n, A.nfp, B.nb));
+            C2_TRY(batch_reserve(A, B.nb));
             P.batches.push_back(B);
         }
     }
@@ -701,6 +931,10 @@ int pm3_finish(const PM3Plan& P, float* f2max, double* ptotal) {
     unsigned r[4];
     std::memcpy(r, h, sizeof(r));
     const unsigned long long* hp = reinterpret_cast<const unsigned long long*>(h + sizeof(r));
+    if (r[1] & 8) {
+        set_error("rhoc holds more particles than n_used=%lld slots allow", (long long)P.n_used);
+        return CUBE2_ERR_ARG;
+    }
     if (r[1]) {
         set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)P.n_used);
         return CUBE2_ERR_ARG;
@@ -777,7 +1011,7 @@ int pm2_prepare(int64_t n_used, const float* d_green2, PM3Plan& P) {
         B.nb = std::min(batch, (size_t)g.ntile - j0);
         B.job0 = j0;
         B.slot0 = j0;
-        C2_TRY(batch_reserve(A.n, A.nfp, B.nb));
+        C2_TRY(batch_reserve(A, B.nb));
         P.batches.push_back(B);
     }
     C2_TRY(nzero_reserve());
@@ -938,6 +1172,10 @@ void pm3_release() {
     g_pm3.jobs.release();
     g_pm3.red.release();
     g_pm3.ptot.release();
+    g_pm3.dep_items.release();
+    g_pm3.dep_scan.release();
+    g_pm3.dep_list.release();
+    g_pm3.dep_cnts.release();
     g_pm3.h_res.release();
     fft_release();
     green_tiled_release();
