@@ -134,6 +134,7 @@ def main(argv=None) -> int:
                     help="time eager dispatch only (value = eager kick), e.g. under counters")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-pm2", action="store_true", help="skip the PM2 measurement")
+    ap.add_argument("--no-drift", action="store_true", help="skip the drift measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM counters)")
     args = ap.parse_args(argv)
@@ -266,6 +267,32 @@ def main(argv=None) -> int:
                                 avg_launch_ms=round(dst["ms"] / dst["launches"], 3))}
         log(f"PM2: {pm2['ms']:.1f} ms ({C1.ntile} tiles, mesh {n2}^3); stages {pm2_stages}")
 
+    # drift (drift.f90) of the same image with a step that moves the fastest
+    # particle one coarse cell, then the next kick's layout from the result
+    drift = None
+    if not args.no_drift:
+        from cube2_amd.drift import drift_device, next_layout
+        vabs = k.d_vp[:k.n_used].abs()
+        vz_max = float(vabs[:, 2].max())
+        dt_d = float(np.float32(C1.ratio_cs / float(vabs.max())))
+        dr_ms, nl_ms = [], []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            img = drift_device(k.d_rhoc, k.d_idx, k.d_xp, k.d_vp, k.n_used, dt_d, vz_max, C1)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            nxt = next_layout(*img, C1)
+            torch.cuda.synchronize()
+            dr_ms.append(1e3 * (t2 - t1))
+            nl_ms.append(1e3 * (time.perf_counter() - t2))
+        assert img[4] == lay.n_particles, "drift lost particles"
+        drift = {"ms": round(min(dr_ms), 3), "next_layout_ms": round(min(nl_ms), 3),
+                 "dt_mid": dt_d, "vz_max": vz_max, "particles": int(img[4]),
+                 "next_n_used": int(nxt[4])}
+        del img, nxt
+        log(f"drift: {drift['ms']:.1f} ms, next layout {drift['next_layout_ms']:.1f} ms")
+
     pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq",
                "precise": "pp_pairs_precise"}[args.pp_mode]
     pp_ms = stats[pp_name]["ms"] / args.steps
@@ -295,8 +322,8 @@ def main(argv=None) -> int:
         log("cpu baseline ...")
         cpu = cpu_baseline(lay, k.ires, k.isort, greens_host, (res.p_cand, res.p_acc), mass, amid,
                            dt, args.cpu_budget)
+        import oracle
         if pm2 is not None:
-            import oracle
             g2 = k.green2().cpu().numpy()
             t1 = time.perf_counter()
             nt2 = 8
@@ -307,6 +334,13 @@ def main(argv=None) -> int:
                                        unit="ms/call", cores=cpu["cores"], kind="port",
                                        sample=f"oracle PM2 on tiles 0-{nt2 - 1} of {C1.ntile} "
                                               f"({t_s:.2f} s), scaled by {C1.ntile}/{nt2}")
+        if drift is not None:
+            t1 = time.perf_counter()
+            oracle.drift(C1, lay.rhoc, lay.idx_b_r, lay.xp, k.vp_host(), drift["dt_mid"],
+                         drift["vz_max"])
+            drift["cpu_baseline"] = dict(value=round(1e3 * (time.perf_counter() - t1), 1),
+                                         unit="ms/call", cores=1, kind="port",
+                                         sample="oracle drift of the whole image, one thread")
 
     out = {
         "metric": "kick ms/step + PP pair-int/s @256^3",
@@ -335,6 +369,7 @@ def main(argv=None) -> int:
         "p_acc": int(res.p_acc),
         "stage_ms_per_step": stages,
         "pm2": pm2,
+        "drift": drift,
         "roofline": roof,
         "tuning": tuning,
         "cpu_baseline": cpu,

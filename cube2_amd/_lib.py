@@ -66,6 +66,9 @@ SIGNATURES = {
     "cube2_green_3d_dev": (C.c_int, [vp_, C.c_int, C.c_float, C.c_float, C.c_float]),
     "cube2_select_levels": (C.c_int, [vp_] * 5),
     "cube2_layout_build_dev": (C.c_int, [vp_, vp_, C.c_int64] + [vp_] * 4 + [C.c_int64, vp_]),
+    "cube2_drift_dev": (C.c_int, [vp_] * 4 + [C.c_int64, C.c_float, C.c_float] + [vp_] * 4 +
+                        [C.c_int64, vp_]),
+    "cube2_slot_positions_dev": (C.c_int, [vp_] * 3 + [C.c_int64, vp_]),
     "cube2_kick_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3
                        + [vp_] * 5 + [C.POINTER(KickResultC)]),
     "cube2_kick_graph_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3

@@ -119,6 +119,7 @@ void c_gpu_velocity_finalize(void) {
     pp_release();
     kick_release();
     layout_release();
+    drift_release();
     Context& C = ctx();
     C.dv64.release();
     C.d_vp.release();
@@ -232,6 +233,21 @@ int cube2_layout_build_dev(const double* d_pos, const float* d_vel, int64_t n, i
                            int64_t* n_used) {
     std::lock_guard<std::recursive_mutex> l(g_mx);
     return layout_build(d_pos, d_vel, n, d_rhoc, d_idx_b_r, d_xp, d_vp, cap, n_used);
+}
+
+int cube2_drift_dev(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                    const float* d_vp, int64_t n_used, float dt_mid, float vz_max,
+                    int* d_rhoc_out, int64_t* d_idx_b_r_out, int16_t* d_xp_out, float* d_vp_out,
+                    int64_t cap, int64_t* n_out) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return drift(d_rhoc, d_idx_b_r, d_xp, d_vp, n_used, dt_mid, vz_max, d_rhoc_out, d_idx_b_r_out,
+                 d_xp_out, d_vp_out, cap, n_out);
+}
+
+int cube2_slot_positions_dev(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                             int64_t n_used, double* d_pos) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return slot_positions(d_rhoc, d_idx_b_r, d_xp, n_used, d_pos);
 }
 
 int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches) {

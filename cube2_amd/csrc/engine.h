@@ -99,6 +99,14 @@ int layout_build(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc
                  int64_t* d_idx_b_r, int16_t* d_xp, float* d_vp, int64_t cap, int64_t* n_used);
 void layout_release();
 
+// Drift of one image to its physical-only successor (drift.hip).
+int drift(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp, const float* d_vp,
+          int64_t n_used, float dt_mid, float vz_max, int* d_rhoc_out, int64_t* d_idx_out,
+          int16_t* d_xp_out, float* d_vp_out, int64_t cap, int64_t* n_out);
+int slot_positions(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                   int64_t n_used, double* d_pos);
+void drift_release();
+
 // ---------------------------------------------------------------- misc
 int fill_zero(void* d, size_t bytes, hipStream_t s);  // a kernel, not a memset
 int selftest_div(uint64_t seed, int n, unsigned long long* mismatches);

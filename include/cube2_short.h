@@ -183,6 +183,32 @@ int cube2_layout_build_dev(const double* d_pos, const float* d_vel, int64_t n, i
                            int64_t* n_used);
 
 /* ---------------------------------------------------------------------
+ * Drift (drift.f90:1-158 for one image, then spine_image of
+ * variables.f90:146-216).  The reference runs it on the host, per tile
+ * with OpenMP.  From the image with halo copies (the kick's d_rhoc,
+ * d_idx_b_r, d_xp, d_vp, n_used) every slot moves by dt_mid*vp; the
+ * particles landing in a physical cell of their slot's tile form the new
+ * physical-only image: d_rhoc_out [ntile][nte^3] (halo cells 0),
+ * d_idx_b_r_out [ntile][nte^2], d_xp_out / d_vp_out [*n_out][3] (cap
+ * entries; n_used always suffices).  Within a cell the particles keep the
+ * arrival order of drift.f90's layered loop.  vz_max bounds |vp_z| as
+ * sim%vz_max does (kick.f90:239; drift.f90:26 sets the layer spacing from
+ * it); CUBE2_ERR_ARG if a particle moves further in z than vz_max allows
+ * or more than ncb coarse cells in any direction.  The halo copies for the
+ * next kick come from cube2_slot_positions_dev + cube2_layout_build_dev.
+ * ------------------------------------------------------------------- */
+int cube2_drift_dev(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                    const float* d_vp, int64_t n_used, float dt_mid, float vz_max,
+                    int* d_rhoc_out, int64_t* d_idx_b_r_out, int16_t* d_xp_out, float* d_vp_out,
+                    int64_t cap, int64_t* n_out);
+
+/* Positions d_pos[n_used][3] (FP64 standard cells, wrapped into [0, ng))
+ * of every slot of an image, halo copies included: (global coarse cell +
+ * the decoded in-cell offset, parameters.f90:87-88,118) * ratio_cs. */
+int cube2_slot_positions_dev(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
+                             int64_t n_used, double* d_pos);
+
+/* ---------------------------------------------------------------------
  * Kick driver: the short-range part of kick.f90:107-238 in one call over
  * device-resident arrays (the Fortran host's level selection, the PM3 and
  * PP calls of kick.f90:144-146,202-205, and the dt limits of

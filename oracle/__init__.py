@@ -73,6 +73,9 @@ def lib():
         L.or_pm2_force.restype = C.c_int
         L.or_pm2_force.argtypes = [C.POINTER(OrGeom), i32p, i64p, i16p, f32p, C.c_float, C.c_float,
                                    C.c_float, f64p, f32p, f64p, C.c_int, C.c_int]
+        L.or_drift.restype = C.c_int
+        L.or_drift.argtypes = [C.POINTER(OrGeom), i32p, i64p, i16p, f32p, C.c_float, C.c_float,
+                               i32p, i64p, i16p, f32p, C.c_int64, C.POINTER(C.c_int64)]
         L.or_velocity_apply.argtypes = [C.POINTER(OrGeom), i32p, i64p, f32p, f64p, f32p]
         L.or_direct_pp_f64.argtypes = [f64p, C.c_int, C.c_double, i32p, C.c_int, f64p, C.c_double,
                                        f64p]
@@ -256,6 +259,24 @@ def pm2_force(geom, rhoc, idx_b_r, xp, green2, mass, amid, dt, tiles=None):
     if rc:
         raise ValueError("oracle rejected the PM2 inputs")
     return dict(dv64=dv, f2max=float(f2[0]), ptotal=pt)
+
+
+def drift(geom, rhoc, idx_b_r, xp, vp, dt_mid, vz_max):
+    """Returns dict(rhoc, idx_b_r, xp, vp, n) of the drifted physical-only image."""
+    L = lib()
+    g = geom_struct(geom)
+    rhoc = _c(rhoc, np.int32)
+    n_in = int(np.asarray(idx_b_r).reshape(-1)[-1])
+    ro = np.zeros_like(rhoc)
+    io = np.zeros(np.asarray(idx_b_r).shape, np.int64)
+    xo = np.zeros((max(n_in, 1), 3), np.int16)
+    vo = np.zeros((max(n_in, 1), 3), np.float32)
+    n = C.c_int64(0)
+    rc = L.or_drift(C.byref(g), rhoc, _c(idx_b_r, np.int64), _c(xp, np.int16), _c(vp, np.float32),
+                    float(dt_mid), float(vz_max), ro, io, xo, vo, n_in, C.byref(n))
+    if rc:
+        raise ValueError(f"oracle drift failed ({rc})")
+    return dict(rhoc=ro, idx_b_r=io, xp=xo[:n.value], vp=vo[:n.value], n=n.value)
 
 
 def velocity_apply(geom, rhoc, idx_b_r, vp, dv64):

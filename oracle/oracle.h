@@ -15,6 +15,7 @@
  *   - PM3 pm_thread chain     kick.f90:403-568 (deposit, FFT, Green multiply,
  *                             potential_to_force, interpolation)
  *   - Green_3D                Green/Green.f90:1-80 (FP64)
+ *   - drift                   drift.f90:1-158, variables.f90:101-216
  *
  * Parity pins (see tests/): golden vectors from the compiled reference
  * pp_force_kernel.c (oracle/_ref), numpy's pocketfft for the FFT, closed
@@ -105,6 +106,16 @@ int or_pm3_force(const or_geom* g, const int* isort, const int* ires, const int*
 int or_pm2_force(const or_geom* g, const int* rhoc, const int64_t* idx_b_r, const int16_t* xp,
                  const float* green2, float mass, float amid, float dt, double* dv64,
                  float* f2max, double* ptotal, int tile0, int ntiles);
+
+/* Drift of one image (drift.f90:1-158, spine_image): from the image with
+ * halo copies (rhoc/idx_b_r/xp/vp as the kick takes them) to the new
+ * physical-only image: rhoc_out [nnt^3][nte^3] (halo cells 0), idx_out
+ * [nnt^3][nte^2] (running row totals), xp_out/vp_out [*nplocal][3] (cap
+ * entries at most).  Returns 0, -2 if a particle moves beyond 2 ncb, -3 if
+ * cap is too small. */
+int or_drift(const or_geom* g, const int* rhoc, const int64_t* idx_b_r, const int16_t* xp,
+             const float* vp, float dt_mid, float vz_max, int* rhoc_out, int64_t* idx_out,
+             int16_t* xp_out, float* vp_out, int64_t cap, int64_t* nplocal);
 
 /* Apply a pending Delta-v to vp for slots in physical cells, return vmax. */
 void or_velocity_apply(const or_geom* g, const int* rhoc, const int64_t* idx_b_r, float* vp,
