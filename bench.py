@@ -271,7 +271,12 @@ def main(argv=None) -> int:
     # particle one coarse cell, then the next kick's layout from the result
     drift = None
     if not args.no_drift:
-        from cube2_amd.drift import drift_device, next_layout
+        from cube2_amd.drift import drift_device, next_layout, refresh_halo_velocities
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        refresh_halo_velocities(k.d_rhoc, k.d_idx, k.d_vp, k.n_used, C1)   # buffer_v
+        torch.cuda.synchronize()
+        bv_ms = 1e3 * (time.perf_counter() - t1)
         vabs = k.d_vp[:k.n_used].abs()
         vz_max = float(vabs[:, 2].max())
         dt_d = float(np.float32(C1.ratio_cs / float(vabs.max())))
@@ -288,6 +293,7 @@ def main(argv=None) -> int:
             nl_ms.append(1e3 * (time.perf_counter() - t2))
         assert img[4] == lay.n_particles, "drift lost particles"
         drift = {"ms": round(min(dr_ms), 3), "next_layout_ms": round(min(nl_ms), 3),
+                 "halo_velocities_ms": round(bv_ms, 3),
                  "dt_mid": dt_d, "vz_max": vz_max, "particles": int(img[4]),
                  "next_n_used": int(nxt[4])}
         del img, nxt

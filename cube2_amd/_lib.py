@@ -69,6 +69,7 @@ SIGNATURES = {
     "cube2_drift_dev": (C.c_int, [vp_] * 4 + [C.c_int64, C.c_float, C.c_float] + [vp_] * 4 +
                         [C.c_int64, vp_]),
     "cube2_slot_positions_dev": (C.c_int, [vp_] * 3 + [C.c_int64, vp_]),
+    "cube2_halo_velocities_dev": (C.c_int, [vp_] * 3 + [C.c_int64]),
     "cube2_kick_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3
                        + [vp_] * 5 + [C.POINTER(KickResultC)]),
     "cube2_kick_graph_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3

@@ -235,6 +235,12 @@ int cube2_layout_build_dev(const double* d_pos, const float* d_vel, int64_t n, i
     return layout_build(d_pos, d_vel, n, d_rhoc, d_idx_b_r, d_xp, d_vp, cap, n_used);
 }
 
+int cube2_halo_velocities_dev(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp,
+                              int64_t n_used) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return halo_velocities(d_rhoc, d_idx_b_r, d_vp, n_used);
+}
+
 int cube2_drift_dev(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
                     const float* d_vp, int64_t n_used, float dt_mid, float vz_max,
                     int* d_rhoc_out, int64_t* d_idx_b_r_out, int16_t* d_xp_out, float* d_vp_out,

@@ -97,6 +97,7 @@ void kick_release();
 // ------------------------------------------------------------- layout
 int layout_build(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc,
                  int64_t* d_idx_b_r, int16_t* d_xp, float* d_vp, int64_t cap, int64_t* n_used);
+int halo_velocities(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp, int64_t n_used);
 void layout_release();
 
 // Drift of one image to its physical-only successor (drift.hip).

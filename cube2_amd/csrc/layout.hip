@@ -7,6 +7,7 @@
 //            (FP64, parameters.f90:87-88);
 //   sort     stable radix sort of (coarse cell, particle index): the order of
 //            the particles of one cell is their input order;
+//   starts   first sorted position of every coarse cell (binary search);
 //   rhoc     particles per coarse cell of every tile's extended region (the
 //            periodic neighbour's cell for halo cells);
 //   slots    exclusive scan of rhoc: slot ranges per cell, idx_b_r = row ends;
@@ -55,9 +56,19 @@ __global__ void k_layout_encode(const double* __restrict__ pos, int64_t n, int n
     idx[i] = (uint32_t)i;
 }
 
-__global__ void k_layout_hist(const uint32_t* __restrict__ gid, int64_t n, int* __restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[gid[i]], 1);
+// gstart[b] = first sorted position of a cell id >= b (b <= nbin): the
+// cell starts straight from the sorted ids, by binary search.
+__global__ void k_layout_starts(const uint32_t* __restrict__ gid_sorted, int64_t n, int64_t nbin,
+                                int* __restrict__ gstart) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nbin) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)gid_sorted[mid] < b) lo = mid + 1;
+        else hi = mid;
+    }
+    gstart[b] = (int)lo;
 }
 
 // Global coarse cell of extended cell (i, j, k) of tile t (periodic).
@@ -165,12 +176,9 @@ int layout_build(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc
     int* slot = g_lay.slot.as<int>();
     {
         Stage st("layout_cells", (double)n * 8 + (double)ncells * 16);
-        C2_TRY(fill_zero(cnt, (size_t)(nbin + 1) * sizeof(int), s));
-        if (n) {
-            k_layout_hist<<<ceil_div(n, 256), 256, 0, s>>>(g_lay.gid.as<uint32_t>(), n, cnt);
-            C2_HIP(hipGetLastError());
-        }
-        C2_TRY(exclusive_scan(cnt, nbin, g_lay.scan_tmp, s));  // cnt -> gstart, total at [nbin]
+        k_layout_starts<<<ceil_div(nbin + 1, 256), 256, 0, s>>>(g_lay.gid_sorted.as<uint32_t>(), n,
+                                                                nbin, cnt);  // total at [nbin]
+        C2_HIP(hipGetLastError());
         k_layout_rhoc<<<ceil_div(ncells, 256), 256, 0, s>>>(cnt, ncells, g.nt, g.ncb, g.nte,
                                                             g.p.nnt, g.nc, d_rhoc, slot);
         C2_HIP(hipGetLastError());
@@ -198,6 +206,81 @@ int layout_build(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc
         C2_HIP(hipGetLastError());
     }
     return sync_and_flush();
+}
+
+// buffer_v for one image (buffer_particle.f90:11-21 through buffer_xvp,
+// :23-345): the velocities of every halo cell's slots from the periodic
+// source cell's slots, which hold the same particles in the same order.  A
+// wavefront per extended cell; physical cells return at once.
+__global__ __launch_bounds__(256) void k_halo_vp(const int* __restrict__ rhoc,
+                                                 const int64_t* __restrict__ idx_b_r,
+                                                 float* __restrict__ vp, int64_t n_used, int nt,
+                                                 int ncb, int nte, int nnt, int* __restrict__ err) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t te3 = (int64_t)nte * nte * nte;
+    if (q >= te3 * nnt * nnt * nnt) return;
+    const int t = (int)(q / te3);
+    const int r = (int)(q - (int64_t)t * te3);
+    const int c[3] = {r % nte, (r / nte) % nte, r / (nte * nte)};
+    bool halo = false;
+    for (int d = 0; d < 3; d++) halo |= c[d] < ncb || c[d] >= ncb + nt;
+    if (!halo) return;
+    const int np = rhoc[q];
+    if (np <= 0) return;
+    // source: the physical cell of the periodic image holding this cell
+    const int nc = nt * nnt;
+    const int tc[3] = {t % nnt, (t / nnt) % nnt, t / (nnt * nnt)};
+    int st[3], sc[3];
+    for (int d = 0; d < 3; d++) {
+        const int gcell = ((tc[d] * nt + c[d] - ncb) % nc + nc) % nc;
+        st[d] = gcell / nt;
+        sc[d] = gcell % nt + ncb;
+    }
+    const int64_t ts = ((int64_t)st[2] * nnt + st[1]) * nnt + st[0];
+    const int64_t qs = ts * te3 + ((int64_t)sc[2] * nte + sc[1]) * nte + sc[0];
+    auto base = [&](int64_t cell, int64_t tile, int k, int j, int i) {
+        int64_t suf = 0;
+        for (int a = i; a < nte; a++) suf += rhoc[cell - i + a];
+        return idx_b_r[tile * nte * nte + (int64_t)k * nte + j] - suf;
+    };
+    const int64_t b0 = base(q, t, c[2], c[1], c[0]);
+    const int64_t b1 = base(qs, ts, sc[2], sc[1], sc[0]);
+    if (rhoc[qs] != np || b0 < 0 || b1 < 0 || b0 + np > n_used || b1 + np > n_used) {
+        if (lane == 0) atomicOr(err, 1);
+        return;
+    }
+    for (int e = lane; e < 3 * np; e += 64) vp[3 * b0 + e] = vp[3 * b1 + e];
+}
+
+int halo_velocities(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp, int64_t n_used) {
+    ensure_params();
+    Context& C = ctx();
+    const Geom& g = C.g;
+    if (!d_rhoc || !d_idx_b_r || n_used < 0 || (n_used > 0 && !d_vp)) {
+        set_error("halo velocities: null argument");
+        return CUBE2_ERR_ARG;
+    }
+    C2_TRY(ensure_init());
+    hipStream_t s = C.stream;
+    C2_TRY(g_lay.red.reserve(sizeof(int)));
+    int* err = g_lay.red.as<int>();
+    C2_TRY(fill_zero(err, sizeof(int), s));
+    const int64_t ncells = (int64_t)g.ntile * g.nte * g.nte * g.nte;
+    {
+        Stage st("halo_vp", (double)n_used * 24);
+        k_halo_vp<<<ceil_div(ncells, 4), 256, 0, s>>>(d_rhoc, d_idx_b_r, d_vp, n_used, g.nt, g.ncb,
+                                                      g.nte, g.p.nnt, err);
+        C2_HIP(hipGetLastError());
+    }
+    int h = 0;
+    C2_HIP(hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    C2_TRY(sync_and_flush());
+    if (h) {
+        set_error("halo velocities: a halo cell and its source cell differ (not one image's layout)");
+        return CUBE2_ERR_ARG;
+    }
+    return CUBE2_OK;
 }
 
 void layout_release() {

@@ -18,6 +18,16 @@ from . import _lib
 from .geometry import Geometry
 
 
+def refresh_halo_velocities(rhoc, idx_b_r, vp, n_used: int, geom: Geometry) -> None:
+    """buffer_v for one image (cube2_halo_velocities_dev): halo slots take
+    their source particles' velocities, in place.  A kick updates physical
+    slots only; the reference's step runs kick, then buffer_v, before the
+    next drift (main.f90:26-31)."""
+    _lib.set_params(geom)
+    _lib.check(_lib.lib().cube2_halo_velocities_dev(rhoc.data_ptr(), idx_b_r.data_ptr(),
+                                                     vp.data_ptr(), int(n_used)))
+
+
 def drift_device(rhoc, idx_b_r, xp, vp, n_used: int, dt_mid: float, vz_max: float,
                  geom: Geometry):
     """Device tensors in, device tensors out: (rhoc, idx_b_r, xp, vp, n).

@@ -182,6 +182,14 @@ int cube2_layout_build_dev(const double* d_pos, const float* d_vel, int64_t n, i
                            int64_t* d_idx_b_r, int16_t* d_xp, float* d_vp, int64_t cap,
                            int64_t* n_used);
 
+/* buffer_v for one periodic image (buffer_particle.f90:11-21, through
+ * buffer_xvp :23-345): copies the velocities of every halo cell's slots
+ * from its periodic source cell.  A kick updates the physical slots only;
+ * call this between the kick and cube2_drift_dev, as the reference's step
+ * runs kick then buffer_v (main.f90:26-31). */
+int cube2_halo_velocities_dev(const int* d_rhoc, const int64_t* d_idx_b_r, float* d_vp,
+                              int64_t n_used);
+
 /* ---------------------------------------------------------------------
  * Drift (drift.f90:1-158 for one image, then spine_image of
  * variables.f90:146-216).  The reference runs it on the host, per tile

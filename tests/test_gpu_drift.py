@@ -100,6 +100,37 @@ def test_drift_then_layout_is_the_next_kick_image(lib, oracle_lib):
     assert np.array_equal(vp.cpu().numpy(), want.vp)
 
 
+def test_kick_halo_refresh_drift_keeps_every_particle(lib):
+    """A kick updates the physical slots only; after buffer_v
+    (cube2_halo_velocities_dev) every copy of a particle carries its
+    velocity, and the drift keeps every particle exactly once."""
+    import torch
+    from cube2_amd.drift import drift_device, refresh_halo_velocities
+    from cube2_amd.geometry import Geometry
+    from cube2_amd.kick import ShortRangeKick
+    from cube2_amd.layout import build_layout
+    g = Geometry(ng=64, nnt=2, nns=2, ratio_cs=4, ngb=8, app=0.05)
+    rng = np.random.default_rng(10)
+    pos = _clumpy(g, 20000, 13)
+    vel = rng.uniform(-2, 2, pos.shape).astype(np.float32)
+    lay = build_layout(pos, vel, g)
+    k = ShortRangeKick(lay)
+    k.kick(1.0, 0.02, 0.05)
+    vp = k.vp_host()
+    phys = lay.physical_mask()
+    # physical velocity of each particle, by source index
+    pv = np.zeros((pos.shape[0], 3), np.float32)
+    pv[lay.src[phys]] = vp[phys]
+    assert not np.array_equal(vp, pv[lay.src])          # halo copies are stale
+    refresh_halo_velocities(k.d_rhoc, k.d_idx, k.d_vp, k.n_used, g)
+    torch.cuda.synchronize()
+    assert np.array_equal(k.vp_host(), pv[lay.src])
+    vz = float(np.abs(pv[:, 2]).max())
+    dt = float(np.float32(g.ratio_cs / np.abs(pv).max()))
+    out = drift_device(k.d_rhoc, k.d_idx, k.d_xp, k.d_vp, k.n_used, dt, vz, g)
+    assert out[4] == pos.shape[0]
+
+
 def test_drift_rejects_a_step_beyond_vz_max(lib):
     from cube2_amd.drift import drift_device
     from cube2_amd.geometry import Geometry
