@@ -17,7 +17,7 @@ CANDIDATES = {
     "pp_blocks_per_cu": [0, 4, 5, 8],
     "pp_direct_lg": [0, 7, 5, 4],
     "fft_min_lgc": [0, 3, 4],
-    "interp_seg": [0, 4, 16],
+    "interp_seg": [0, 2, 8],
 }
 
 

@@ -280,7 +280,7 @@ int cube2_get_tuning(int key, int* value) {
         set_error("bad tuning key %d", key);
         return CUBE2_ERR_ARG;
     }
-    static const int builtin[CUBE2_TUNE_COUNT] = {0, 6, 0, 8};
+    static const int builtin[CUBE2_TUNE_COUNT] = {0, 6, 0, 4};
     const int v = ctx().tune[key];
     *value = v ? v : builtin[key];
     if (key == CUBE2_TUNE_PP_BLOCKS_PER_CU && !v) *value = 0;  // occupancy-derived at launch

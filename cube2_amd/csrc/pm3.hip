@@ -37,7 +37,8 @@ struct PM3Job {
 };
 
 struct PM3State {
-    DevBuf rho, force, jobs, red, ptot, dep_items, dep_scan, dep_list, dep_cnts;
+    DevBuf rho, force, jobs, red, ptot, dep_items, dep_scan, dep_list, dep_cnts, int_items,
+        int_list;
     HostBuf h_res;  // f2max bits, error flag, per-job particle counts (pinned)
 };
 static PM3State g_pm3;
@@ -461,67 +462,109 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
     block_max_to<1>(f2bits, fv);
 }
 
-// nseg wavefronts per (job, physical coarse cell), each taking every nseg-th
-// group of 64 particles (dense cells would otherwise leave one wavefront as
-// the tail): TSC gather and FP64 Delta-v.
-__global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __restrict__ force,
-                                                    double* __restrict__ dv64, int accumulate,
-                                                    const float* __restrict__ scal, int nseg,
-                                                    unsigned long long* __restrict__ ptot) {
-    const float amid = scal[1], dt = scal[2];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// Interpolation work items: one wavefront per (job, physical coarse cell,
+// segment of seg64*64 particles), so a dense cell spreads over many
+// wavefronts and a light one takes one.  items[jb * per + r] = segments of
+// cell r of job jb.
+__global__ void k_interp_count(MeshArgs A, int seglen, int* __restrict__ items) {
     const int64_t per = (int64_t)A.ntt * A.ntt * A.ntt;
-    const int64_t w0 = (int64_t)blockIdx.x * kWaves + wave;
-    if (w0 >= per * A.njobs * nseg) return;
-    const int seg = (int)(w0 % nseg);  // nseg wavefronts share a cell
-    const int64_t w = w0 / nseg;
-    const int jb = (int)(w / per);
-    const int r = (int)(w - (int64_t)jb * per);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * A.njobs) return;
+    const int jb = (int)(t / per);
+    const int r = (int)(t - (int64_t)jb * per);
     const PM3Job J = A.jobs[jb];
     const int ci = J.u[0] * A.ntt + A.ncb + r % A.ntt;
     const int cj = J.u[1] * A.ntt + A.ncb + (r / A.ntt) % A.ntt;
     const int ck = J.u[2] * A.ntt + A.ncb + r / (A.ntt * A.ntt);
-    const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
-    const int np = A.rhoc[cell];
-    const int64_t base = A.nzero[cell];
-    if (np <= 0 || base < 0 || base + np > A.n_used) return;
-    if (seg == 0 && lane == 0) atomicAdd(&ptot[jb], (unsigned long long)np);
+    const int np = A.rhoc[(((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci];
+    items[t] = np > 0 ? (np + seglen - 1) / seglen : 0;
+}
+
+// TSC gather of the force planes and FP64 Delta-v (kick.f90:497-538), a
+// wavefront per work item (list[it] names its cell, offs the cells' first
+// items); persistent over the items.
+__global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __restrict__ force,
+                                                    double* __restrict__ dv64, int accumulate,
+                                                    const float* __restrict__ scal, int seglen,
+                                                    const int* __restrict__ offs,
+                                                    const int* __restrict__ list, int cap,
+                                                    unsigned long long* __restrict__ ptot,
+                                                    int* __restrict__ err) {
+    const float amid = scal[1], dt = scal[2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t per = (int64_t)A.ntt * A.ntt * A.ntt;
+    const int total = offs[per * A.njobs];
+    if (total > cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, 8);
+        return;
+    }
     const int nf = A.nfp + 2;
     const size_t fs = (size_t)nf * nf * nf;
-    const float* F = force + (size_t)jb * 3 * fs;
-    for (int l = seg * 64 + lane; l < np; l += 64 * nseg) {
-        const int64_t ip = base + l;
-        const int16_t* p = A.xp + 3 * ip;
-        int cx, cy, cz;
-        float wx[3], wy[3], wz[3];
-        tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
-        tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
-        tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
-        float X = 0.0f, Y = 0.0f, Z = 0.0f;
+    for (int it = blockIdx.x * kWaves + wave; it < total; it += gridDim.x * kWaves) {
+        const int jq = list[it];
+        const int seg = it - offs[jq];
+        const int jb = (int)(jq / per);
+        const int r = (int)(jq - (int64_t)jb * per);
+        const PM3Job J = A.jobs[jb];
+        const int ci = J.u[0] * A.ntt + A.ncb + r % A.ntt;
+        const int cj = J.u[1] * A.ntt + A.ncb + (r / A.ntt) % A.ntt;
+        const int ck = J.u[2] * A.ntt + A.ncb + r / (A.ntt * A.ntt);
+        const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
+        const int np = A.rhoc[cell];
+        const int64_t base = A.nzero[cell];
+        if (np <= 0 || base < 0 || base + np > A.n_used) continue;
+        if (seg == 0 && lane == 0) atomicAdd(&ptot[jb], (unsigned long long)np);
+        const float* F = force + (size_t)jb * 3 * fs;
+        const int lend = min(np, (seg + 1) * seglen);
+        for (int l = seg * seglen + lane; l < lend; l += 64) {
+            const int64_t ip = base + l;
+            const int16_t* p = A.xp + 3 * ip;
+            int cx, cy, cz;
+            float wx[3], wy[3], wz[3];
+            tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
+            tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
+            tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+            float X = 0.0f, Y = 0.0f, Z = 0.0f;
 #pragma unroll
-        for (int a3 = 0; a3 < 3; a3++)
+            for (int a3 = 0; a3 < 3; a3++)
 #pragma unroll
-            for (int a2 = 0; a2 < 3; a2++)
+                for (int a2 = 0; a2 < 3; a2++)
 #pragma unroll
-                for (int a1 = 0; a1 < 3; a1++) {
+                    for (int a1 = 0; a1 < 3; a1++) {
 #pragma clang fp contract(off)
-                    const size_t u = ((size_t)(cz + a3) * nf + (cy + a2)) * nf + (cx + a1);
-                    X = X + F[u] * wx[a1] * wy[a2] * wz[a3];
-                    Y = Y + F[fs + u] * wx[a1] * wy[a2] * wz[a3];
-                    Z = Z + F[2 * fs + u] * wx[a1] * wy[a2] * wz[a3];
+                        const size_t u = ((size_t)(cz + a3) * nf + (cy + a2)) * nf + (cx + a1);
+                        X = X + F[u] * wx[a1] * wy[a2] * wz[a3];
+                        Y = Y + F[fs + u] * wx[a1] * wy[a2] * wz[a3];
+                        Z = Z + F[2 * fs + u] * wx[a1] * wy[a2] * wz[a3];
+                    }
+            const float acc[3] = {X, Y, Z};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double t;
+                {
+#pragma clang fp contract(off)
+                    t = (double)(acc[c] * amid * dt / 6.0f) / 3.14159265358979323846;
                 }
-        const float acc[3] = {X, Y, Z};
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            double t;
-            {
-#pragma clang fp contract(off)
-                t = (double)(acc[c] * amid * dt / 6.0f) / 3.14159265358979323846;
+                if (accumulate) dv64[3 * ip + c] += t;
+                else dv64[3 * ip + c] = t;
             }
-            if (accumulate) dv64[3 * ip + c] += t;
-            else dv64[3 * ip + c] = t;
         }
     }
+}
+
+// Blocks of the persistent interpolation: what fits the CUs at once.
+static int interp_grid() {
+    static int cached = 0;
+    if (cached) return cached;
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pm3_interp, 256, 0) != hipSuccess ||
+        per < 1)
+        per = 2;
+    cached = std::max(1, ncu) * per;
+    return cached;
 }
 
 // Pending Delta-v applied without PP; vmax over physical slots.
@@ -669,6 +712,13 @@ static int deposit_cap(const MeshArgs& A, size_t nb) {
     return (int)std::min<int64_t>(cap, (int64_t)1 << 30);
 }
 
+// Interpolation work items of a batch at most: every cell one, plus one per
+// 64 particles (the shortest segment), the jobs' physical cells disjoint.
+static int interp_cap(const MeshArgs& A, size_t nb) {
+    const int64_t cells = (int64_t)nb * A.ntt * A.ntt * A.ntt;
+    return (int)std::min<int64_t>(cells + A.n_used / 64 + 1, (int64_t)1 << 30);
+}
+
 static int batch_reserve(const MeshArgs& A, size_t nb) {
     const size_t rs = (size_t)(A.n + 2) * A.n * A.n;
     const size_t fs = (size_t)(A.nfp + 2) * (A.nfp + 2) * (A.nfp + 2);
@@ -679,8 +729,11 @@ static int batch_reserve(const MeshArgs& A, size_t nb) {
         set_error("deposit: %lld cell groups in one batch", (long long)ng);
         return CUBE2_ERR_CAPACITY;
     }
+    const int64_t nci = (int64_t)nb * A.ntt * A.ntt * A.ntt;  // interpolation cells
     C2_TRY(g_pm3.dep_items.reserve((size_t)(ng + 1) * sizeof(int)));
-    C2_TRY(g_pm3.dep_scan.reserve(scan_tmp_ints(ng) * sizeof(int)));
+    C2_TRY(g_pm3.dep_scan.reserve(scan_tmp_ints(std::max(ng, nci)) * sizeof(int)));
+    C2_TRY(g_pm3.int_items.reserve((size_t)(nci + 1) * sizeof(int)));
+    C2_TRY(g_pm3.int_list.reserve((size_t)interp_cap(A, nb) * sizeof(int)));
     C2_TRY(g_pm3.dep_list.reserve((size_t)deposit_cap(A, nb) * sizeof(int)));
     C2_TRY(g_pm3.dep_cnts.reserve((size_t)std::max<int64_t>(ng, 1) * sizeof(int)));
     return CUBE2_OK;
@@ -784,11 +837,20 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
         C2_HIP(hipStreamSynchronize(s));
     }
     if (dv64) {
-        const int nseg = ctx().tune[CUBE2_TUNE_INTERP_SEG] ? ctx().tune[CUBE2_TUNE_INTERP_SEG] : 8;
-        const int64_t waves = (int64_t)nb * base.ntt * base.ntt * base.ntt * nseg;
+        const int seg64 = ctx().tune[CUBE2_TUNE_INTERP_SEG] ? ctx().tune[CUBE2_TUNE_INTERP_SEG] : 4;
+        const int seglen = 64 * seg64;
+        const int64_t ncell = (int64_t)nb * base.ntt * base.ntt * base.ntt;
         Stage st(l2 ? "pm2_interp" : "pm3_interp", (double)nb * 3 * fs * 4);
-        k_pm3_interp<<<ceil_div(waves, kWaves), 256, 0, s>>>(A, force, dv64, accumulate, d_scal,
-                                                              nseg, ptot);
+        int* items = g_pm3.int_items.as<int>();
+        int* list = g_pm3.int_list.as<int>();
+        const int cap = interp_cap(base, nb);
+        k_interp_count<<<ceil_div(ncell, 256), 256, 0, s>>>(A, seglen, items);
+        C2_HIP(hipGetLastError());
+        C2_TRY(exclusive_scan(items, ncell, g_pm3.dep_scan, s));
+        k_dep_list<<<ceil_div(ncell, 256), 256, 0, s>>>(items, ncell, cap, list);
+        C2_HIP(hipGetLastError());
+        k_pm3_interp<<<interp_grid(), 256, 0, s>>>(A, force, dv64, accumulate, d_scal, seglen,
+                                                   items, list, cap, ptot, err);
         C2_HIP(hipGetLastError());
     }
     return CUBE2_OK;
@@ -1176,6 +1238,8 @@ void pm3_release() {
     g_pm3.dep_scan.release();
     g_pm3.dep_list.release();
     g_pm3.dep_cnts.release();
+    g_pm3.int_items.release();
+    g_pm3.int_list.release();
     g_pm3.h_res.release();
     fft_release();
     green_tiled_release();

@@ -110,9 +110,9 @@ int cube2_set_pp_mode(int mode);
  *                     (built-in: the occupancy maximum);
  *   PP_DIRECT_LG      1..7, chunks of >= 2^v particles sum directly (6);
  *   FFT_MIN_LGC       1..6, minimum log2 column-tile width of the y/z FFT
- *                     passes (built-in: ~4096 LDS elements per tile);
- *   INTERP_SEG        1..32, wavefronts per coarse cell in the PM3
- *                     interpolation (8).
+ *                     passes (built-in: ~2048 LDS elements per tile);
+ *   INTERP_SEG        1..32, particles per wavefront work item of the mesh
+ *                     interpolation, in units of 64 (4).
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,

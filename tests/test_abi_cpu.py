@@ -158,8 +158,8 @@ def test_tuning_api_validates_and_round_trips():
     assert L.cube2_set_tuning(99, 1) == _lib.ERR_ARG
     assert L.cube2_set_tuning(_lib.TUNE_PP_DIRECT_LG, 9) == _lib.ERR_ARG
     assert L.cube2_set_tuning(_lib.TUNE_INTERP_SEG, 33) == _lib.ERR_ARG
-    _lib.set_tuning(_lib.TUNE_INTERP_SEG, 4)
-    assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 4
+    _lib.set_tuning(_lib.TUNE_INTERP_SEG, 16)
+    assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 16
     _lib.set_tuning(_lib.TUNE_INTERP_SEG, 0)
-    assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 8          # built-in choice
+    assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 4          # built-in choice
     assert _lib.get_tuning(_lib.TUNE_PP_DIRECT_LG) == 6
