@@ -420,43 +420,74 @@ static int deposit_grid(size_t lds) {
 }
 
 // Force planes over Fortran indices [0, nfp+1]^3 (kick.f90:545-568) and the
-// interior max |F|^2 (kick.f90:494).  Block = 64 x-points x 4 rows; grid =
-// (x tiles * y tiles, z, job): 32-bit index math, coalesced along x.
+// interior max |F|^2 (kick.f90:494).  A block owns a tile of W x-points x
+// kGradH rows and marches through up to Zc output planes: the phi planes it
+// needs pass through a 5-plane LDS ring, each read from HBM once (with its
+// 2-wide x/y halo), instead of each z-plane block re-reading its 4
+// neighbour planes (neighbouring planes' blocks run on other XCDs, each
+// with its own L2).
+constexpr int kGradH = 8;
+constexpr int kGradW = 64;
+
 __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi, size_t rs,
                                                   float* __restrict__ force, int n, int nfp,
-                                                  int nfb, unsigned* __restrict__ f2bits) {
+                                                  int nfb, int W, int ntx, int Zc,
+                                                  unsigned* __restrict__ f2bits) {
+    __shared__ float ring[5][kGradH + 4][kGradW + 4];
     const int nf = nfp + 2;
-    const int xt = (nf + 63) >> 6;
-    const int i0 = (int)(blockIdx.x % xt) * 64 + (threadIdx.x & 63);
-    const int j0 = (int)(blockIdx.x / xt) * 4 + (threadIdx.x >> 6);
-    const int k0 = blockIdx.y;
+    const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+    const int x0 = tx * W, y0 = ty * kGradH;
+    const int k_begin = blockIdx.y * Zc, k_end = min(nf, k_begin + Zc);
     const int jb = blockIdx.z;
-    float f2 = 0.0f;
-    if (i0 < nf && j0 < nf) {
-        const float* P = phi + (size_t)jb * rs;
-        const int x = i0 + nfb - 1, y = j0 + nfb - 1, z = k0 + nfb - 1;
-        const int sr = n + 2;
-        const size_t sp = (size_t)n * (n + 2);
-        const size_t c = (size_t)z * sp + (size_t)y * sr + x;
-        // real(4) weights of alpha = 4/3 (parameters.f90:49-50)
-        const float al = 4.0f / 3.0f;
-        const float w0 = (al - 1.0f) / 4.0f, w1 = -al / 2.0f, w2 = al / 2.0f, w3 = (1.0f - al) / 4.0f;
-        float fx, fy, fz;
-        {
-#pragma clang fp contract(off)
-            // phi arrives divided by n^3 (folded into the Green's table)
-            fx = P[c - 2] * w0 + P[c - 1] * w1 + P[c + 1] * w2 + P[c + 2] * w3;
-            fy = P[c - 2 * sr] * w0 + P[c - sr] * w1 + P[c + sr] * w2 + P[c + 2 * sr] * w3;
-            fz = P[c - 2 * sp] * w0 + P[c - sp] * w1 + P[c + sp] * w2 + P[c + 2 * sp] * w3;
+    const float* P = phi + (size_t)jb * rs;
+    const int sr = n + 2;
+    const size_t sp = (size_t)n * sr;
+    const int ex = min(W, nf - x0), ey = min(kGradH, nf - y0);  // outputs of the tile
+    const int lw = ex + 4, lh = ey + 4;
+    // phi plane of output plane k (storage z = k + nfb - 1) into a ring slot
+    auto load = [&](int k) {
+        const int slot = (k + 5) % 5;
+        const size_t zoff = (size_t)(k + nfb - 1) * sp;
+        for (int e = threadIdx.x; e < lw * lh; e += blockDim.x) {
+            const int lx = e % lw, ly = e / lw;
+            ring[slot][ly][lx] = P[zoff + (size_t)(y0 + ly + nfb - 3) * sr + (x0 + lx + nfb - 3)];
         }
-        const size_t fs = (size_t)nf * nf * nf;
-        const size_t u = ((size_t)k0 * nf + j0) * nf + i0;
-        float* F = force + (size_t)jb * 3 * fs;
-        F[u] = fx;
-        F[fs + u] = fy;
-        F[2 * fs + u] = fz;
-        if (i0 >= 1 && i0 <= nfp && j0 >= 1 && j0 <= nfp && k0 >= 1 && k0 <= nfp)
-            f2 = fx * fx + fy * fy + fz * fz;
+    };
+    // real(4) weights of alpha = 4/3 (parameters.f90:49-50)
+    const float al = 4.0f / 3.0f;
+    const float w0 = (al - 1.0f) / 4.0f, w1 = -al / 2.0f, w2 = al / 2.0f, w3 = (1.0f - al) / 4.0f;
+    const size_t fs = (size_t)nf * nf * nf;
+    float* F = force + (size_t)jb * 3 * fs;
+    float f2 = 0.0f;
+    for (int k = k_begin - 2; k < k_begin + 2; k++) load(k);
+    for (int k0 = k_begin; k0 < k_end; k0++) {
+        load(k0 + 2);
+        __syncthreads();
+        const int sm2 = (k0 + 3) % 5, sm1 = (k0 + 4) % 5, s0 = k0 % 5, sp1 = (k0 + 1) % 5,
+                  sp2 = (k0 + 2) % 5;
+        for (int e = threadIdx.x; e < ex * ey; e += blockDim.x) {
+            const int lx = e % ex, ly = e / ex;
+            const int cx = lx + 2, cy = ly + 2;
+            float fx, fy, fz;
+            {
+#pragma clang fp contract(off)
+                // phi arrives divided by n^3 (folded into the Green's table)
+                fx = ring[s0][cy][cx - 2] * w0 + ring[s0][cy][cx - 1] * w1 +
+                     ring[s0][cy][cx + 1] * w2 + ring[s0][cy][cx + 2] * w3;
+                fy = ring[s0][cy - 2][cx] * w0 + ring[s0][cy - 1][cx] * w1 +
+                     ring[s0][cy + 1][cx] * w2 + ring[s0][cy + 2][cx] * w3;
+                fz = ring[sm2][cy][cx] * w0 + ring[sm1][cy][cx] * w1 + ring[sp1][cy][cx] * w2 +
+                     ring[sp2][cy][cx] * w3;
+            }
+            const int i0 = x0 + lx, j0 = y0 + ly;
+            const size_t u = ((size_t)k0 * nf + j0) * nf + i0;
+            F[u] = fx;
+            F[fs + u] = fy;
+            F[2 * fs + u] = fz;
+            if (i0 >= 1 && i0 <= nfp && j0 >= 1 && j0 <= nfp && k0 >= 1 && k0 <= nfp)
+                f2 = fmaxf(f2, fx * fx + fy * fy + fz * fz);
+        }
+        __syncthreads();  // the next load overwrites plane k0 - 2
     }
     const float fv[1] = {f2};
     block_max_to<1>(f2bits, fv);
@@ -748,6 +779,10 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     const int n = base.n, nfp = base.nfp;
     const size_t rs = (size_t)(n + 2) * n * n;
     const size_t fs = (size_t)(nfp + 2) * (nfp + 2) * (nfp + 2);
+    if (base.nfb < 3) {  // the 4-point gradient reads 2 cells past the force planes
+        set_error("mesh buffer of %d fine cells is below the gradient stencil's 3", base.nfb);
+        return CUBE2_ERR_GEOMETRY;
+    }
     C2_TRY(batch_reserve(base, nb));
     MeshArgs A = base;
     A.jobs = d_jobs;
@@ -828,8 +863,11 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     {
         Stage st(l2 ? "pm2_grad" : "pm3_grad", mesh_bytes + (double)nb * 3 * fs * 4);
         const int nfq = nfp + 2;
-        dim3 grid((unsigned)(((nfq + 63) / 64) * ((nfq + 3) / 4)), (unsigned)nfq, (unsigned)nb);
-        k_pm3_grad<<<grid, 256, 0, s>>>(rho, rs, force, n, nfp, base.nfb, f2bits);
+        const int ntx = (nfq + kGradW - 1) / kGradW, W = (nfq + ntx - 1) / ntx;
+        const int nzc = (nfq + 63) / 64, Zc = (nfq + nzc - 1) / nzc;
+        const int nty = (nfq + kGradH - 1) / kGradH;
+        dim3 grid((unsigned)(ntx * nty), (unsigned)nzc, (unsigned)nb);
+        k_pm3_grad<<<grid, 256, 0, s>>>(rho, rs, force, n, nfp, base.nfb, W, ntx, Zc, f2bits);
         C2_HIP(hipGetLastError());
     }
     if (dbg && dbg->force) {
