@@ -77,6 +77,8 @@ def lib():
         L.or_drift.argtypes = [C.POINTER(OrGeom), i32p, i64p, i16p, f32p, C.c_float, C.c_float,
                                i32p, i64p, i16p, f32p, C.c_int64, C.POINTER(C.c_int64)]
         L.or_velocity_apply.argtypes = [C.POINTER(OrGeom), i32p, i64p, f32p, f64p, f32p]
+        L.or_direct_pp_f64_cells.argtypes = [f64p, C.c_int, C.c_double, i32p, C.c_int, f64p,
+                                             C.c_double, f64p]
         L.or_direct_pp_f64.argtypes = [f64p, C.c_int, C.c_double, i32p, C.c_int, f64p, C.c_double,
                                        f64p]
         _LIB = L
@@ -287,12 +289,14 @@ def velocity_apply(geom, rhoc, idx_b_r, vp, dv64):
     return vp2, vm
 
 
-def direct_pp_f64(pos, box, targets, ranges, app):
+def direct_pp_f64(pos, box, targets, ranges, app, cells=False):
+    """FP64 direct sum for the targets; cells=True bins the particles so each
+    target scans its 27 neighbour cells (same pairs, other summation order)."""
     pos = _c(pos, np.float64)
     targets = _c(targets, np.int32)
     acc = np.zeros((targets.size, 3), np.float64)
-    lib().or_direct_pp_f64(pos, pos.shape[0], float(box), targets, targets.size,
-                           _c(ranges, np.float64), float(app), acc)
+    f = lib().or_direct_pp_f64_cells if cells else lib().or_direct_pp_f64
+    f(pos, pos.shape[0], float(box), targets, targets.size, _c(ranges, np.float64), float(app), acc)
     return acc
 
 

@@ -572,3 +572,77 @@ void or_direct_pp_f64(const double* pos, int n, double box, const int* targets, 
         acc[3 * q + 2] = a[2];
     }
 }
+
+/* The same FP64 direct sum with the particles binned into cubic cells of
+ * about the smallest range (at most 256 per side), each target scanning the
+ * cells within its own range (falls back to all pairs when fewer than 3
+ * cells fit the box). */
+void or_direct_pp_f64_cells(const double* pos, int n, double box, const int* targets, int nt,
+                            const double* range_t, double app, double* acc) {
+    double rmin = 0.0;
+    for (int q = 0; q < nt; q++)
+        if (range_t[q] > 0.0 && (rmin == 0.0 || range_t[q] < rmin)) rmin = range_t[q];
+    int nc = rmin > 0.0 ? (int)floor(box / rmin) : 1;
+    if (nc > 256) nc = 256;
+    if (nc < 3) {
+        or_direct_pp_f64(pos, n, box, targets, nt, range_t, app, acc);
+        return;
+    }
+    const double cs = box / nc;
+    const size_t ncell = (size_t)nc * nc * nc;
+    int* cnt = (int*)calloc(ncell + 1, sizeof(int));
+    int* cell_of = (int*)malloc(sizeof(int) * (size_t)n);
+    int* order = (int*)malloc(sizeof(int) * (size_t)n);
+    for (int j = 0; j < n; j++) {
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            double x = pos[3 * j + d] - box * floor(pos[3 * j + d] / box);
+            int k = (int)(x / cs);
+            c[d] = k >= nc ? nc - 1 : (k < 0 ? 0 : k);
+        }
+        cell_of[j] = (c[2] * nc + c[1]) * nc + c[0];
+        cnt[cell_of[j] + 1]++;
+    }
+    for (size_t c = 0; c < ncell; c++) cnt[c + 1] += cnt[c];
+    int* fill = (int*)malloc(sizeof(int) * ncell);
+    memcpy(fill, cnt, sizeof(int) * ncell);
+    for (int j = 0; j < n; j++) order[fill[cell_of[j]]++] = j;
+    free(fill);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nt; q++) {
+        const int i = targets[q];
+        const double rg = range_t[q];
+        double a[3] = {0, 0, 0};
+        const int ci = cell_of[i];
+        const int cx = ci % nc, cy = (ci / nc) % nc, cz = ci / (nc * nc);
+        int m = (int)ceil(rg / cs);
+        if (2 * m + 1 > nc) m = (nc - 1) / 2;  /* the whole box, each cell once */
+        for (int dz = -m; dz <= m; dz++)
+            for (int dy = -m; dy <= m; dy++)
+                for (int dx = -m; dx <= m; dx++) {
+                    const int c = ((((cz + dz) % nc + nc) % nc) * nc + ((cy + dy) % nc + nc) % nc) * nc +
+                                  ((cx + dx) % nc + nc) % nc;
+                    for (int s = cnt[c]; s < cnt[c + 1]; s++) {
+                        const int j = order[s];
+                        double dd[3], r2 = 0;
+                        for (int d = 0; d < 3; d++) {
+                            double v = pos[3 * j + d] - pos[3 * i + d];
+                            v -= box * floor(v / box + 0.5);
+                            dd[d] = v;
+                            r2 += v * v;
+                        }
+                        if (r2 > 0 && r2 < rg * rg) {
+                            const double r = sqrt(r2);
+                            const double f = F_ra64(r, app) - F_ra64(r, rg);
+                            for (int d = 0; d < 3; d++) a[d] += f * dd[d] / r;
+                        }
+                    }
+                }
+        acc[3 * q + 0] = a[0];
+        acc[3 * q + 1] = a[1];
+        acc[3 * q + 2] = a[2];
+    }
+    free(cnt);
+    free(cell_of);
+    free(order);
+}

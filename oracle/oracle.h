@@ -124,6 +124,8 @@ void or_velocity_apply(const or_geom* g, const int* rhoc, const int64_t* idx_b_r
 /* FP64 direct sum of the PP kernel over all pairs within range_i, with the
  * periodic minimum image, for a list of target particles.  pos in
  * standard cells [0, box), range per target. */
+void or_direct_pp_f64_cells(const double* pos, int n, double box, const int* targets, int nt,
+                            const double* range_t, double app, double* acc);
 void or_direct_pp_f64(const double* pos /*[n][3]*/, int n, double box, const int* targets,
                       int nt, const double* range_t, double app, double* acc /*[nt][3]*/);
 

@@ -1,0 +1,94 @@
+"""Precision sweep at 256^3 (C1 particles): the PP softening swept over
+0.02, 0.05 and 0.1 fine cells (level-2 fine cells of 0.5 standard cells:
+app = 0.01, 0.025, 0.05 standard cells) to stress close pairs.  For each
+pair arithmetic (REF: the reference's FP32 rounding; FAST; PRECISE: FP64
+pair sums of a cancellation-free law) the PP Delta-v of a seeded 10^4
+particle subsample is compared with an FP64 direct sum of the same law
+over the same pairs (the oracle); relative RMS and the largest error
+(relative to the RMS force) are printed and bounded."""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS_FINE = [0.02, 0.05, 0.1]
+# relative RMS bounds per arithmetic (measured on MI355X: ref and fast about
+# 9e-6, precise about 9e-8, at every softening; about 4x margin)
+BOUND = {"ref": 4e-5, "fast": 4e-5, "precise": 4e-7}
+
+
+@pytest.fixture(scope="module")
+def c1():
+    import torch
+    assert torch.cuda.is_available()
+    from cube2_amd import _lib
+    from cube2_amd.geometry import C1
+    from cube2_amd.kick import select_levels
+    from cube2_amd.layout import build_layout
+    from cube2_amd.synth import config1
+    _lib.lib()
+    pos, _ = config1(seed=1)
+    lay = build_layout(pos, None, C1)
+    ires, _, _ = select_levels(C1, lay.rhoc)
+    return pos, lay, ires
+
+
+def _targets(pos, lay, ires, n=10000):
+    from cube2_amd.geometry import APM3
+    g = lay.geom
+    rng = np.random.default_rng(2024)
+    tg = rng.choice(pos.shape[0], n, replace=False)
+    phys = np.flatnonzero(lay.physical_mask())
+    slot_of = np.empty(pos.shape[0], np.int64)
+    slot_of[lay.src[phys]] = phys
+    # level of each target's subtile -> its PP range (apm3, parameters.f90:39)
+    c = np.floor(np.mod(pos[tg], g.ng) / g.ratio_cs).astype(np.int64) // g.ntt
+    t, s3 = c // g.nns, c % g.nns
+    it = (((t[:, 2] * g.nnt + t[:, 1]) * g.nnt + t[:, 0]) * g.nns + s3[:, 2]) * g.nns ** 2 + \
+        s3[:, 1] * g.nns + s3[:, 0]
+    ranges = APM3[ires[it] - 1].astype(np.float64)
+    return tg.astype(np.int32), slot_of[tg], ranges
+
+
+def test_precision_sweep_against_fp64(c1, oracle_lib):
+    from cube2_amd import _lib
+    from cube2_amd.kick import ShortRangeKick
+    pos, lay0, ires = c1
+    amid, dt = 0.02, 1e-3
+    tg, slots, ranges = _targets(pos, lay0, ires)
+    # the FP64 sum runs over the positions the kernels decode (int16 codes)
+    phys = np.flatnonzero(lay0.physical_mask())
+    pdec = np.empty_like(pos)
+    pdec[lay0.src[phys]] = lay0.decoded_positions()[phys]
+    rows = []
+    for eps in EPS_FINE:
+        g = lay0.geom.with_(app=eps * 0.5)
+        lay = dataclasses.replace(lay0, geom=g)
+        acc = oracle_lib.direct_pp_f64(pdec, g.ng, tg, ranges, g.app, cells=True)
+        truth = acc * amid * dt / (6 * math.pi)
+        scale = math.sqrt((truth ** 2).sum(1).mean())
+        for name, mode in (("ref", _lib.PP_REF), ("fast", _lib.PP_FAST),
+                           ("precise", _lib.PP_PRECISE)):
+            _lib.set_pp_mode(mode)
+            k = ShortRangeKick(lay)
+            k.set_levels(ires)
+            k.kick(1.0, amid, dt, pm3=False, pp=True)
+            dv = k.vp_host()[slots].astype(np.float64)       # zero initial velocities
+            err = dv - truth
+            rel_rms = math.sqrt((err ** 2).sum() / (truth ** 2).sum())
+            max_rel = float(np.sqrt((err ** 2).sum(1)).max() / scale)
+            rows.append((eps, name, rel_rms, max_rel))
+            print(f"eps {eps} fine cells ({g.app} std): {name:8s} relative RMS {rel_rms:.3e}, "
+                  f"max |error| / RMS force {max_rel:.3e}", flush=True)
+    _lib.set_pp_mode(_lib.PP_REF)
+    for eps, name, rel_rms, _ in rows:
+        assert rel_rms < BOUND[name], (eps, name, rel_rms)
+    # the FP64 sums are at least ten times closer than the reference's rounding
+    for eps in EPS_FINE:
+        r = {n: e for (x, n, e, _) in rows if x == eps}
+        assert r["precise"] * 10 <= r["ref"] or r["ref"] < 1e-6
