@@ -130,3 +130,15 @@ def test_indexed_sort_orders_and_permutes(oracle_lib):
     idx = oracle_lib.indexed_sort(arr)
     assert np.all(np.diff(arr[idx - 1]) >= 0)
     assert sorted(idx.tolist()) == list(range(1, arr.size + 1))
+
+
+def test_cell_binned_fp64_direct_sum_equals_all_pairs(oracle_lib):
+    """The binned FP64 direct sum visits the same pairs as the all-pairs one
+    (periodic minimum image, per-target ranges, a dense clump)."""
+    rng = np.random.default_rng(0)
+    pos = np.concatenate([rng.uniform(0, 32, (20000, 3)), 16 + rng.normal(0, 0.5, (3000, 3))]) % 32
+    tg = rng.choice(pos.shape[0], 400, replace=False).astype(np.int32)
+    rg = rng.choice([1.75, 0.875, 0.4375, 0.29], 400)
+    a = oracle_lib.direct_pp_f64(pos, 32.0, tg, rg, 0.05)
+    b = oracle_lib.direct_pp_f64(pos, 32.0, tg, rg, 0.05, cells=True)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
