@@ -151,5 +151,9 @@ def parse_parameters_f90(path: str | Path) -> Geometry:
 C0 = Geometry(ng=64, nnt=1, nns=1, ratio_cs=4, np_nc=4, ngb=16, app=0.05)
 # C1: 256^3 particles, fine mesh 512^3 as 8^3 subtiles of 64^3 fine cells.
 C1 = Geometry(ng=256, nnt=4, nns=2, ratio_cs=4, np_nc=4, ngb=24, app=0.05)
+# C2 (batching): 16^3 = 4096 tiles of 32^3 fine cells at level 2 (16 standard
+# cells, 4 coarse cells per tile edge), one subtile per tile, a one-coarse-
+# cell tile buffer (all PM3 and PP read; PM2 is not in this kick).
+C2 = Geometry(ng=256, nnt=16, nns=1, ratio_cs=4, np_nc=4, ngb=4, app=0.05)
 # The reference's own compiled default (cuda_parameters_generated.h).
 REFERENCE_DEFAULT = Geometry()

@@ -84,6 +84,26 @@ class ShortRangeKick:
         self.greens = {} if greens is None else dict(greens)
         torch.cuda.synchronize()
 
+    @classmethod
+    def from_device(cls, geom: Geometry, rhoc, idx_b_r, xp, vp, n_used: int,
+                    greens: dict | None = None) -> "ShortRangeKick":
+        """A kick over an image already in HBM (e.g. from the device layout
+        builder), without a host copy of the particle arrays."""
+        import torch
+        self = cls.__new__(cls)
+        self.torch = torch
+        self.geom = geom
+        self.layout = None
+        self.dev = rhoc.device
+        _lib.set_params(geom)
+        self.d_rhoc, self.d_idx, self.d_xp, self.d_vp = rhoc, idx_b_r, xp, vp
+        self.n_used = int(n_used)
+        self.ix = ixyz3(geom)
+        self.ires, self.isort, self.napm = select_levels(geom, rhoc.cpu().numpy(), self.ix)
+        self.greens = {} if greens is None else dict(greens)
+        torch.cuda.synchronize()
+        return self
+
     def set_levels(self, ires: np.ndarray) -> None:
         self.ires = np.ascontiguousarray(ires, np.int32)
         self.isort = np.argsort(-self.ires, kind="stable").astype(np.int32) + 1

@@ -109,6 +109,22 @@ def config0(seed: int = 0, geom: Geometry | None = None):
     return zeldovich_lattice(64, g, 100.0, 49.0, seed)
 
 
+def config2(seed: int = 2, geom: Geometry | None = None, cap: int = 200_000):
+    """Batching workload: per-tile particle counts lognormal (median 4000,
+    sigma 1.5, capped at 200k), uniform within each tile of the C2 geometry.
+    Returns positions, velocities and the counts (tile order z, y, x)."""
+    from .geometry import C2
+    g = geom or C2
+    rng = np.random.default_rng(seed)
+    cnt = np.minimum(np.rint(np.exp(rng.normal(np.log(4000.0), 1.5, g.ntile))), cap).astype(np.int64)
+    edge = g.ng / g.nnt
+    t = np.repeat(np.arange(g.ntile), cnt)
+    corner = np.stack([t % g.nnt, (t // g.nnt) % g.nnt, t // (g.nnt * g.nnt)], 1) * edge
+    pos = corner + rng.uniform(0.0, edge, (t.size, 3))
+    vel = (rng.standard_normal((t.size, 3)) * 0.1).astype(np.float32)
+    return pos, vel, cnt
+
+
 def config1(seed: int = 1, geom: Geometry | None = None):
     """256^3 particles in 300 Mpc/h: 216^3 Zel'dovich lattice (60%) plus 2000
     Plummer spheres holding the remaining 40%."""
