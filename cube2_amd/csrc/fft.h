@@ -24,12 +24,20 @@ void fft_release();
 // pointer; nullptr: no scaling).
 int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale, hipStream_t s);
 // C2R along x for every row pair.
-int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s);
+// The part of the real result a caller reads: planes z0 .. z0+nz-1 and, in
+// them, rows 2*yp0 .. 2*(yp0+nyp)-1.  The inverse passes skip the rest
+// (those points are left undefined).
+struct FFTWin {
+    int z0, nz, yp0, nyp;
+};
+
+int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s,
+              const FFTWin* w = nullptr);
 // Complex transforms along y (axis 1) or z (axis 2) of the half spectrum.
 //   mode 0: forward, 1: inverse, 2: forward, multiply by the Green's table,
 //   inverse; green is the tile-ordered copy from green_tiled().
 int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
-             const float* green, hipStream_t s);
+             const float* green, hipStream_t s, const FFTWin* w = nullptr);
 
 // Tile-ordered device copy of a Green's table G[kz][ky][kx] / n^3 for the z pass,
 // made once per (G, n) and cached (G's contents must not change).

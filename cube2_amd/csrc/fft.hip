@@ -250,9 +250,19 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d
     }
 }
 
+// Row pair g of a window of nz planes from z0 and nyp row pairs from yp0.
+__device__ inline float* row_pair_w(float* d, int g, int n, size_t rs, FFTWin w) {
+    const int per_job = w.nz * w.nyp;
+    const int job = g / per_job;
+    const int rem = g - job * per_job;
+    const int z = w.z0 + rem / w.nyp, yp = w.yp0 + rem % w.nyp;
+    return d + (size_t)job * rs + ((size_t)z * n + 2 * yp) * (n + 2);
+}
+
 template <bool R7>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d, FFTPlan P,
-                                                           int npairs, size_t rs, int lgL) {
+                                                           int npairs, size_t rs, int lgL,
+                                                           FFTWin w) {
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1, half = n / 2;
     const int L = 1 << lgL, es = L + 1;  // padded: the lanes of a row hit distinct banks
@@ -264,7 +274,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
 #pragma unroll 2
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) continue;
-        const float2* ra = reinterpret_cast<const float2*>(row_pair(d, g0 + l, n, rs));
+        const float2* ra = reinterpret_cast<const float2*>(row_pair_w(d, g0 + l, n, rs, w));
         const float2* rb = ra + nh;
         for (int k = lane; k < n; k += 64) {
             float2 a, b;
@@ -288,7 +298,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
     fft_lds<1, kFFTElems, R7>(A, tw, P, lgL, 1, es);
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) break;
-        float* ra = row_pair(d, g0 + l, n, rs);
+        float* ra = row_pair_w(d, g0 + l, n, rs, w);
         for (int t = lane; t < n; t += 64) {
             const float2 v = A[l + t * es];
             ra[t] = v.x;
@@ -302,7 +312,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
 template <int MODE, int CAP, bool R7>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dreal, FFTPlan P,
                                                           size_t rs, int axis, int lgC,
-                                                          const float* __restrict__ green) {
+                                                          const float* __restrict__ green, int o0) {
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1;
     const int C = 1 << lgC;
@@ -311,7 +321,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
     float2* A = sm + n;
     load_twiddles(tw, P);
     const int tile = blockIdx.x % ntiles;
-    const int o = blockIdx.x / ntiles;
+    const int o = o0 + blockIdx.x / ntiles;  // plane (axis 1) or row (axis 2)
     const int job = blockIdx.y;
     const int kx0 = tile * C, Cw = min(C, nh - kx0);
     float2* d = reinterpret_cast<float2*>(dreal + (size_t)job * rs);
@@ -469,19 +479,24 @@ int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale,
     return CUBE2_OK;
 }
 
-int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s) {
+int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s, const FFTWin* win) {
     const int n = P.n;
     const int lgL = lg_floor(std::max(1, kFFTElems / n));
     const int L = 1 << lgL;
-    const int64_t npairs = (int64_t)nb * n * (n / 2);
+    const FFTWin w = win ? *win : FFTWin{0, n, 0, n / 2};
+    if (w.z0 < 0 || w.nz < 1 || w.z0 + w.nz > n || w.yp0 < 0 || w.nyp < 1 || w.yp0 + w.nyp > n / 2) {
+        set_error("FFT window outside the mesh");
+        return CUBE2_ERR_ARG;
+    }
+    const int64_t npairs = (int64_t)nb * w.nz * w.nyp;
     if (npairs >= (int64_t)1 << 31) {
         set_error("FFT batch too large");
         return CUBE2_ERR_CAPACITY;
     }
     const size_t lds = (size_t)(n + (L + 1) * n) * sizeof(float2);
     const unsigned grid = (unsigned)((npairs + L - 1) / L);
-    if (P.has7) k_fft_x_c2r<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL);
-    else k_fft_x_c2r<false><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL);
+    if (P.has7) k_fft_x_c2r<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, w);
+    else k_fft_x_c2r<false><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, w);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
@@ -500,28 +515,39 @@ int fft_cols_lgC(int n) {
 
 template <bool R7>
 static void launch_cols(bool wide, int mode, dim3 grid, size_t lds, hipStream_t s, float* d,
-                        const FFTPlan& P, size_t rs, int axis, int lgC, const float* green) {
+                        const FFTPlan& P, size_t rs, int axis, int lgC, const float* green,
+                        int o0) {
     if (wide) {
-        if (mode == 0) k_fft_cols<0, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else if (mode == 1) k_fft_cols<1, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else k_fft_cols<2, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        if (mode == 0) k_fft_cols<0, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
+        else if (mode == 1) k_fft_cols<1, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
+        else k_fft_cols<2, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
     } else {
-        if (mode == 0) k_fft_cols<0, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else if (mode == 1) k_fft_cols<1, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
-        else k_fft_cols<2, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green);
+        if (mode == 0) k_fft_cols<0, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
+        else if (mode == 1) k_fft_cols<1, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
+        else k_fft_cols<2, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
     }
 }
 
 int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
-             const float* green, hipStream_t s) {
+             const float* green, hipStream_t s, const FFTWin* win) {
     const int n = P.n, nh = n / 2 + 1;
     const int lgC = fft_cols_lgC(n);
     const int C = 1 << lgC;
     const int ntiles = (nh + C - 1) / C;
-    dim3 grid((unsigned)(n * ntiles), (unsigned)nb);
+    // a window restricts the planes of an inverse y pass
+    int o0 = 0, no = n;
+    if (win && axis == 1 && mode == 1) {
+        if (win->z0 < 0 || win->nz < 1 || win->z0 + win->nz > n) {
+            set_error("FFT window outside the mesh");
+            return CUBE2_ERR_ARG;
+        }
+        o0 = win->z0;
+        no = win->nz;
+    }
+    dim3 grid((unsigned)(no * ntiles), (unsigned)nb);
     const size_t lds = (size_t)(n + C * n) * sizeof(float2);
-    if (P.has7) launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green);
-    else launch_cols<false>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green);
+    if (P.has7) launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0);
+    else launch_cols<false>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
