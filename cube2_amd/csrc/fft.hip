@@ -312,7 +312,8 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
 template <int MODE, int CAP, bool R7>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dreal, FFTPlan P,
                                                           size_t rs, int axis, int lgC,
-                                                          const float* __restrict__ green, int o0) {
+                                                          const float* __restrict__ green, int o0,
+                                                          int ilo, int ihi) {
     extern __shared__ float2 sm[];
     const int n = P.n, nh = n / 2 + 1;
     const int C = 1 << lgC;
@@ -350,7 +351,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
             fft_lds<1, CAP, R7>(A, tw, P, lgC, 1, C);
         }
     }
-    for (int e = threadIdx.x; e < n * C; e += kFFTThreads) {
+    // rows ilo .. ihi-1 go back (an inverse z pass stores only the planes
+    // the later passes read)
+    for (int e = (ilo << lgC) + threadIdx.x; e < (ihi << lgC); e += kFFTThreads) {
         const int i = e >> lgC, c = e & (C - 1);
         if (c < Cw) d[obase + i * rowstride + c] = A[e];
     }
@@ -516,15 +519,15 @@ int fft_cols_lgC(int n) {
 template <bool R7>
 static void launch_cols(bool wide, int mode, dim3 grid, size_t lds, hipStream_t s, float* d,
                         const FFTPlan& P, size_t rs, int axis, int lgC, const float* green,
-                        int o0) {
+                        int o0, int ilo, int ihi) {
     if (wide) {
-        if (mode == 0) k_fft_cols<0, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
-        else if (mode == 1) k_fft_cols<1, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
-        else k_fft_cols<2, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
+        if (mode == 0) k_fft_cols<0, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else if (mode == 1) k_fft_cols<1, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else k_fft_cols<2, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
     } else {
-        if (mode == 0) k_fft_cols<0, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
-        else if (mode == 1) k_fft_cols<1, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
-        else k_fft_cols<2, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0);
+        if (mode == 0) k_fft_cols<0, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else if (mode == 1) k_fft_cols<1, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else k_fft_cols<2, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
     }
 }
 
@@ -534,20 +537,26 @@ int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
     const int lgC = fft_cols_lgC(n);
     const int C = 1 << lgC;
     const int ntiles = (nh + C - 1) / C;
-    // a window restricts the planes of an inverse y pass
-    int o0 = 0, no = n;
-    if (win && axis == 1 && mode == 1) {
+    // a window restricts the planes of an inverse y pass and the planes an
+    // inverse z pass stores
+    int o0 = 0, no = n, ilo = 0, ihi = n;
+    if (win) {
         if (win->z0 < 0 || win->nz < 1 || win->z0 + win->nz > n) {
             set_error("FFT window outside the mesh");
             return CUBE2_ERR_ARG;
         }
-        o0 = win->z0;
-        no = win->nz;
+        if (axis == 1 && mode == 1) {
+            o0 = win->z0;
+            no = win->nz;
+        } else if (axis == 2 && mode == 2) {
+            ilo = win->z0;
+            ihi = win->z0 + win->nz;
+        }
     }
     dim3 grid((unsigned)(no * ntiles), (unsigned)nb);
     const size_t lds = (size_t)(n + C * n) * sizeof(float2);
-    if (P.has7) launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0);
-    else launch_cols<false>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0);
+    if (P.has7) launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0, ilo, ihi);
+    else launch_cols<false>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0, ilo, ihi);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

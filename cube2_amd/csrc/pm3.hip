@@ -839,17 +839,17 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
         Stage st(l2 ? "pm2_fft_y" : "pm3_fft_y", 2 * mesh_bytes);
         C2_TRY(fft_cols(rho, *P, (int)nb, rs, 1, 0, nullptr, s));
     }
-    {
-        Stage st(l2 ? "pm2_fft_z_green" : "pm3_fft_z_green", 2 * mesh_bytes + (double)(n / 2 + 1) * n * n * 4);
-        C2_TRY(fft_cols(rho, *P, (int)nb, rs, 2, 2, green, s));
-    }
     // The gradient reads phi at storage indices nfb-3 .. nfp+nfb+2 on every
-    // axis: after the inverse z transform only those planes, and in them
-    // those rows, need the inverse y and x transforms (the full potential
-    // is kept when a caller inspects it).
+    // axis: the inverse z transform stores only those planes, the inverse y
+    // and x transforms run on them and, in them, on those rows (the full
+    // potential is kept when a caller inspects it).
     const int wlo = base.nfb - 3, whi = nfp + base.nfb + 3;
     const FFTWin win{wlo, whi - wlo, wlo / 2, (whi + 1) / 2 - wlo / 2};
     const FFTWin* w = (dbg && dbg->phi) ? nullptr : &win;
+    {
+        Stage st(l2 ? "pm2_fft_z_green" : "pm3_fft_z_green", 2 * mesh_bytes + (double)(n / 2 + 1) * n * n * 4);
+        C2_TRY(fft_cols(rho, *P, (int)nb, rs, 2, 2, green, s, w));
+    }
     {
         Stage st(l2 ? "pm2_fft_y_inv" : "pm3_fft_y_inv", 2 * mesh_bytes);
         C2_TRY(fft_cols(rho, *P, (int)nb, rs, 1, 1, nullptr, s, w));
