@@ -17,8 +17,10 @@ _LIB = None
 
 OK, ERR_ARG, ERR_GEOMETRY, ERR_CAPACITY, ERR_HIP, ERR_STATE = range(6)
 PP_EXACT, PP_FAST, PP_REF, PP_PRECISE = 0, 1, 2, 3
-TUNE_PP_BLOCKS_PER_CU, TUNE_PP_DIRECT_LG, TUNE_FFT_MIN_LGC, TUNE_INTERP_SEG = 0, 1, 2, 3
-TUNE_NAMES = {"pp_blocks_per_cu": 0, "pp_direct_lg": 1, "fft_min_lgc": 2, "interp_seg": 3}
+TUNE_PP_BLOCKS_PER_CU, TUNE_PP_DIRECT_LG, TUNE_FFT_MIN_LGC, TUNE_INTERP_SEG, TUNE_PP_DENSE_LG = \
+    0, 1, 2, 3, 4
+TUNE_NAMES = {"pp_blocks_per_cu": 0, "pp_direct_lg": 1, "fft_min_lgc": 2, "interp_seg": 3,
+              "pp_dense_lg": 4}
 
 
 class Cube2Error(RuntimeError):

@@ -15,6 +15,7 @@ from . import _lib
 
 CANDIDATES = {
     "pp_blocks_per_cu": [0, 4, 5, 8],
+    "pp_dense_lg": [0, 5, 6],
     "pp_direct_lg": [0, 7, 5, 4],
     "fft_min_lgc": [0, 3, 4],
     "interp_seg": [0, 2, 8],

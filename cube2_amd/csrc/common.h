@@ -97,7 +97,7 @@ struct Context {
     int64_t staged_xp_n = -1;
     DevBuf d_scal;
     HostBuf h_scal;
-    int tune[CUBE2_TUNE_COUNT] = {0, 0, 0, 0};  // 0: built-in choice
+    int tune[CUBE2_TUNE_COUNT] = {0, 0, 0, 0, 0};  // 0: built-in choice
     // true while work is being recorded into a HIP graph: no host syncs,
     // allocations, memsets or profiler events may be issued
     bool capturing = false;

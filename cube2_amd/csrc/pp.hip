@@ -258,17 +258,24 @@ __global__ void k_pp_ptotal(const int* __restrict__ rhoc, int nt, int ncb, int n
 }
 
 // Work chunks of the pair kernel: <= 64 consecutive physical particles of
-// one PP cell.  flags[a] = 1 where a chunk starts.
+// one PP cell, <= 2^dense_lg in cells of >= 64 particles.  flags[a] = 1
+// where a chunk starts.
 // The sorted count n comes from the scan on the device; flags[a] = 0 for
 // n <= a < cap, and for every a if the call has failed.
 __global__ void k_chunk_flags(const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
                               const int* __restrict__ cell_start, const int* __restrict__ n_sorted,
-                              const int* __restrict__ err, int cap, int* __restrict__ flags) {
+                              const int* __restrict__ err, int cap, int dense_lg,
+                              int* __restrict__ flags) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= cap) return;
     const int n = (*err & 4) ? 0 : *n_sorted;
     int f = 0;
-    if (a < n && pos4[a].w > 0.0f) f = ((a - cell_start[cellid[a]]) & 63) == 0;
+    if (a < n && pos4[a].w > 0.0f) {
+        const uint32_t c = cellid[a];
+        const int c0 = cell_start[c];
+        const int mask = cell_start[c + 1] - c0 >= 64 ? (1 << dense_lg) - 1 : 63;
+        f = ((a - c0) & mask) == 0;
+    }
     flags[a] = f;
 }
 
@@ -284,7 +291,7 @@ __device__ inline int spread3(int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4
 // One wavefront per PP cell of at least 64 particles (found as its first
 // chunk): ilist[cell range] = the cell's sorted positions ordered by an
 // 8x8x8 Morton sub-cell key (counting sort in LDS), so that consecutive
-// 64-particle chunks of a dense cell are spatially compact.  The order of
+// chunks of a dense cell are spatially compact.  The order of
 // the cell list itself (the j stream) is not changed.
 __global__ __launch_bounds__(256) void k_pp_subsort(
     const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
@@ -564,7 +571,7 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             Stage st("pp_chunks", (double)P.n_used * 24);
             k_chunk_flags<<<ceil_div(P.cap, 256), 256, 0, s>>>(A.pos4, A.cellid, A.cell_start,
                                                                 n_sorted, &red->err, (int)P.cap,
-                                                                flags);
+                                                                pp_dense_lg(), flags);
             C2_HIP(hipGetLastError());
             C2_TRY(exclusive_scan(flags, (int64_t)P.cap, g_pp.scan_tmp, s));
             k_chunk_emit<<<ceil_div(P.cap, 256), 256, 0, s>>>(flags, (int)P.cap, g_pp.chunks.as<int>());

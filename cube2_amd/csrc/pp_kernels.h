@@ -36,4 +36,10 @@ struct PPArgs {
 
 int launch_pp_pairs(const PPArgs& a, int mode, hipStream_t s);
 
+// log2 of the pair kernel's chunk size in PP cells of >= 64 particles
+inline int pp_dense_lg() {
+    const int v = ctx().tune[CUBE2_TUNE_PP_DENSE_LG];
+    return v ? v : 4;
+}
+
 }  // namespace cube2

@@ -1,19 +1,24 @@
 // PP pair kernel.
 //
-// Persistent wavefronts take chunks of <= 64 physical particles of one PP
-// cell.  The lanes are split into I i-slots x J j-slots (I the chunk size
-// rounded up to a power of two, I*J = 64).  The j of the (2R+1)^2
+// Persistent wavefronts take chunks of physical particles of one PP cell:
+// the whole cell if it holds fewer than 64, else 2^dense_lg at a time (16
+// built in).  The lanes are split into I i-slots x J j-slots (I the chunk
+// size rounded up to a power of two, I*J = 64).  The j of the (2R+1)^2
 // neighbour rows are contiguous runs of the sorted particle stream; a
 // wavefront concatenates them into one candidate list and fetches it 64 at
 // a time (one coalesced load, prefetched a batch ahead, staged in LDS).
 //
-// Full chunks (I = 64: cells of >= 64 particles) sum directly: one i per
-// lane, the force law evaluated as pairs are accepted, partial sums in
-// registers in the reference's j order.  Their i are taken in 8x8x8 Morton
-// sub-cell order (so a chunk is spatially compact), and each fetched batch
-// of j is first reduced to those within the range of the chunk's bounding
-// box.  Smaller chunks append accepted pairs to a per-wavefront LDS queue
-// (ballot + prefix count) and evaluate the force law 64 pairs at a time.
+// Chunks of I >= 2^direct_lg (I >= 4 built in) sum directly: lane (i, s)
+// evaluates the force law on the pairs of i with the j of its j-slot s as
+// they are accepted, partial sums in registers in the reference's j order,
+// combined over the J slots at the end.  The i of cells of >= 64 particles
+// are taken in 8x8x8 Morton sub-cell order (so a chunk is spatially
+// compact), and each fetched batch of j is first reduced to those within
+// the range of the chunk's bounding box: the smaller the chunk, the fewer
+// lanes idle on a j accepted by only some of its i (REF at 256^3: 112 ms
+// with 64-particle chunks, 92 ms with 16).  Smaller chunks append accepted
+// pairs to a per-wavefront LDS queue (ballot + prefix count) and evaluate
+// the force law 64 pairs at a time.
 //
 // kRef: the pair force is the reference C kernel's, operation for operation
 //   (pp_force_kernel.c:308-318, F_ra :45-74), with each correctly rounded
@@ -202,13 +207,10 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
 // register budget is capped for 6 wavefronts per SIMD: the candidate loop is
 // latency-bound, and occupancy 6 measured faster than 4 (REF at C1: 113 vs
 // 155 ms) despite a small spill in the REF variant.
-// Chunks of at least 2^direct_lg particles sum their pairs directly in
-// registers (one i per lane, j in sorted order); smaller chunks queue
-// accepted pairs in LDS and evaluate the force law 64 pairs at a time.
 template <int kMode>
 __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
                                                   const int* __restrict__ nchunks_dev,
-                                                  int direct_lg) {
+                                                  int direct_lg, int dense_lg) {
     __shared__ WaveLDS<kMode> lds[4];
     const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     WaveLDS<kMode>& L = lds[wave];
@@ -219,15 +221,15 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const int a0 = uni(chunks[ck]);
         const uint32_t cid = (uint32_t)uni((int)A.cellid[a0]);
         const int cend = uni(A.cell_start[cid + 1]);
-        const int ni = min(64, cend - a0);
+        // cells of >= 64 particles are visited in sub-cell order (ilist), so
+        // that a chunk is spatially compact
+        const bool grouped = cend - uni(A.cell_start[cid]) >= 64;
+        const int ni = min(grouped ? 1 << dense_lg : 64, cend - a0);
         int lgI = 0;
         while ((1 << lgI) < ni) lgI++;
         const int I = 1 << lgI, J = 64 >> lgI;
         const int islot = lane & (I - 1), jslot = lane >> lgI;
         const bool vi = islot < ni;
-        // cells of >= 64 particles are visited in sub-cell order (ilist), so
-        // that a chunk is spatially compact
-        const bool grouped = cend - uni(A.cell_start[cid]) >= 64;
         const int ia = grouped ? A.ilist[a0 + (vi ? islot : 0)] : a0 + (vi ? islot : 0);
         const float4 p = A.pos4[ia];
         const PPTile T = A.tiles[uni(find_tile2(A.tiles, A.ntile, cid))];
@@ -462,14 +464,15 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
                        : mode == CUBE2_PP_FAST ? "pp_pairs_fastq" : "pp_pairs_precise";
     Stage st(name);
     const unsigned blocks = (unsigned)std::max(1, grid);
-    // full-wavefront chunks sum directly unless tuned otherwise
-    const int direct_lg = ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] ? ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] : 6;
+    // chunks of more than 2 particles sum directly unless tuned otherwise
+    const int direct_lg = ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] ? ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] : 2;
+    const int dense_lg = pp_dense_lg();
     if (mode == CUBE2_PP_REF)
-        k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg);
+        k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
     else if (mode == CUBE2_PP_FAST)
-        k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg);
+        k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
     else
-        k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg);
+        k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

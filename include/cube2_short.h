@@ -108,18 +108,22 @@ int cube2_set_pp_mode(int mode);
  * choice.  Valid values:
  *   PP_BLOCKS_PER_CU  1..16, blocks per CU of the persistent pair kernel
  *                     (built-in: the occupancy maximum);
- *   PP_DIRECT_LG      1..7, chunks of >= 2^v particles sum directly (6);
+ *   PP_DIRECT_LG      1..7, chunks of > 2^(v-1) particles (lanes for >= 2^v
+ *                     i slots) sum directly, smaller ones queue pairs (2);
  *   FFT_MIN_LGC       1..6, minimum log2 column-tile width of the y/z FFT
  *                     passes (built-in: ~2048 LDS elements per tile);
  *   INTERP_SEG        1..32, particles per wavefront work item of the mesh
- *                     interpolation, in units of 64 (4).
+ *                     interpolation, in units of 64 (4);
+ *   PP_DENSE_LG       3..6, log2 of the pair kernel's chunk size in PP
+ *                     cells of >= 64 particles (4).
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
     CUBE2_TUNE_PP_DIRECT_LG = 1,
     CUBE2_TUNE_FFT_MIN_LGC = 2,
     CUBE2_TUNE_INTERP_SEG = 3,
-    CUBE2_TUNE_COUNT = 4
+    CUBE2_TUNE_PP_DENSE_LG = 4,
+    CUBE2_TUNE_COUNT = 5
 };
 int cube2_set_tuning(int key, int value);
 int cube2_get_tuning(int key, int* value); /* the value in use (built-in choice resolved) */

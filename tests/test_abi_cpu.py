@@ -162,4 +162,6 @@ def test_tuning_api_validates_and_round_trips():
     assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 16
     _lib.set_tuning(_lib.TUNE_INTERP_SEG, 0)
     assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 4          # built-in choice
-    assert _lib.get_tuning(_lib.TUNE_PP_DIRECT_LG) == 6
+    assert _lib.get_tuning(_lib.TUNE_PP_DIRECT_LG) == 2
+    assert _lib.get_tuning(_lib.TUNE_PP_DENSE_LG) == 4
+    assert L.cube2_set_tuning(_lib.TUNE_PP_DENSE_LG, 2) == _lib.ERR_ARG

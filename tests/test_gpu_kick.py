@@ -204,8 +204,8 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
     vp0 = k.d_vp.clone()
     r0 = k.kick_graph(1.0, 0.02, 1e-3)
     base = k.vp_host()
-    settings = [(lib.TUNE_PP_BLOCKS_PER_CU, 3), (lib.TUNE_PP_DIRECT_LG, 3),
-                (lib.TUNE_FFT_MIN_LGC, 4), (lib.TUNE_INTERP_SEG, 2)]
+    settings = [(lib.TUNE_PP_BLOCKS_PER_CU, 3), (lib.TUNE_PP_DIRECT_LG, 6),
+                (lib.TUNE_FFT_MIN_LGC, 4), (lib.TUNE_INTERP_SEG, 2), (lib.TUNE_PP_DENSE_LG, 6)]
     try:
         for key, v in settings:
             lib.set_tuning(key, v)
