@@ -114,6 +114,16 @@ void ensure_params();  // params only (no HIP call)
 // ------------------------------------------------------------- helpers
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Workgroups of a launch go to the 8 XCDs of the device round-robin (id b
+// to XCD b % 8), and each XCD has its own L2.  This bijection of [0, W)
+// gives XCD x the contiguous logical range [x*q + min(x, r), ...) (q = W/8,
+// r = W%8), taken in order, so that neighbouring logical blocks (which share
+// cache lines) run on one XCD at about the same time.
+__device__ inline unsigned xcd_remap(unsigned b, unsigned W) {
+    const unsigned q = W >> 3, r = W & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
 // Quantised position of a particle inside its coarse cell, as the
 // reference decodes it: ((int16)(xp + ishift) + rshift) * x_resolution
 // with ishift = -2^15, rshift = 0.5 - ishift, x_resolution = 2^-16

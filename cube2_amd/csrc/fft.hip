@@ -321,9 +321,13 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
     float2* tw = sm;
     float2* A = sm + n;
     load_twiddles(tw, P);
-    const int tile = blockIdx.x % ntiles;
-    const int o = o0 + blockIdx.x / ntiles;  // plane (axis 1) or row (axis 2)
-    const int job = blockIdx.y;
+    // neighbouring tiles of a plane or row share cache lines: they run on one
+    // XCD
+    const unsigned lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int bx = (int)(lb % gridDim.x);
+    const int tile = bx % ntiles;
+    const int o = o0 + bx / ntiles;  // plane (axis 1) or row (axis 2)
+    const int job = (int)(lb / gridDim.x);
     const int kx0 = tile * C, Cw = min(C, nh - kx0);
     float2* d = reinterpret_cast<float2*>(dreal + (size_t)job * rs);
     const size_t rowstride = axis == 1 ? (size_t)nh : (size_t)n * nh;

@@ -317,8 +317,10 @@ __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __rest
                                                        size_t rs, int nbx, int* __restrict__ err) {
     __shared__ unsigned long long box[kBrick * kBrick * kBrick];
     const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
-    const int jb = blockIdx.y;
-    const int b = blockIdx.x;
+    // neighbouring bricks visit the same particles: they run on one XCD
+    const unsigned lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int jb = (int)(lb / gridDim.x);
+    const int b = (int)(lb % gridDim.x);
     const int bx = b % nbx, by = (b / nbx) % nbx, bz = b / (nbx * nbx);
     const int lo[3] = {bx * kBrick, by * kBrick, bz * kBrick};
     for (int q = threadIdx.x; q < kBrick * kBrick * kBrick; q += blockDim.x) box[q] = 0ull;
@@ -435,10 +437,15 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
                                                   unsigned* __restrict__ f2bits) {
     __shared__ float ring[5][kGradH + 4][kGradW + 4];
     const int nf = nfp + 2;
-    const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+    // neighbouring tiles share halo lines: they run on one XCD
+    const unsigned lb = xcd_remap(
+        (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
+        gridDim.x * gridDim.y * gridDim.z);
+    const int bxy = (int)(lb % gridDim.x);
+    const int tx = bxy % ntx, ty = bxy / ntx;
     const int x0 = tx * W, y0 = ty * kGradH;
-    const int k_begin = blockIdx.y * Zc, k_end = min(nf, k_begin + Zc);
-    const int jb = blockIdx.z;
+    const int k_begin = (int)((lb / gridDim.x) % gridDim.y) * Zc, k_end = min(nf, k_begin + Zc);
+    const int jb = (int)(lb / (gridDim.x * gridDim.y));
     const float* P = phi + (size_t)jb * rs;
     const int sr = n + 2;
     const size_t sp = (size_t)n * sr;
