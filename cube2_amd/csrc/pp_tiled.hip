@@ -46,33 +46,41 @@ namespace cube2 {
 
 constexpr int kQ = 128;  // queue entries per wavefront
 
-// FP64 reciprocal with two Newton steps from v_rcp_f64: relative error
-// below 2^-52 for normal inputs.
-__device__ inline double recip64(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(e, y, y);
-    e = fma(-x, y, 1.0);
-    return fma(e, y, y);
+// 1/x from an approximation y0: with e = 1 - x*y0 (one FMA, rounded once),
+// 1/x = y0/(1-e) = y0 (1 + e + e^2 + e^3/(1-e)), so y0 + y0 (e + e^2) is
+// within |e|^3 + 3*2^-53 (relative) of 1/x.  For |e| <= 2^-18 that is below
+// 2^-51.
+__device__ inline double recip_refine(double x, double y0) {
+    const double e = fma(-x, y0, 1.0);
+    return fma(fma(e, e, e), y0, y0);
 }
+
+// FP64 reciprocal of a float: v_rcp_f64 and one cubic refinement.  The
+// self-test checks |1 - x*y| <= 2^-51 for every float x with exponent in
+// [-40, 40].
+__device__ inline double recip64(double x) { return recip_refine(x, __builtin_amdgcn_rcp(x)); }
 
 __device__ inline float div_by(float a, double y) { return (float)((double)a * y); }
 
 struct FraK {
-    float a;       // softening diameter
-    double inv_a;  // 1/a in FP64
-    float c;       // (1/35)/(a*a), as basic_functions.f08:103 rounds it
+    float a;        // softening diameter
+    double inv_a;   // 1/a in FP64
+    double half_a;  // a/2 in FP64
+    float c;        // (1/35)/(a*a), as basic_functions.f08:103 rounds it
 };
 
 // F_ra(r, a) of basic_functions.f08:95-109 with the reference's rounding, for
 // ep = RN(2r/a) <= 2 (the caller handles a = 0 and ep > 2, where F_ra is
 // 1/(r*r)).  Evaluated without divergence: the powers are shared, both
 // polynomials are formed in the reference's operation order, and one is
-// selected.  RN(12/e2) is 12 * (FP64 reciprocal), rounded once to float.
-__device__ inline float fra_ref_poly(float ep, const FraK& K) {
+// selected.  RN(12/e2) is 12 * (FP64 reciprocal), rounded once to float;
+// the reciprocal refines y0 = (a/2 * 1/r)^2, which is within a few float
+// roundings of 1/e2 (ep = RN(2r/a), e2 = RN(ep*ep)).
+__device__ inline float fra_ref_poly(float ep, const FraK& K, double inv_r) {
 #pragma clang fp contract(off)
     const float e2 = ep * ep, e3 = e2 * ep, e4 = e3 * ep, e5 = e4 * ep, e6 = e5 * ep;
-    const float t12 = (float)(12.0 * recip64((double)e2));
+    const double q = K.half_a * inv_r;
+    const float t12 = (float)(12.0 * recip_refine((double)e2, q * q));
     const float hi = K.c * (t12 - 224.0f + 896.0f * ep - 840.0f * e2 + 224.0f * e3 + 70.0f * e4 -
                             48.0f * e5 + 7.0f * e6);
     const float lo = K.c * (224.0f * ep - 224.0f * e3 + 70.0f * e4 + 48.0f * e5 - 21.0f * e6);
@@ -109,22 +117,23 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
     if (kMode == kModeRef) {
 #pragma clang fp contract(off)
         const float rm = sqrtf(e.w);
-        const float frr = (float)recip64((double)(rm * rm));  // RN(1/(r*r))
+        const double inv_rm = recip64((double)rm);
+        // RN(1/(r*r)): the reciprocal of RN(r*r) refined from (1/r)^2
+        const float frr = (float)recip_refine((double)(rm * rm), inv_rm * inv_rm);
         // range term: r < range, so ep <= 2 except by rounding at the cutoff
         const float epr = div_by(2.0f * rm, Kr.inv_a);
-        const float fr = epr > 2.0f ? frr : fra_ref_poly(epr, Kr);
+        const float fr = epr > 2.0f ? frr : fra_ref_poly(epr, Kr, inv_rm);
         // softening term: 1/(r*r) unless r is within app; the polynomial runs
         // only when some lane of the wavefront needs it
         float fa = frr;
         if (Ka.a != 0.0f) {
             const float epa = div_by(2.0f * rm, Ka.inv_a);
             if (__ballot(epa <= 2.0f)) {
-                const float pa = fra_ref_poly(epa, Ka);
+                const float pa = fra_ref_poly(epa, Ka, inv_rm);
                 if (epa <= 2.0f) fa = pa;
             }
         }
         const float fpp = fa - fr;
-        const double inv_rm = recip64((double)rm);
         fx = div_by(fpp * e.x, inv_rm);
         fy = div_by(fpp * e.y, inv_rm);
         fz = div_by(fpp * e.z, inv_rm);
@@ -247,8 +256,10 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         FraK Ka, Kr;
         Ka.a = A.app;
         Ka.inv_a = A.app > 0.0f ? 1.0 / (double)A.app : 0.0;
+        Ka.half_a = 0.5 * (double)A.app;
         Kr.a = range;
         Kr.inv_a = 1.0 / (double)range;
+        Kr.half_a = 0.5 * (double)range;
         {
 #pragma clang fp contract(off)
             Ka.c = 1.0f / 35.0f / (A.app * A.app);
@@ -477,7 +488,13 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
     return CUBE2_OK;
 }
 
-// --- arithmetic self-test: the reciprocal-based quotient against IEEE a/b.
+// --- arithmetic self-test: the reciprocal-based quotients of the
+// reference-arithmetic kernel against IEEE float division.
+// seed != 0: n seeded random operands (a, b with exponents in [-40, 40],
+// r and the length l in [-20, 20]); seed == 0: every float with exponent in
+// [-40, 40] (n = 81 * 2^23) as b and r, with a = l = 1.  Checked: a/b,
+// 12/b, |1 - b*recip64(b)| <= 2^-51, 1/RN(r*r) refined from (1/r)^2, and
+// 12/e2 for ep = RN(2r/l), e2 = RN(ep*ep) refined from (l/2 * 1/r)^2.
 __global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -488,24 +505,43 @@ __global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
         s ^= s >> 27;
         return (uint32_t)((s * 0x2545F4914F6CDD1Dull) >> 32);
     };
-    // normal floats with exponents in [-40, 40]
-    const uint32_t ma = nxt() & 0x7FFFFF, mb = nxt() & 0x7FFFFF;
-    const int ea = (int)(nxt() % 81) - 40 + 127, eb = (int)(nxt() % 81) - 40 + 127;
-    const float a = __uint_as_float(((uint32_t)ea << 23) | ma);
-    const float b = __uint_as_float(((uint32_t)eb << 23) | mb);
-    float ieee;
+    auto mk = [](int e, uint32_t m) { return __uint_as_float(((uint32_t)(e + 127) << 23) | (m & 0x7FFFFF)); };
+    float a, b, r, l;
+    if (seed == 0) {
+        b = mk((t >> 23) - 40, (uint32_t)t);
+        r = b;
+        a = 1.0f;
+        l = 1.0f;
+    } else {
+        a = mk((int)(nxt() % 81) - 40, nxt());
+        b = mk((int)(nxt() % 81) - 40, nxt());
+        r = mk((int)(nxt() % 41) - 20, nxt());
+        l = mk((int)(nxt() % 41) - 20, nxt());
+    }
+    float ieee, ieee12, ieee_rr, ieee_t12, p, ep, e2;
     {
 #pragma clang fp contract(off)
         ieee = a / b;
-    }
-    const float mine = div_by(a, recip64((double)b));
-    const float twelve = (float)(12.0 * recip64((double)b));
-    float ieee12;
-    {
-#pragma clang fp contract(off)
         ieee12 = 12.0f / b;
+        p = r * r;
+        ieee_rr = 1.0f / p;
+        ep = 2.0f * r / l;
+        e2 = ep * ep;
+        ieee_t12 = 12.0f / e2;
     }
-    if (__float_as_uint(ieee) != __float_as_uint(mine) || __float_as_uint(ieee12) != __float_as_uint(twelve))
+    const double yb = recip64((double)b);
+    const float mine = div_by(a, yb);
+    const float twelve = (float)(12.0 * yb);
+    const bool resid = fabs(fma(-(double)b, yb, 1.0)) <= 0x1p-51;
+    const double inv_r = recip64((double)r);
+    const float rr = (float)recip_refine((double)p, inv_r * inv_r);
+    const float ep2 = div_by(2.0f * r, 1.0 / (double)l);
+    const double q = 0.5 * (double)l * inv_r;
+    const float t12 = (float)(12.0 * recip_refine((double)e2, q * q));
+    if (__float_as_uint(ieee) != __float_as_uint(mine) ||
+        __float_as_uint(ieee12) != __float_as_uint(twelve) || !resid ||
+        __float_as_uint(ieee_rr) != __float_as_uint(rr) || __float_as_uint(ep) != __float_as_uint(ep2) ||
+        __float_as_uint(ieee_t12) != __float_as_uint(t12))
         atomicAdd(bad, 1ull);
 }
 

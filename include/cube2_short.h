@@ -289,8 +289,10 @@ int cube2_pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ra
  * n+2 floats, FFTW/cuFFT in-place layout): dir=-1 R2C, +1 C2R, unnormalised. */
 int cube2_fft3d_dev(float* d_data, int n, int nb, int dir);
 
-/* Checks the FP64-reciprocal quotient used by CUBE2_PP_REF against IEEE
- * float division on n seeded random operand pairs; *mismatches = count. */
+/* Checks the FP64-reciprocal quotients used by CUBE2_PP_REF (a/b, 12/b,
+ * 1/(r*r), 12/ep^2) against IEEE float division on n seeded random operand
+ * sets; seed 0 instead takes every float with exponent in [-40, 40]
+ * (n = 81 * 2^23).  *mismatches = count. */
 int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches);
 
 int cube2_profile_enable(int on);
