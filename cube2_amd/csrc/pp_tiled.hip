@@ -62,6 +62,23 @@ __device__ inline double recip64(double x) { return recip_refine(x, __builtin_am
 
 __device__ inline float div_by(float a, double y) { return (float)((double)a * y); }
 
+// RN(sqrt(x)) for normal positive finite x (pair distances: r^2 >= 2^-28,
+// the positions being multiples of ratio_cs * 2^-16 apart).  v_sqrt_f32 is
+// within one ulp; of s and its neighbours s-, s+ the rounding test keeps s-
+// if x <= s- * s and s+ if x > s * s+ (FMA residuals, exact in sign: the
+// residuals are integer multiples of ulp^2 and midpoints squared are not).
+// No scaling or special-value fix-ups, which only zero, infinities, NaNs and
+// tiny inputs need.  The self-test checks it against IEEE sqrtf for every
+// float with exponent in [-40, 40].
+__device__ inline float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __int_as_float(__float_as_int(s) - 1);
+    const float hi = __int_as_float(__float_as_int(s) + 1);
+    float r = fmaf(-lo, s, x) <= 0.0f ? lo : s;
+    r = fmaf(-hi, s, x) > 0.0f ? hi : r;
+    return r;
+}
+
 struct FraK {
     float a;        // softening diameter
     double inv_a;   // 1/a in FP64
@@ -116,7 +133,7 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
                                   float& fz) {
     if (kMode == kModeRef) {
 #pragma clang fp contract(off)
-        const float rm = sqrtf(e.w);
+        const float rm = sqrt_rn(e.w);
         const double inv_rm = recip64((double)rm);
         // RN(1/(r*r)): the reciprocal of RN(r*r) refined from (1/r)^2
         const float frr = (float)recip_refine((double)(rm * rm), inv_rm * inv_rm);
@@ -494,7 +511,8 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
 // r and the length l in [-20, 20]); seed == 0: every float with exponent in
 // [-40, 40] (n = 81 * 2^23) as b and r, with a = l = 1.  Checked: a/b,
 // 12/b, |1 - b*recip64(b)| <= 2^-51, 1/RN(r*r) refined from (1/r)^2, and
-// 12/e2 for ep = RN(2r/l), e2 = RN(ep*ep) refined from (l/2 * 1/r)^2.
+// 12/e2 for ep = RN(2r/l), e2 = RN(ep*ep) refined from (l/2 * 1/r)^2, and
+// sqrt_rn(b) against IEEE sqrtf.
 __global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -518,9 +536,10 @@ __global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
         r = mk((int)(nxt() % 41) - 20, nxt());
         l = mk((int)(nxt() % 41) - 20, nxt());
     }
-    float ieee, ieee12, ieee_rr, ieee_t12, p, ep, e2;
+    float ieee, ieee12, ieee_rr, ieee_t12, p, ep, e2, ieee_sqrt;
     {
 #pragma clang fp contract(off)
+        ieee_sqrt = sqrtf(b);
         ieee = a / b;
         ieee12 = 12.0f / b;
         p = r * r;
@@ -541,7 +560,8 @@ __global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
     if (__float_as_uint(ieee) != __float_as_uint(mine) ||
         __float_as_uint(ieee12) != __float_as_uint(twelve) || !resid ||
         __float_as_uint(ieee_rr) != __float_as_uint(rr) || __float_as_uint(ep) != __float_as_uint(ep2) ||
-        __float_as_uint(ieee_t12) != __float_as_uint(t12))
+        __float_as_uint(ieee_t12) != __float_as_uint(t12) ||
+        __float_as_uint(ieee_sqrt) != __float_as_uint(sqrt_rn(b)))
         atomicAdd(bad, 1ull);
 }
 

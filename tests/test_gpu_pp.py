@@ -227,9 +227,9 @@ def test_ref_mode_matches_reference_arithmetic(lib, oracle_lib, c0_layout_still,
 
 def test_reciprocal_division_is_ieee(lib):
     """The FP64-reciprocal quotients of the reference-arithmetic kernel equal
-    IEEE float division (a/b, 12/b, 1/(r*r), 12/ep^2) on 2^26 seeded random
-    operand sets, and on every float with exponent in [-40, 40] (the
-    reciprocal's residual within 2^-51 there)."""
+    IEEE float division (a/b, 12/b, 1/(r*r), 12/ep^2), and its square root
+    IEEE sqrtf, on 2^26 seeded random operand sets and on every float with
+    exponent in [-40, 40] (the reciprocal's residual within 2^-51 there)."""
     import ctypes as C
     bad = C.c_ulonglong(0)
     for seed in (1, 2):
