@@ -66,6 +66,8 @@ SIGNATURES = {
     "cube2_velocity_apply_dev": (C.c_int, [vp_, vp_, vp_, C.c_int64, vp_]),
     "cube2_pm2_force_dev": (C.c_int, [vp_] * 3 + [C.c_int64, vp_] + [C.c_float] * 3 + [vp_] * 2),
     "cube2_green_3d_dev": (C.c_int, [vp_, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "cube2_green_file_write_dev": (C.c_int, [vp_, C.c_int, C.c_char_p]),
+    "cube2_green_file_read_dev": (C.c_int, [vp_, C.c_int, C.c_char_p]),
     "cube2_select_levels": (C.c_int, [vp_] * 5),
     "cube2_layout_build_dev": (C.c_int, [vp_, vp_, C.c_int64] + [vp_] * 4 + [C.c_int64, vp_]),
     "cube2_drift_dev": (C.c_int, [vp_] * 4 + [C.c_int64, C.c_float, C.c_float] + [vp_] * 4 +

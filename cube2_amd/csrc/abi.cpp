@@ -166,6 +166,16 @@ int cube2_green_3d_dev(float* d_G, int n, float apm_soft, float apm_range, float
     return green_3d(d_G, n, apm_soft, apm_range, gridsize);
 }
 
+int cube2_green_file_write_dev(const float* d_G, int n, const char* path) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return green_file_write(d_G, n, path);
+}
+
+int cube2_green_file_read_dev(float* d_G, int n, const char* path) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return green_file_read(d_G, n, path);
+}
+
 int cube2_fft3d_dev(float* d_data, int n, int nb, int dir) {
     std::lock_guard<std::recursive_mutex> l(g_mx);
     return fft3d(d_data, n, nb, dir);

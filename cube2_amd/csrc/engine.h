@@ -73,6 +73,8 @@ int pm3_debug_job(int it, const int* ires, const int* ixyz3, const int* ratio_sf
 int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t n_used,
                    float* vmax);
 int green_3d(float* d_G, int n, float apm_soft, float apm_range, float gridsize);
+int green_file_write(const float* d_G, int n, const char* path);
+int green_file_read(float* d_G, int n, const char* path);
 int fft3d(float* d, int n, int nb, int dir);
 void pm3_release();
 int green_cache_get(const float* host, size_t count, const float** dev);

@@ -64,8 +64,24 @@ class KickResult:
                     dt_vmax=VBUF * 14 / vm if vm > 0 else math.inf)
 
 
+def green_file_name(kind: str, image: int = 1) -> str:
+    """Cache file name of a Green's table (initialize.f90:102-175): kind 'Gk2'
+    or 'Gk3_<ratio_sf>', then output_suffix() = '_<image>.bin'
+    (basic_functions.f08:76-81)."""
+    return f"{kind}_{int(image)}.bin"
+
+
 class ShortRangeKick:
-    """One image resident in HBM; each ``kick()`` runs PM3 then PP."""
+    """One image resident in HBM; each ``kick()`` runs PM3 then PP.
+
+    Green's tables are built on the device on first use.  With
+    ``use_green_cache(dir, read)`` they follow the reference's read_Gks switch
+    (initialize.f90:102-175): read from ``dir`` when ``read``, else built and
+    written there, one stream file per table."""
+
+    green_dir: str | None = None
+    read_greens: bool = False
+    image: int = 1
 
     def __init__(self, layout: Layout, device: str = "cuda:0", greens: dict | None = None):
         import torch
@@ -108,28 +124,43 @@ class ShortRangeKick:
         self.ires = np.ascontiguousarray(ires, np.int32)
         self.isort = np.argsort(-self.ires, kind="stable").astype(np.int32) + 1
 
+    def use_green_cache(self, directory: str | None, read: bool = False, image: int = 1) -> None:
+        """Tables not yet built are read from (read=True) or written to
+        ``directory`` (None: neither), as files named by green_file_name."""
+        self.green_dir, self.read_greens, self.image = directory, bool(read), int(image)
+
+    def _green_table(self, kind: str, n: int, soft: float, rng: float, gridsize: float):
+        import os
+        t = self.torch.empty((n, n, n // 2 + 1), dtype=self.torch.float32, device=self.dev)
+        path = None
+        if self.green_dir is not None:
+            path = os.path.join(self.green_dir, green_file_name(kind, self.image)).encode()
+        if path is not None and self.read_greens:
+            _lib.check(_lib.lib().cube2_green_file_read_dev(t.data_ptr(), n, path))
+            return t
+        _lib.check(_lib.lib().cube2_green_3d_dev(t.data_ptr(), n, soft, rng, gridsize))
+        if path is not None:
+            _lib.check(_lib.lib().cube2_green_file_write_dev(t.data_ptr(), n, path))
+        return t
+
     def green(self, level: int):
-        """Device Gk3 table of a level, built once with the FP64 Green kernel."""
+        """Device Gk3 table of a level (Gk3_<ratio_sf>, initialize.f90:142-168),
+        built once with the FP64 Green kernel (or read from the cache)."""
         if level not in self.greens:
-            g = self.geom
-            n = g.nft(level)
-            t = self.torch.empty((n, n, n // 2 + 1), dtype=self.torch.float32, device=self.dev)
-            _lib.check(_lib.lib().cube2_green_3d_dev(
-                t.data_ptr(), n, float(APM3F), float(APM3F * RATIO_SF[level - 1]),
-                float(np.float32(1.0) / np.float32(RATIO_SF[level - 1]))))
-            self.greens[level] = t
+            rsf = int(RATIO_SF[level - 1])
+            self.greens[level] = self._green_table(
+                f"Gk3_{rsf}", self.geom.nft(level), float(APM3F), float(APM3F * rsf),
+                float(np.float32(1.0) / np.float32(rsf)))
         return self.greens[level]
 
     def green2(self):
         """Device Gk2 table (initialize.f90:135): Green_3D(ngt, apm2 = 3.5,
-        apm1 = 3.5 ratio_cs, gridsize 1), built once with the FP64 kernel."""
+        apm1 = 3.5 ratio_cs, gridsize 1), built once with the FP64 kernel (or
+        read from the cache)."""
         if "pm2" not in self.greens:
             g = self.geom
-            n = g.ngt
-            t = self.torch.empty((n, n, n // 2 + 1), dtype=self.torch.float32, device=self.dev)
-            _lib.check(_lib.lib().cube2_green_3d_dev(t.data_ptr(), n, float(APM3F),
-                                                     float(APM3F * g.ratio_cs), 1.0))
-            self.greens["pm2"] = t
+            self.greens["pm2"] = self._green_table("Gk2", g.ngt, float(APM3F),
+                                                   float(APM3F * g.ratio_cs), 1.0)
         return self.greens["pm2"]
 
     def pm2(self, mass: float, a_mid: float, dt: float, res: KickResult) -> None:

@@ -167,6 +167,16 @@ int cube2_velocity_apply_dev(const int* d_rhoc, const int64_t* d_idx_b_r, float*
  * initialize.f90:142-168. */
 int cube2_green_3d_dev(float* d_G, int n, float apm_soft, float apm_range, float gridsize);
 
+/* Green's table cache files (initialize.f90:102-175, the read_Gks switch):
+ * Fortran access='stream' files of real(4) Gk(n/2+1,n,n), column-major, no
+ * record markers, i.e. the bytes of d_G[n][n][n/2+1]; the reference names
+ * them output_dir()//'Gk2' or 'Gk3_<ratio_sf>' //'_<image>.bin'
+ * (basic_functions.f08:76-81).  write: d_G to path, replacing the file.
+ * read: the first n*n*(n/2+1) floats of path into d_G; CUBE2_ERR_ARG if the
+ * file is missing or shorter (the Fortran read fails at end of file). */
+int cube2_green_file_write_dev(const float* d_G, int n, const char* path);
+int cube2_green_file_read_dev(float* d_G, int n, const char* path);
+
 /* Level selection and job order of kick.f90:107-131 (host arrays). */
 int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort, int* napm);
 
