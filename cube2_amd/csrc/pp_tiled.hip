@@ -368,10 +368,10 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                 }
                 wave_lds_sync();
                 if (vb + 64 < M) qcur = fetch(vb + 64);
-                for (int t = 0; t < nb; t += J) {
+                // (i, j) of slot t: the difference vector and whether it pairs
+                auto test = [&](int t, float4& e) {
                     const int jj = t + jslot;
                     bool ok = false;
-                    float4 e;
                     if (vi && jj < nb) {
 #pragma clang fp contract(off)
                         const float4 qj = L.jb[jj];
@@ -384,8 +384,14 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                             e.w = e.x * e.x + e.y * e.y + e.z * e.z;
                         ok = e.w > 0.0f && e.w < pp2;
                     }
-                    if (direct) {
-                        if (ok) {
+                    return ok;
+                };
+                // two loops on the chunk-uniform branch, so that the direct
+                // loop carries only its accumulators
+                if (direct) {
+                    for (int t = 0; t < nb; t += J) {
+                        float4 e;
+                        if (test(t, e)) {
 #pragma clang fp contract(off)
                             float fx, fy, fz;
                             pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
@@ -394,20 +400,24 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                             daz += (Acc)fz;
                             npa++;
                         }
-                        continue;
                     }
-                    const uint64_t m = __ballot(ok);
-                    if (m) {
-                        if (ok) {
-                            const int s = (qtail + __popcll(m & lt)) & (kQ - 1);
-                            L.q[s] = e;
-                            L.qi[s] = (unsigned char)islot;
-                        }
-                        qtail += __popcll(m);
-                        wave_lds_sync();
-                        if (qtail - qhead >= 64) {
-                            flush<kMode>(L, lane, qhead, 64, Ka, Kr);
-                            qhead += 64;
+                } else {
+                    for (int t = 0; t < nb; t += J) {
+                        float4 e;
+                        const bool ok = test(t, e);
+                        const uint64_t m = __ballot(ok);
+                        if (m) {
+                            if (ok) {
+                                const int s = (qtail + __popcll(m & lt)) & (kQ - 1);
+                                L.q[s] = e;
+                                L.qi[s] = (unsigned char)islot;
+                            }
+                            qtail += __popcll(m);
+                            wave_lds_sync();
+                            if (qtail - qhead >= 64) {
+                                flush<kMode>(L, lane, qhead, 64, Ka, Kr);
+                                qhead += 64;
+                            }
                         }
                     }
                 }
