@@ -39,7 +39,7 @@ int pp_tiled_grid(int mode);
 
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
-    DevBuf chunk_flags, chunks, ilist;
+    DevBuf chunk_flags, chunks, ilist, big;
     HostBuf h_res;  // PPRed, per-location counts, tile_first (pinned)
     std::vector<PPTile> h_tiles;
     int64_t c_tot = 0;
@@ -96,6 +96,37 @@ __device__ inline float decode_coarse(int rel, int16_t v) {
 
 constexpr int kWaves = 4;
 constexpr int kMaxSub = 12 * 12 * 12;
+// Coarse cells of more than kBigCell particles (C1's clumps put up to 115,201
+// in one) are counted and scattered by a block of kBigWaves wavefronts each,
+// not by one wavefront, whose serial loop over them outlasted the rest of
+// the launch (scatter 3.0 ms, of which 2.3 the largest cells' tail).
+constexpr int kBigCell = 2048;
+constexpr int kBigWaves = 8;
+constexpr int kBigBlocks = 512;  // grid of the big-cell kernels (grid-stride over the list)
+
+// PP sub-cell of a particle of coarse cell c, clamped into the cell's rcp^3
+// sub-cells (out: whether it had to be), and its decoded coarse coordinates.
+__device__ inline int pp_sub(const CellRef& c, int rcp, const int16_t* p, float& x0, float& x1,
+                             float& x2, bool& out) {
+    x0 = decode_coarse(c.ci - 1, p[0]);
+    x1 = decode_coarse(c.cj - 1, p[1]);
+    x2 = decode_coarse(c.ck - 1, p[2]);
+    int bx = pp_index(x0, rcp) - c.ci * rcp;
+    int by = pp_index(x1, rcp) - c.cj * rcp;
+    int bz = pp_index(x2, rcp) - c.ck * rcp;
+    out = (unsigned)bx >= (unsigned)rcp || (unsigned)by >= (unsigned)rcp ||
+          (unsigned)bz >= (unsigned)rcp;
+    bx = min(max(bx, 0), rcp - 1);
+    by = min(max(by, 0), rcp - 1);
+    bz = min(max(bz, 0), rcp - 1);
+    return (bz * rcp + by) * rcp + bx;
+}
+
+// Tile-local PP cell key of sub-cell b of coarse cell c.
+__device__ inline int64_t pp_key(const CellRef& c, int rcp, int S, int b) {
+    const int bx = b % rcp, by = (b / rcp) % rcp, bz = b / (rcp * rcp);
+    return ((int64_t)(c.ck * rcp + bz) * S + (c.cj * rcp + by)) * S + (c.ci * rcp + bx);
+}
 
 // One wavefront per coarse cell of each tile's extended region: histogram
 // of its particles over its rcp^3 PP cells; writes every one of those
@@ -106,7 +137,9 @@ __global__ __launch_bounds__(256) void k_pp_count(const int* __restrict__ rhoc,
                                                   const PPTile* __restrict__ tiles, int nt,
                                                   int ncb, int nte, int64_t ncells, int64_t n_used,
                                                   int* __restrict__ counts,
-                                                  int* __restrict__ err) {
+                                                  int* __restrict__ err,
+                                                  int* __restrict__ big_list,
+                                                  int* __restrict__ nbig) {
     __shared__ int bins[kWaves][kMaxSub];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * kWaves + wave;
@@ -124,6 +157,7 @@ __global__ __launch_bounds__(256) void k_pp_count(const int* __restrict__ rhoc,
         for (int b = lane; b < nb; b += 64) bins[wave][b] = 0;
     }
     __syncthreads();
+    bool big = false;
     if (ok) {
         int np = rhoc[c.cell];
         const int64_t base = nzero[c.cell];
@@ -131,36 +165,116 @@ __global__ __launch_bounds__(256) void k_pp_count(const int* __restrict__ rhoc,
             if (lane == 0) atomicOr(err, 2);
             np = 0;
         }
-        const int ri = c.ci - 1, rj = c.cj - 1, rk = c.ck - 1;
+        big = np > kBigCell;
+        if (big) {
+            if (lane == 0) big_list[atomicAdd(nbig, 1)] = (int)w;
+            np = 0;
+        }
         for (int l = lane; l < np; l += 64) {
-            const int16_t* p = xp + 3 * (base + l);
-            int bx = pp_index(decode_coarse(ri, p[0]), rcp) - c.ci * rcp;
-            int by = pp_index(decode_coarse(rj, p[1]), rcp) - c.cj * rcp;
-            int bz = pp_index(decode_coarse(rk, p[2]), rcp) - c.ck * rcp;
-            if ((unsigned)bx >= (unsigned)rcp || (unsigned)by >= (unsigned)rcp ||
-                (unsigned)bz >= (unsigned)rcp) {
-                atomicOr(err, 1);
-                bx = min(max(bx, 0), rcp - 1);
-                by = min(max(by, 0), rcp - 1);
-                bz = min(max(bz, 0), rcp - 1);
-            }
-            atomicAdd(&bins[wave][(bz * rcp + by) * rcp + bx], 1);
+            float x0, x1, x2;
+            bool out;
+            const int b = pp_sub(c, rcp, xp + 3 * (base + l), x0, x1, x2, out);
+            if (out) atomicOr(err, 1);
+            atomicAdd(&bins[wave][b], 1);
         }
     }
     __syncthreads();
-    if (ok) {
-        for (int b = lane; b < nb; b += 64) {
-            int bx = b % rcp, by = (b / rcp) % rcp, bz = b / (rcp * rcp);
-            int64_t key = ((int64_t)(c.ck * rcp + bz) * S + (c.cj * rcp + by)) * S + (c.ci * rcp + bx);
-            counts[off + key] = bins[wave][b];
+    if (ok && !big) {
+        for (int b = lane; b < nb; b += 64) counts[off + pp_key(c, rcp, S, b)] = bins[wave][b];
+    }
+}
+
+// The listed coarse cells, one block each: the same histogram.
+__global__ __launch_bounds__(1024) void k_pp_count_big(const int* __restrict__ rhoc,
+                                                       const int64_t* __restrict__ nzero,
+                                                       const int16_t* __restrict__ xp,
+                                                       const PPTile* __restrict__ tiles, int nt,
+                                                       int ncb, int nte,
+                                                       const int* __restrict__ big_list,
+                                                       const int* __restrict__ nbig,
+                                                       int* __restrict__ counts,
+                                                       int* __restrict__ err) {
+    __shared__ int bins[kMaxSub];
+    const int nbg = *nbig;
+    for (int i = blockIdx.x; i < nbg; i += gridDim.x) {
+        const CellRef c = coarse_cell(big_list[i], nt, ncb, nte);
+        const PPTile T = tiles[c.t];
+        const int rcp = T.rcp, nb = rcp * rcp * rcp;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
+        __syncthreads();
+        const int np = rhoc[c.cell];  // listed cells passed the consistency check
+        const int64_t base = nzero[c.cell];
+        for (int l = threadIdx.x; l < np; l += blockDim.x) {
+            float x0, x1, x2;
+            bool out;
+            const int b = pp_sub(c, rcp, xp + 3 * (base + l), x0, x1, x2, out);
+            if (out) atomicOr(err, 1);
+            atomicAdd(&bins[b], 1);
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+            counts[T.cell_off + pp_key(c, rcp, T.S, b)] = bins[b];
+        __syncthreads();  // bins are cleared for the next cell
+    }
+}
+
+// Range of the physical particles of coarse cell c (their subtile's level),
+// 0 for halo copies.
+__device__ inline float cell_range(const CellRef& c, const int* __restrict__ ires_loc,
+                                   const float* __restrict__ apm3, int nt, int nnt, int nns,
+                                   int ntt) {
+    const bool phys = c.ci >= 1 && c.ci <= nt && c.cj >= 1 && c.cj <= nt && c.ck >= 1 && c.ck <= nt;
+    if (!phys) return 0.0f;
+    const int tx = c.t % nnt, ty = (c.t / nnt) % nnt, tz = c.t / (nnt * nnt);
+    const int sx = (c.ci - 1) / ntt, sy = (c.cj - 1) / ntt, sz = (c.ck - 1) / ntt;
+    const int loc = (((tz * nnt + ty) * nnt + tx) * nns + sz) * nns * nns + sy * nns + sx;
+    return apm3[ires_loc[loc] - 1];
+}
+
+// Scatters particles l_begin .. l_end-1 of coarse cell c (one wavefront, in
+// slot order) to cell_start[key] + cur[sub-cell] + (rank among the earlier
+// particles of its PP cell in this run), advancing cur.  Ranks come from
+// comparing every lane's sub-cell (scalar broadcasts, no dependency chain);
+// the last lane of each sub-cell advances its cursor.
+__device__ inline void scatter_run(const CellRef& c, int rcp, int S, int64_t off,
+                                   const int16_t* __restrict__ xp, int64_t base, int l_begin,
+                                   int l_end, float range, int ratio_cs, int* cur,
+                                   const int* __restrict__ cell_start, float4* __restrict__ pos4,
+                                   uint32_t* __restrict__ slot, uint32_t* __restrict__ cellid,
+                                   int lane) {
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int l0 = l_begin; l0 < l_end; l0 += 64) {
+        const int l = l0 + lane;
+        const bool valid = l < l_end;
+        int b = -1;
+        float x0 = 0, x1 = 0, x2 = 0;
+        if (valid) {
+            bool out;
+            b = pp_sub(c, rcp, xp + 3 * (base + l), x0, x1, x2, out);
+        }
+        uint64_t same = 0;
+#pragma unroll
+        for (int k = 0; k < 64; k++)
+            same |= (uint64_t)(__builtin_amdgcn_readlane(b, k) == b) << k;
+        int mypos = 0;
+        if (valid) mypos = cur[b] + __popcll(same & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && !(same & ~lt & ~(1ull << lane))) cur[b] += __popcll(same & lt) + 1;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            const int64_t key = pp_key(c, rcp, S, b);
+            const int dst = cell_start[off + key] + mypos;
+            pos4[dst] = make_float4((float)ratio_cs * x0, (float)ratio_cs * x1,
+                                    (float)ratio_cs * x2, range);
+            slot[dst] = (uint32_t)(base + l);
+            cellid[dst] = (uint32_t)(off + key);
         }
     }
 }
 
 // Same traversal; scatters each particle to cell_start[key] + (rank among
 // the earlier particles of its PP cell), preserving slot order within a
-// cell.  Ranks come from ballots over the wavefront and an LDS cursor per
-// PP cell.
+// cell.  Cells of more than kBigCell particles are left to k_pp_scatter_big.
 __global__ __launch_bounds__(256) void k_pp_scatter(
     const int* __restrict__ rhoc, const int64_t* __restrict__ nzero, const int16_t* __restrict__ xp,
     const PPTile* __restrict__ tiles, const int* __restrict__ ires_loc,
@@ -189,55 +303,56 @@ __global__ __launch_bounds__(256) void k_pp_scatter(
     if (!ok) return;  // no block-level barrier below
     const int np = rhoc[c.cell];
     const int64_t base = nzero[c.cell];
-    if (np <= 0 || base < 0 || base + np > n_used) return;  // as k_pp_count: counted as empty
-    const bool phys = c.ci >= 1 && c.ci <= nt && c.cj >= 1 && c.cj <= nt && c.ck >= 1 && c.ck <= nt;
-    float range = 0.0f;
-    if (phys) {
-        const int tx = c.t % nnt, ty = (c.t / nnt) % nnt, tz = c.t / (nnt * nnt);
-        const int sx = (c.ci - 1) / ntt, sy = (c.cj - 1) / ntt, sz = (c.ck - 1) / ntt;
-        const int loc = (((tz * nnt + ty) * nnt + tx) * nns + sz) * nns * nns + sy * nns + sx;
-        range = apm3[ires_loc[loc] - 1];
-    }
-    const int ri = c.ci - 1, rj = c.cj - 1, rk = c.ck - 1;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int l0 = 0; l0 < np; l0 += 64) {
-        const int l = l0 + lane;
-        const bool valid = l < np;
-        int b = -1;
-        float x0 = 0, x1 = 0, x2 = 0;
-        int64_t key = 0;
-        if (valid) {
-            const int16_t* p = xp + 3 * (base + l);
-            x0 = decode_coarse(ri, p[0]);
-            x1 = decode_coarse(rj, p[1]);
-            x2 = decode_coarse(rk, p[2]);
-            int i0 = pp_index(x0, rcp), i1 = pp_index(x1, rcp), i2 = pp_index(x2, rcp);
-            int bx = min(max(i0 - c.ci * rcp, 0), rcp - 1);
-            int by = min(max(i1 - c.cj * rcp, 0), rcp - 1);
-            int bz = min(max(i2 - c.ck * rcp, 0), rcp - 1);
-            b = (bz * rcp + by) * rcp + bx;
-            key = ((int64_t)(c.ck * rcp + bz) * S + (c.cj * rcp + by)) * S + (c.ci * rcp + bx);
-        }
-        // rank among the earlier lanes with the same PP cell: compare with
-        // every lane's cell (scalar broadcasts, no dependency chain), then the
-        // last lane of each cell advances that cell's cursor
-        uint64_t same = 0;
-#pragma unroll
-        for (int k = 0; k < 64; k++)
-            same |= (uint64_t)(__builtin_amdgcn_readlane(b, k) == b) << k;
-        int mypos = 0;
-        if (valid) mypos = cur[wave][b] + __popcll(same & lt);
+    if (np <= 0 || np > kBigCell || base < 0 || base + np > n_used) return;
+    const float range = cell_range(c, ires_loc, apm3, nt, nnt, nns, ntt);
+    scatter_run(c, rcp, S, off, xp, base, 0, np, range, ratio_cs, cur[wave], cell_start, pos4,
+                slot, cellid, lane);
+}
+
+// The listed coarse cells, one block each: wavefront s takes the s-th
+// contiguous segment of the cell's particles; per-segment histograms and
+// their exclusive sums over the segments give each wavefront's starting
+// cursors, so the order is the one-wavefront order exactly.
+__global__ __launch_bounds__(64 * kBigWaves) void k_pp_scatter_big(
+    const int* __restrict__ rhoc, const int64_t* __restrict__ nzero, const int16_t* __restrict__ xp,
+    const PPTile* __restrict__ tiles, const int* __restrict__ ires_loc,
+    const float* __restrict__ apm3, int nt, int ncb, int nte, int nnt, int nns, int ntt,
+    int ratio_cs, const int* __restrict__ big_list, const int* __restrict__ nbig,
+    const int* __restrict__ cell_start, const int* __restrict__ err, float4* __restrict__ pos4,
+    uint32_t* __restrict__ slot, uint32_t* __restrict__ cellid) {
+    if (*err & 4) return;
+    __shared__ int hist[kBigWaves][kMaxSub];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nbg = *nbig;
+    for (int i = blockIdx.x; i < nbg; i += gridDim.x) {
+        const CellRef c = coarse_cell(big_list[i], nt, ncb, nte);
+        const PPTile T = tiles[c.t];
+        const int rcp = T.rcp, nb = rcp * rcp * rcp;
+        const int np = rhoc[c.cell];
+        const int64_t base = nzero[c.cell];
+        const int seg = (np + 64 * kBigWaves - 1) / (64 * kBigWaves) * 64;
+        const int l0 = min(np, wave * seg), l1 = min(np, l0 + seg);
+        for (int b = lane; b < nb; b += 64) hist[wave][b] = 0;
         __builtin_amdgcn_wave_barrier();
-        if (valid && !(same & ~lt & ~(1ull << lane)))
-            cur[wave][b] += __popcll(same & lt) + 1;
-        __builtin_amdgcn_wave_barrier();
-        if (valid) {
-            const int dst = cell_start[off + key] + mypos;
-            pos4[dst] = make_float4((float)ratio_cs * x0, (float)ratio_cs * x1,
-                                    (float)ratio_cs * x2, range);
-            slot[dst] = (uint32_t)(base + l);
-            cellid[dst] = (uint32_t)(off + key);
+        for (int l = l0 + lane; l < l1; l += 64) {
+            float x0, x1, x2;
+            bool out;
+            atomicAdd(&hist[wave][pp_sub(c, rcp, xp + 3 * (base + l), x0, x1, x2, out)], 1);
         }
+        __syncthreads();
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            int run = 0;
+            for (int s = 0; s < kBigWaves; s++) {
+                const int t = hist[s][b];
+                hist[s][b] = run;
+                run += t;
+            }
+        }
+        __syncthreads();
+        const float range = cell_range(c, ires_loc, apm3, nt, nnt, nns, ntt);
+        scatter_run(c, rcp, T.S, T.cell_off, xp, base, l0, l1, range, ratio_cs, hist[wave],
+                    cell_start, pos4, slot, cellid, lane);
+        __syncthreads();  // hist is rewritten for the next cell
     }
 }
 
@@ -490,6 +605,7 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.chunk_flags.reserve((P.cap + 1) * sizeof(int)));
     C2_TRY(g_pp.chunks.reserve((P.cap + 1) * sizeof(int)));
     C2_TRY(g_pp.ilist.reserve(P.cap * sizeof(int)));
+    C2_TRY(g_pp.big.reserve(std::max<int64_t>(P.ncells_ext, 1) * sizeof(int)));
     C2_TRY(g_pp.scan_tmp.reserve(scan_tmp_ints(std::max<int64_t>(P.c_tot, (int64_t)P.cap)) * sizeof(int)));
     C2_TRY(nzero_reserve());
     C2_TRY(g_pp.h_res.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long) +
@@ -518,7 +634,11 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
         Stage st("pp_count", (double)P.n_used * 6 + P.c_tot * 4.0);
         k_pp_count<<<ceil_div(P.ncells_ext, kWaves), 256, 0, s>>>(
             d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g.nt, g.ncb, g.nte, P.ncells_ext,
-            P.n_used, counts, &red->err);
+            P.n_used, counts, &red->err, g_pp.big.as<int>(), &red->nbig);
+        C2_HIP(hipGetLastError());
+        k_pp_count_big<<<kBigBlocks, 1024, 0, s>>>(d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g.nt,
+                                                   g.ncb, g.nte, g_pp.big.as<int>(), &red->nbig,
+                                                   counts, &red->err);
         C2_HIP(hipGetLastError());
     }
     {
@@ -533,6 +653,12 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g_pp.ires_loc.as<int>(),
             g_pp.apm3.as<float>(), g.nt, g.ncb, g.nte, g.p.nnt, g.p.nns, g.ntt, g.p.ratio_cs,
             P.ncells_ext, P.n_used, counts, &red->err, g_pp.pos4.as<float4>(),
+            g_pp.slot.as<uint32_t>(), g_pp.cellid.as<uint32_t>());
+        C2_HIP(hipGetLastError());
+        k_pp_scatter_big<<<kBigBlocks, 64 * kBigWaves, 0, s>>>(
+            d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g_pp.ires_loc.as<int>(),
+            g_pp.apm3.as<float>(), g.nt, g.ncb, g.nte, g.p.nnt, g.p.nns, g.ntt, g.p.ratio_cs,
+            g_pp.big.as<int>(), &red->nbig, counts, &red->err, g_pp.pos4.as<float4>(),
             g_pp.slot.as<uint32_t>(), g_pp.cellid.as<uint32_t>());
         C2_HIP(hipGetLastError());
     }
@@ -702,7 +828,7 @@ void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
                       &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks,
-                      &p.ilist})
+                      &p.ilist, &p.big})
         b->release();
     p.valid = false;
 }

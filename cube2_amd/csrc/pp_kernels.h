@@ -15,7 +15,7 @@ struct PPRed {
     unsigned f2max_bits;               // max |a|^2 (non-negative float bits)
     unsigned vmax_bits[3];             // max |vp| per component
     int err;
-    int pad;
+    int nbig;  // coarse cells listed for the block-per-cell count and scatter
 };
 
 struct PPArgs {

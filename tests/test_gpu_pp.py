@@ -143,6 +143,37 @@ def test_mixed_levels_exact_and_cell_lists(lib, oracle_lib, mixed_layout):
         assert np.array_equal(got["cell_start"], want["cell_start"]), it
 
 
+def test_dense_coarse_cells_cell_lists_and_exact_sums(lib, oracle_lib):
+    """Coarse cells of more than 2048 particles (counted and scattered by a
+    block of wavefronts, one contiguous segment each) give the oracle's cell
+    lists and exact-order velocities bit for bit; one dense clump sits on a
+    tile face, so its halo copies form dense cells too."""
+    from cube2_amd.geometry import APM3, RATIO_SF, Geometry, ixyz3
+    from cube2_amd.layout import build_layout
+    g = Geometry(ng=64, nnt=2, nns=2, ratio_cs=4, ngb=8, app=0.05)
+    rng = np.random.default_rng(23)
+    pos = np.concatenate([rng.uniform(0, 64, (20000, 3)),
+                          np.array([34.0, 34.0, 34.0]) + rng.normal(0, 0.3, (20000, 3)),
+                          np.array([32.3, 10.0, 50.0]) + rng.normal(0, 0.4, (9000, 3))]) % 64
+    lay = build_layout(pos, rng.normal(0, 1, pos.shape).astype(np.float32), g)
+    print(f"densest coarse cell: {int(lay.rhoc.max())} particles; "
+          f"{int((lay.rhoc > 2048).sum())} cells above 2048", flush=True)
+    assert lay.rhoc.max() > 8 * 2048 and (lay.rhoc > 2048).sum() >= 3
+    ires = rng.integers(2, 7, g.ns3).astype(np.int32)
+    k, res = _dev_pp(lib, lay, ires=ires, mode=lib.PP_EXACT)
+    o = _oracle_pp(oracle_lib, lay, ires)
+    assert np.array_equal(k.vp_host().view(np.uint32), o["vp"].view(np.uint32))
+    assert (res.p_cand, res.p_acc) == (o["p_cand"], o["p_acc"])
+    for it in range(g.ntile):
+        got = k.celllist(it)
+        want = oracle_lib.pp_tile_celllist(g, ires, ixyz3(g), APM3, RATIO_SF, lay.rhoc,
+                                           lay.idx_b_r, lay.xp, it)
+        assert np.array_equal(got["keys"], want["keys"]), it
+        assert np.array_equal(got["slot"], want["slot"]), it
+        assert np.array_equal(got["cell_start"], want["cell_start"]), it
+    lib.set_pp_mode(lib.PP_REF)
+
+
 def _rel_rms(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-300)))
 
