@@ -39,7 +39,7 @@ int pp_tiled_grid(int mode);
 
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
-    DevBuf chunk_flags, chunks, ilist, big;
+    DevBuf chunk_flags, chunks, ilist, big, dense;
     HostBuf h_res;  // PPRed, per-location counts, tile_first (pinned)
     std::vector<PPTile> h_tiles;
     int64_t c_tot = 0;
@@ -380,7 +380,8 @@ __global__ void k_pp_ptotal(const int* __restrict__ rhoc, int nt, int ncb, int n
 __global__ void k_chunk_flags(const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
                               const int* __restrict__ cell_start, const int* __restrict__ n_sorted,
                               const int* __restrict__ err, int cap, int dense_lg,
-                              int* __restrict__ flags) {
+                              int* __restrict__ flags, int* __restrict__ dense_list,
+                              int* __restrict__ ndense) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= cap) return;
     const int n = (*err & 4) ? 0 : *n_sorted;
@@ -388,8 +389,10 @@ __global__ void k_chunk_flags(const float4* __restrict__ pos4, const uint32_t* _
     if (a < n && pos4[a].w > 0.0f) {
         const uint32_t c = cellid[a];
         const int c0 = cell_start[c];
-        const int mask = cell_start[c + 1] - c0 >= 64 ? (1 << dense_lg) - 1 : 63;
+        const bool dense = cell_start[c + 1] - c0 >= 64;
+        const int mask = dense ? (1 << dense_lg) - 1 : 63;
         f = ((a - c0) & mask) == 0;
+        if (dense && a == c0) dense_list[atomicAdd(ndense, 1)] = (int)c;
     }
     flags[a] = f;
 }
@@ -403,25 +406,22 @@ __global__ void k_chunk_emit(const int* __restrict__ scanned, int n, int* __rest
 // Spreads the low 3 bits of v to bits 0, 3, 6 (Morton interleave).
 __device__ inline int spread3(int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4); }
 
-// One wavefront per PP cell of at least 64 particles (found as its first
-// chunk): ilist[cell range] = the cell's sorted positions ordered by an
-// 8x8x8 Morton sub-cell key (counting sort in LDS), so that consecutive
-// chunks of a dense cell are spatially compact.  The order of
-// the cell list itself (the j stream) is not changed.
+// One wavefront per PP cell of at least 64 particles (listed by
+// k_chunk_flags): ilist[cell range] = the cell's sorted positions ordered by
+// an 8x8x8 Morton sub-cell key (counting sort in LDS), so that consecutive
+// chunks of a dense cell are spatially compact.  The order of the cell list
+// itself (the j stream) is not changed.
 __global__ __launch_bounds__(256) void k_pp_subsort(
-    const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
-    const int* __restrict__ cell_start, const PPTile* __restrict__ tiles, int ntile,
-    const int* __restrict__ chunks, const int* __restrict__ nchunks_dev, int ratio_cs,
-    int* __restrict__ ilist) {
+    const float4* __restrict__ pos4, const int* __restrict__ cell_start,
+    const PPTile* __restrict__ tiles, int ntile, const int* __restrict__ dense_list,
+    const int* __restrict__ ndense_dev, int ratio_cs, int* __restrict__ ilist) {
     __shared__ int bins[kWaves][512];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nchunks = *nchunks_dev;
-    // grid-stride over chunks; no block-level barrier in this kernel
-    for (int ck = blockIdx.x * kWaves + wave; ck < nchunks; ck += gridDim.x * kWaves) {
-    const int a0 = chunks[ck];
-    const uint32_t cid = cellid[a0];
+    const int ndense = *ndense_dev;
+    // grid-stride over the listed cells; no block-level barrier in this kernel
+    for (int qc = blockIdx.x * kWaves + wave; qc < ndense; qc += gridDim.x * kWaves) {
+    const uint32_t cid = (uint32_t)dense_list[qc];
     const int c0 = cell_start[cid], c1 = cell_start[cid + 1];
-    if (a0 != c0 || c1 - c0 < 64) continue;
     int lo = 0, hi = ntile - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -606,6 +606,7 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.chunks.reserve((P.cap + 1) * sizeof(int)));
     C2_TRY(g_pp.ilist.reserve(P.cap * sizeof(int)));
     C2_TRY(g_pp.big.reserve(std::max<int64_t>(P.ncells_ext, 1) * sizeof(int)));
+    C2_TRY(g_pp.dense.reserve((P.cap / 64 + 1) * sizeof(int)));
     C2_TRY(g_pp.scan_tmp.reserve(scan_tmp_ints(std::max<int64_t>(P.c_tot, (int64_t)P.cap)) * sizeof(int)));
     C2_TRY(nzero_reserve());
     C2_TRY(g_pp.h_res.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long) +
@@ -697,7 +698,8 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             Stage st("pp_chunks", (double)P.n_used * 24);
             k_chunk_flags<<<ceil_div(P.cap, 256), 256, 0, s>>>(A.pos4, A.cellid, A.cell_start,
                                                                 n_sorted, &red->err, (int)P.cap,
-                                                                pp_dense_lg(), flags);
+                                                                pp_dense_lg(), flags,
+                                                                g_pp.dense.as<int>(), &red->ndense);
             C2_HIP(hipGetLastError());
             C2_TRY(exclusive_scan(flags, (int64_t)P.cap, g_pp.scan_tmp, s));
             k_chunk_emit<<<ceil_div(P.cap, 256), 256, 0, s>>>(flags, (int)P.cap, g_pp.chunks.as<int>());
@@ -707,8 +709,8 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             Stage st("pp_subsort", (double)P.n_used * 24);
             // one wavefront per chunk (only first chunks of dense cells work)
             k_pp_subsort<<<(unsigned)std::max(1, P.grid_pairs), 256, 0, s>>>(
-                A.pos4, A.cellid, A.cell_start, A.tiles, P.ntile, g_pp.chunks.as<int>(),
-                flags + P.cap, g.p.ratio_cs, g_pp.ilist.as<int>());
+                A.pos4, A.cell_start, A.tiles, P.ntile, g_pp.dense.as<int>(), &red->ndense,
+                g.p.ratio_cs, g_pp.ilist.as<int>());
             C2_HIP(hipGetLastError());
         }
         C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), flags + P.cap, P.grid_pairs, P.mode, s));
@@ -828,7 +830,7 @@ void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
                       &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks,
-                      &p.ilist, &p.big})
+                      &p.ilist, &p.big, &p.dense})
         b->release();
     p.valid = false;
 }

@@ -15,7 +15,9 @@ struct PPRed {
     unsigned f2max_bits;               // max |a|^2 (non-negative float bits)
     unsigned vmax_bits[3];             // max |vp| per component
     int err;
-    int nbig;  // coarse cells listed for the block-per-cell count and scatter
+    int nbig;    // coarse cells listed for the block-per-cell count and scatter
+    int ndense;  // PP cells of >= 64 particles listed for the sub-cell sort
+    int pad;
 };
 
 struct PPArgs {
