@@ -230,9 +230,10 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
 // Persistent: wavefront w of the grid takes chunks w, w + W, w + 2W, ...
 // (W = waves in the grid); the chunk count is read on the device.  Chunk-
 // level values are kept wavefront-uniform (scalar registers), and the
-// register budget is capped for 6 wavefronts per SIMD: the candidate loop is
-// latency-bound, and occupancy 6 measured faster than 4 (REF at C1: 113 vs
-// 155 ms) despite a small spill in the REF variant.
+// register budget is capped for 6 wavefronts per SIMD (REF at 256^3: 80.7
+// ms; budgets for 5 and 7 wavefronts 89.2 and 86.0 ms; 4 blocks per CU
+// instead of 6: 105 ms).  The REF variant spills a few registers outside its
+// inner loops.
 template <int kMode>
 __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
                                                   const int* __restrict__ nchunks_dev,
