@@ -68,6 +68,16 @@ SIGNATURES = {
     "cube2_green_3d_dev": (C.c_int, [vp_, C.c_int, C.c_float, C.c_float, C.c_float]),
     "cube2_green_file_write_dev": (C.c_int, [vp_, C.c_int, C.c_char_p]),
     "cube2_green_file_read_dev": (C.c_int, [vp_, C.c_int, C.c_char_p]),
+    "cube2_gemm_bf16_dev": (C.c_int, [vp_, C.c_int64, C.c_int64, vp_, C.c_int64, C.c_int64, vp_,
+                                      C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cube2_cnn_im2col_dev": (C.c_int, [vp_, vp_, C.c_int, C.c_int, C.c_int]),
+    "cube2_cnn_col2im_dev": (C.c_int, [vp_, vp_, C.c_int, C.c_int, C.c_int]),
+    "cube2_cnn_bias_act_dev": (C.c_int, [vp_, vp_, C.c_int64, C.c_int, C.c_int]),
+    "cube2_cnn_act_bwd_dev": (C.c_int, [vp_, vp_, vp_, vp_, C.c_int64, C.c_int, C.c_int]),
+    "cube2_to_bf16_dev": (C.c_int, [vp_, vp_, C.c_int64]),
+    "cube2_cnn_adam_dev": (C.c_int, [vp_, vp_, vp_, vp_, vp_, C.c_int64, C.c_float, C.c_float,
+                                     C.c_float, C.c_float, C.c_int]),
+    "cube2_cnn_mse_dev": (C.c_int, [vp_, vp_, vp_, vp_, C.c_int64]),
     "cube2_select_levels": (C.c_int, [vp_] * 5),
     "cube2_layout_build_dev": (C.c_int, [vp_, vp_, C.c_int64] + [vp_] * 4 + [C.c_int64, vp_]),
     "cube2_drift_dev": (C.c_int, [vp_] * 4 + [C.c_int64, C.c_float, C.c_float] + [vp_] * 4 +

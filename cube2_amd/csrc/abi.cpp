@@ -47,6 +47,15 @@ static int stage_inputs(const int* rhoc, const int64_t* idx_b_r, const int16_t* 
     std::exit(EXIT_FAILURE);
 }
 
+// Learned PP stand-in: each operation, then a wait for the library stream.
+template <class F>
+static int cnn_call(F f) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    C2_TRY(f());
+    C2_HIP(hipStreamSynchronize(ctx().stream));
+    return CUBE2_OK;
+}
+
 extern "C" {
 
 void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* ratio_sf, int* rhoc,
@@ -174,6 +183,35 @@ int cube2_green_file_write_dev(const float* d_G, int n, const char* path) {
 int cube2_green_file_read_dev(float* d_G, int n, const char* path) {
     std::lock_guard<std::recursive_mutex> l(g_mx);
     return green_file_read(d_G, n, path);
+}
+
+int cube2_gemm_bf16_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+                        int64_t sbk, int64_t sbn, float* C, int64_t ldc, int M, int N, int K,
+                        int acc) {
+    return cnn_call([&] { return cnn_gemm(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc); });
+}
+int cube2_cnn_im2col_dev(const float* X, uint16_t* col, int B, int D, int Cin) {
+    return cnn_call([&] { return cnn_im2col(X, col, B, D, Cin); });
+}
+int cube2_cnn_col2im_dev(const float* dcol, float* dX, int B, int D, int Cin) {
+    return cnn_call([&] { return cnn_col2im(dcol, dX, B, D, Cin); });
+}
+int cube2_cnn_bias_act_dev(float* Y, const float* bias, int64_t rows, int C, int relu) {
+    return cnn_call([&] { return cnn_bias_act(Y, bias, rows, C, relu); });
+}
+int cube2_cnn_act_bwd_dev(const float* Y, float* dY, uint16_t* dYb, float* dbias, int64_t rows,
+                          int C, int relu) {
+    return cnn_call([&] { return cnn_act_bwd(Y, dY, dYb, dbias, rows, C, relu); });
+}
+int cube2_to_bf16_dev(const float* x, uint16_t* y, int64_t n) {
+    return cnn_call([&] { return cnn_to_bf16(x, y, n); });
+}
+int cube2_cnn_adam_dev(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n,
+                       float lr, float b1, float b2, float eps, int t) {
+    return cnn_call([&] { return cnn_adam(w, g, m, v, wb, n, lr, b1, b2, eps, t); });
+}
+int cube2_cnn_mse_dev(const float* out, const float* target, float* dout, float* loss, int64_t n) {
+    return cnn_call([&] { return cnn_mse(out, target, dout, loss, n); });
 }
 
 int cube2_fft3d_dev(float* d_data, int n, int nb, int dir) {

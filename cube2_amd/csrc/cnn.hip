@@ -1,0 +1,277 @@
+// Kernels of the learned PP stand-in: a 3-D CNN of 3^3 valid convolutions
+// in channels-last layout (activation [patch][z][y][x][channel], FP32),
+// each layer an im2col (BF16) and a BF16 GEMM on the matrix cores with FP32
+// accumulation; backward by the same GEMM on transposed operands.
+//
+// GEMM: C[m][n] (+)= sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak],
+// B(k,n) = B[k*sbk + n*sbn], C row-major with leading dimension ldc.  One
+// wavefront owns a 32 x 32 tile of C as 2 x 2 v_mfma_f32_16x16x16_bf16
+// tiles; operand lanes: A row m0 + (lane % 16), K values 4 (lane / 16) +
+// 0..3; B column n0 + (lane % 16), the same K values; result lane holds rows
+// 4 (lane / 16) + 0..3 of column lane % 16.  Out-of-range operands are
+// zeros, so any M, N, K work.
+#include "common.h"
+#include "engine.h"
+
+namespace cube2 {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ inline uint16_t to_bf16(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t* __restrict__ A, int64_t sam,
+                                                   int64_t sak, const uint16_t* __restrict__ B,
+                                                   int64_t sbk, int64_t sbn, float* __restrict__ C,
+                                                   int64_t ldc, int M, int N, int K, int acc) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m0 = blockIdx.y * 64 + (wave >> 1) * 32, n0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int r16 = lane & 15, kq = (lane >> 4) * 4;
+    v4f d[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) d[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        v4s a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int m = m0 + i * 16 + r16, n = n0 + i * 16 + r16;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k = k0 + kq + q;
+                a[i][q] = (m < M && k < K) ? (short)A[(int64_t)m * sam + (int64_t)k * sak] : (short)0;
+                b[i][q] = (n < N && k < K) ? (short)B[(int64_t)k * sbk + (int64_t)n * sbn] : (short)0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                d[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], d[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int m = m0 + i * 16 + kq + q, n = n0 + j * 16 + r16;
+                if (m < M && n < N) {
+                    float* c = C + (int64_t)m * ldc + n;
+                    *c = acc ? *c + d[i][j][q] : d[i][j][q];
+                }
+            }
+}
+
+// col[(p, z, y, x)][((dz*3 + dy)*3 + dx)*Cin + c] = X[p][z+dz][y+dy][x+dx][c]
+// for the (D-2)^3 output positions of a D^3 input.
+__global__ void k_im2col3(const float* __restrict__ X, uint16_t* __restrict__ col, int B, int D,
+                          int Cin) {
+    const int P = D - 2, Kd = 27 * Cin;
+    const int64_t tot = (int64_t)B * P * P * P * Kd;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    const int k = (int)(e % Kd);
+    int64_t r = e / Kd;
+    const int x = (int)(r % P), y = (int)((r / P) % P), z = (int)((r / ((int64_t)P * P)) % P);
+    const int64_t p = r / ((int64_t)P * P * P);
+    const int c = k % Cin, o = k / Cin, dx = o % 3, dy = (o / 3) % 3, dz = o / 9;
+    col[e] = to_bf16(X[(((p * D + z + dz) * D + y + dy) * D + x + dx) * Cin + c]);
+}
+
+// dX[p][iz][iy][ix][c] = sum over the offsets (dz, dy, dx) whose output
+// position (iz-dz, iy-dy, ix-dx) exists of dcol[that position][offset, c].
+__global__ void k_col2im3(const float* __restrict__ dcol, float* __restrict__ dX, int B, int D,
+                          int Cin) {
+    const int P = D - 2, Kd = 27 * Cin;
+    const int64_t tot = (int64_t)B * D * D * D * Cin;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    const int c = (int)(e % Cin);
+    int64_t r = e / Cin;
+    const int ix = (int)(r % D), iy = (int)((r / D) % D), iz = (int)((r / ((int64_t)D * D)) % D);
+    const int64_t p = r / ((int64_t)D * D * D);
+    float s = 0.f;
+    for (int dz = 0; dz < 3; dz++) {
+        const int z = iz - dz;
+        if (z < 0 || z >= P) continue;
+        for (int dy = 0; dy < 3; dy++) {
+            const int y = iy - dy;
+            if (y < 0 || y >= P) continue;
+            for (int dx = 0; dx < 3; dx++) {
+                const int x = ix - dx;
+                if (x < 0 || x >= P) continue;
+                const int64_t row = ((p * P + z) * P + y) * P + x;
+                s += dcol[row * Kd + ((dz * 3 + dy) * 3 + dx) * Cin + c];
+            }
+        }
+    }
+    dX[e] = s;
+}
+
+__global__ void k_bias_act(float* __restrict__ Y, const float* __restrict__ bias, int64_t n, int C,
+                           int relu) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float v = Y[e] + bias[e % C];
+    Y[e] = relu ? fmaxf(v, 0.f) : v;
+}
+
+// dY *= (Y > 0) for a ReLU layer; dY as BF16 for the GEMMs.
+__global__ void k_act_bwd(const float* __restrict__ Y, float* __restrict__ dY,
+                          uint16_t* __restrict__ dYb, int64_t n, int relu) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float g = dY[e];
+    if (relu && !(Y[e] > 0.f)) g = 0.f;
+    dY[e] = g;
+    dYb[e] = to_bf16(g);
+}
+
+// dbias[c] = sum over rows of dY[row][c]: one block per channel, fixed order.
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dY, float* __restrict__ db,
+                                                int64_t rows, int C) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += dY[r * C + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) y[e] = to_bf16(x[e]);
+}
+
+// Adam on FP32 master weights (bias-corrected step t >= 1); refreshes the
+// BF16 copy the GEMMs read when wb is given.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, uint16_t* __restrict__ wb, int64_t n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float gi = g[e];
+    const float mi = b1 * m[e] + (1.f - b1) * gi, vi = b2 * v[e] + (1.f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    const float wi = w[e] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[e] = wi;
+    if (wb) wb[e] = to_bf16(wi);
+}
+
+// Mean squared error over n values and its gradient 2 (out - t) / n.
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out,
+                                             const float* __restrict__ t, float* __restrict__ d,
+                                             float* __restrict__ loss, int64_t n) {
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const float r = out[e] - t[e];
+        d[e] = 2.f * r / (float)n;
+        s += r * r;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = red[0] / (float)n;
+}
+
+static int cnn_check(bool ok, const char* what) {
+    if (ok) return CUBE2_OK;
+    set_error("cnn: bad arguments to %s", what);
+    return CUBE2_ERR_ARG;
+}
+
+static unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+int cnn_gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+             int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc) {
+    C2_TRY(cnn_check(A && B && C && M >= 0 && N >= 0 && K >= 0 && ldc >= N, "gemm"));
+    C2_TRY(ensure_init());
+    if (M == 0 || N == 0) return CUBE2_OK;
+    dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+    k_gemm_bf16<<<grid, 256, 0, ctx().stream>>>(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_im2col(const float* X, uint16_t* col, int B, int D, int Cin) {
+    C2_TRY(cnn_check(X && col && B >= 0 && D >= 3 && Cin >= 1, "im2col"));
+    C2_TRY(ensure_init());
+    const int64_t P = D - 2, tot = (int64_t)B * P * P * P * 27 * Cin;
+    if (tot) k_im2col3<<<blocks_for(tot), 256, 0, ctx().stream>>>(X, col, B, D, Cin);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_col2im(const float* dcol, float* dX, int B, int D, int Cin) {
+    C2_TRY(cnn_check(dcol && dX && B >= 0 && D >= 3 && Cin >= 1, "col2im"));
+    C2_TRY(ensure_init());
+    const int64_t tot = (int64_t)B * D * D * D * Cin;
+    if (tot) k_col2im3<<<blocks_for(tot), 256, 0, ctx().stream>>>(dcol, dX, B, D, Cin);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_bias_act(float* Y, const float* bias, int64_t rows, int C, int relu) {
+    C2_TRY(cnn_check(Y && bias && rows >= 0 && C >= 1, "bias_act"));
+    C2_TRY(ensure_init());
+    const int64_t n = rows * C;
+    if (n) k_bias_act<<<blocks_for(n), 256, 0, ctx().stream>>>(Y, bias, n, C, relu);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_act_bwd(const float* Y, float* dY, uint16_t* dYb, float* dbias, int64_t rows, int C,
+                int relu) {
+    C2_TRY(cnn_check(Y && dY && dYb && dbias && rows >= 0 && C >= 1, "act_bwd"));
+    C2_TRY(ensure_init());
+    const int64_t n = rows * C;
+    hipStream_t s = ctx().stream;
+    if (n) k_act_bwd<<<blocks_for(n), 256, 0, s>>>(Y, dY, dYb, n, relu);
+    k_colsum<<<C, 256, 0, s>>>(dY, dbias, rows, C);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_to_bf16(const float* x, uint16_t* y, int64_t n) {
+    C2_TRY(cnn_check(x && y && n >= 0, "to_bf16"));
+    C2_TRY(ensure_init());
+    if (n) k_to_bf16<<<blocks_for(n), 256, 0, ctx().stream>>>(x, y, n);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n, float lr,
+             float b1, float b2, float eps, int t) {
+    C2_TRY(cnn_check(w && g && m && v && n >= 0 && t >= 1, "adam"));
+    C2_TRY(ensure_init());
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    if (n) k_adam<<<blocks_for(n), 256, 0, ctx().stream>>>(w, g, m, v, wb, n, lr, b1, b2, eps, c1, c2);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_mse(const float* out, const float* target, float* dout, float* loss, int64_t n) {
+    C2_TRY(cnn_check(out && target && dout && loss && n >= 1, "mse"));
+    C2_TRY(ensure_init());
+    k_mse<<<1, 256, 0, ctx().stream>>>(out, target, dout, loss, n);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+}  // namespace cube2

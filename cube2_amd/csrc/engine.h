@@ -75,6 +75,17 @@ int velocity_apply(const int* d_rhoc, const int64_t* d_idx, float* d_vp, int64_t
 int green_3d(float* d_G, int n, float apm_soft, float apm_range, float gridsize);
 int green_file_write(const float* d_G, int n, const char* path);
 int green_file_read(float* d_G, int n, const char* path);
+int cnn_gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+             int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc);
+int cnn_im2col(const float* X, uint16_t* col, int B, int D, int Cin);
+int cnn_col2im(const float* dcol, float* dX, int B, int D, int Cin);
+int cnn_bias_act(float* Y, const float* bias, int64_t rows, int C, int relu);
+int cnn_act_bwd(const float* Y, float* dY, uint16_t* dYb, float* dbias, int64_t rows, int C,
+                int relu);
+int cnn_to_bf16(const float* x, uint16_t* y, int64_t n);
+int cnn_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n, float lr,
+             float b1, float b2, float eps, int t);
+int cnn_mse(const float* out, const float* target, float* dout, float* loss, int64_t n);
 int fft3d(float* d, int n, int nb, int dir);
 void pm3_release();
 int green_cache_get(const float* host, size_t count, const float** dev);

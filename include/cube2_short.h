@@ -177,6 +177,35 @@ int cube2_green_3d_dev(float* d_G, int n, float apm_soft, float apm_range, float
 int cube2_green_file_write_dev(const float* d_G, int n, const char* path);
 int cube2_green_file_read_dev(float* d_G, int n, const char* path);
 
+/* ---------------------------------------------------------------------
+ * Learned PP stand-in (no counterpart in the reference): the operations of
+ * a 3-D CNN of 3^3 valid convolutions, channels-last FP32 activations
+ * [patch][z][y][x][c], BF16 operands (uint16 bit patterns) on the matrix
+ * cores with FP32 accumulation.  Device pointers; calls return after the
+ * work completes.  cube2_amd/learned.py builds the network on them.
+ * ------------------------------------------------------------------- */
+/* C[m*ldc + n] (+= if acc) sum_k A[m*sam + k*sak] * B[k*sbk + n*sbn]. */
+int cube2_gemm_bf16_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+                        int64_t sbk, int64_t sbn, float* C, int64_t ldc, int M, int N, int K,
+                        int acc);
+/* col[(p,z,y,x)][((dz*3+dy)*3+dx)*Cin + c] = X[p][z+dz][y+dy][x+dx][c] (BF16),
+ * for the (D-2)^3 outputs of each of B inputs of D^3 x Cin. */
+int cube2_cnn_im2col_dev(const float* X, uint16_t* col, int B, int D, int Cin);
+/* The adjoint: dX (B x D^3 x Cin) = sum of the dcol entries that read each element. */
+int cube2_cnn_col2im_dev(const float* dcol, float* dX, int B, int D, int Cin);
+/* Y[r][c] += bias[c], then max(Y, 0) if relu. */
+int cube2_cnn_bias_act_dev(float* Y, const float* bias, int64_t rows, int C, int relu);
+/* dY *= (Y > 0) if relu; dYb = BF16(dY); dbias[c] = sum_r dY[r][c]. */
+int cube2_cnn_act_bwd_dev(const float* Y, float* dY, uint16_t* dYb, float* dbias, int64_t rows,
+                          int C, int relu);
+int cube2_to_bf16_dev(const float* x, uint16_t* y, int64_t n);
+/* One Adam step (t >= 1, bias-corrected) on FP32 weights; wb (may be NULL)
+ * receives their BF16 copy. */
+int cube2_cnn_adam_dev(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n,
+                       float lr, float b1, float b2, float eps, int t);
+/* *loss = mean (out - target)^2 over n values, dout = 2 (out - target) / n. */
+int cube2_cnn_mse_dev(const float* out, const float* target, float* dout, float* loss, int64_t n);
+
 /* Level selection and job order of kick.f90:107-131 (host arrays). */
 int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort, int* napm);
 
