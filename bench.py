@@ -197,6 +197,70 @@ def bench_c2(args) -> int:
     return 0
 
 
+def bench_c4(args) -> int:
+    """Learned stand-in C4: training pairs from the plain PP on four C1-style
+    boxes, 1500 Adam steps of the CNN on the BF16-MFMA kernels (timed), then
+    the held-out box's sparse cells predicted (timed) and scored against plain
+    PP.  One JSON line; not the headline metric."""
+    import torch
+    from cube2_amd import _lib
+    from cube2_amd.learned import CHANNELS, PATCH, LearnedPP, evaluate, pp_pairs, train, training_box
+    xs, ys = [], []
+    for seed, z in ((10, 3.0), (11, 1.0), (12, 0.0), (13, 0.0)):
+        g, pos = training_box(seed, z, npd=32)
+        p, y, _, _ = pp_pairs(g, pos)
+        xs.append(p)
+        ys.append(y)
+    X, Y = np.concatenate(xs), np.concatenate(ys)
+    g, pos = training_box(99, 0.0, npd=32)
+    hp, hy, inv, acc = pp_pairs(g, pos)
+    log(f"C4: {X.shape[0]} training pairs, {hp.shape[0]} held-out sparse cells")
+    net = LearnedPP(seed=0)
+    steps, batch = 1500, 256
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = train(net, X, Y, steps=steps, batch=batch)
+    torch.cuda.synchronize()
+    train_ms = 1e3 * (time.perf_counter() - t0) / steps
+    net.predict(hp[:4096])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net.predict(hp)
+    torch.cuda.synchronize()
+    infer_ms = 1e3 * (time.perf_counter() - t0)
+    r = evaluate(net, hp, hy, inv, acc)
+    # GEMM flops per patch: forward sum over layers of rows*K*N*2; backward twice
+    # that less layer 0's input gradient
+    fwd, bwd, D = 0, 0, PATCH
+    for l in range(4):
+        P = D - 2
+        f = P ** 3 * 27 * CHANNELS[l] * CHANNELS[l + 1] * 2
+        fwd += f
+        bwd += f * (2 if l else 1)
+        D = P
+    tflops = batch * (fwd + bwd) / (train_ms * 1e-3) / 1e12
+    out = {
+        "metric": "C4 learned stand-in: training ms/step, held-out error vs plain PP",
+        "value": round(train_ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(train_ms, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (FP32 accumulation and master weights)",
+        "data": "synthetic (four C1-style boxes, 32^3 lattice + 40% Plummer spheres, seeds 10-13; held-out seed 99)",
+        "config": {"workload": "C4: 3-D CNN 4 x 3^3 convolutions, 32 channels, 9^3 count stencil -> PP "
+                               "acceleration of fine cells in PP cells of < 64 particles",
+                   "batch": batch, "training_pairs": int(X.shape[0])},
+        "train_gemm_tflops": round(tflops, 4),
+        "roofline": {"kernel": "k_gemm_bf16 (whole training step)", "bound": "mfma",
+                     "achieved": round(tflops, 4), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 6), "traffic": None},
+        "inference_ms": round(infer_ms, 3), "inference_cells": int(hp.shape[0]),
+        "held_out_rel_rms_error": r, "without_network_rel_rms_error": 1.0,
+        "loss_first_last": [round(float(np.mean(hist[:50])), 4), round(float(np.mean(hist[-100:])), 4)],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,8 +276,9 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-pm2", action="store_true", help="skip the PM2 measurement")
     ap.add_argument("--no-drift", action="store_true", help="skip the drift measurement")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2"],
-                    help="c1: the headline kick at 256^3; c2: the 4096-tile batching workload")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c4"],
+                    help="c1: the headline kick at 256^3; c2: the 4096-tile batching workload; "
+                         "c4: the learned stand-in (training and held-out error)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM counters)")
     args = ap.parse_args(argv)
@@ -227,6 +292,8 @@ def main(argv=None) -> int:
 
     if args.workload == "c2":
         return bench_c2(args)
+    if args.workload == "c4":
+        return bench_c4(args)
 
     import torch
     from cube2_amd import _lib
