@@ -84,6 +84,7 @@ struct FraK {
     double inv_a;   // 1/a in FP64
     double half_a;  // a/2 in FP64
     float c;        // (1/35)/(a*a), as basic_functions.f08:103 rounds it
+    float r2max;    // a^2 (1 + 2^-17): for r^2 above it, RN(2r/a) > 2 for certain
 };
 
 // F_ra(r, a) of basic_functions.f08:95-109 with the reference's rounding, for
@@ -143,7 +144,7 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
         // softening term: 1/(r*r) unless r is within app; the polynomial runs
         // only when some lane of the wavefront needs it
         float fa = frr;
-        if (Ka.a != 0.0f) {
+        if (Ka.a != 0.0f && __ballot(e.w <= Ka.r2max)) {
             const float epa = div_by(2.0f * rm, Ka.inv_a);
             if (__ballot(epa <= 2.0f)) {
                 const float pa = fra_ref_poly(epa, Ka, inv_rm);
@@ -275,9 +276,11 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         Ka.a = A.app;
         Ka.inv_a = A.app > 0.0f ? 1.0 / (double)A.app : 0.0;
         Ka.half_a = 0.5 * (double)A.app;
+        Ka.r2max = (float)((double)A.app * A.app * (1.0 + 0x1p-17));
         Kr.a = range;
         Kr.inv_a = 1.0 / (double)range;
         Kr.half_a = 0.5 * (double)range;
+        Kr.r2max = (float)((double)range * range * (1.0 + 0x1p-17));
         {
 #pragma clang fp contract(off)
             Ka.c = 1.0f / 35.0f / (A.app * A.app);
