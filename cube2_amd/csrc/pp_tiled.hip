@@ -373,22 +373,20 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                 wave_lds_sync();
                 if (vb + 64 < M) qcur = fetch(vb + 64);
                 // (i, j) of slot t: the difference vector and whether it pairs
+                // (branch-free: the slot is read clamped and masked after)
                 auto test = [&](int t, float4& e) {
-                    const int jj = t + jslot;
-                    bool ok = false;
-                    if (vi && jj < nb) {
 #pragma clang fp contract(off)
-                        const float4 qj = L.jb[jj];
-                        e.x = qj.x - p.x;
-                        e.y = qj.y - p.y;
-                        e.z = qj.z - p.z;
-                        if (kMode == kModePrecise)
-                            e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
-                        else
-                            e.w = e.x * e.x + e.y * e.y + e.z * e.z;
-                        ok = e.w > 0.0f && e.w < pp2;
-                    }
-                    return ok;
+                    const int jj = t + jslot;
+                    const bool in = vi & (jj < nb);
+                    const float4 qj = L.jb[in ? jj : 0];
+                    e.x = qj.x - p.x;
+                    e.y = qj.y - p.y;
+                    e.z = qj.z - p.z;
+                    if (kMode == kModePrecise)
+                        e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
+                    else
+                        e.w = e.x * e.x + e.y * e.y + e.z * e.z;
+                    return in & (e.w > 0.0f) & (e.w < pp2);
                 };
                 // two loops on the chunk-uniform branch, so that the direct
                 // loop carries only its accumulators
