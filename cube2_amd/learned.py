@@ -169,12 +169,14 @@ def pp_pairs(g, pos, occupancy: int = 64):
     from .kick import ShortRangeKick
     from .layout import build_layout
     lay = build_layout(pos, None, g)
-    _lib.set_pp_mode(_lib.PP_PRECISE)
     k = ShortRangeKick(lay)
     k.set_levels(np.full(g.ns3, 2, np.int32))
     mass, amid, dt = 1.0, 1.0, 1.0
-    k.kick(mass, amid, dt, pm3=False, pp=True)
-    _lib.set_pp_mode(_lib.PP_REF)
+    _lib.set_pp_mode(_lib.PP_PRECISE)
+    try:
+        k.kick(mass, amid, dt, pm3=False, pp=True)
+    finally:
+        _lib.set_pp_mode(_lib.PP_REF)  # the library's default arithmetic
     phys = lay.physical_mask()
     acc = np.zeros((pos.shape[0], 3))
     # Delta-v = a * a_mid * dt / (6 pi) per particle (kick.f90's PP scaling)
