@@ -36,15 +36,28 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t* __restrict__ 
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) d[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    // A rows contiguous in k and 8-byte aligned: one 8-byte load per lane
+    const bool avec = sak == 1 && (sam & 3) == 0 && ((uintptr_t)A & 7) == 0;
     for (int k0 = 0; k0 < K; k0 += 16) {
         v4s a[2], b[2];
+        const bool kfull = k0 + 16 <= K;
 #pragma unroll
         for (int i = 0; i < 2; i++) {
             const int m = m0 + i * 16 + r16, n = n0 + i * 16 + r16;
+            if (avec && kfull) {
+                a[i] = m < M ? *reinterpret_cast<const v4s*>(A + (int64_t)m * sam + k0 + kq)
+                             : v4s{0, 0, 0, 0};
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int k = k0 + kq + q;
+                    a[i][q] = (m < M && k < K) ? (short)A[(int64_t)m * sam + (int64_t)k * sak]
+                                               : (short)0;
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int k = k0 + kq + q;
-                a[i][q] = (m < M && k < K) ? (short)A[(int64_t)m * sam + (int64_t)k * sak] : (short)0;
                 b[i][q] = (n < N && k < K) ? (short)B[(int64_t)k * sbk + (int64_t)n * sbn] : (short)0;
             }
         }
