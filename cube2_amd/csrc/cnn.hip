@@ -24,11 +24,19 @@ __device__ inline uint16_t to_bf16(float f) {
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
+// Split K: block z of gridDim.z takes k in [z*kc, (z+1)*kc) and stores its
+// partial product to C + z*zs (the partials are then summed in order).
 __global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t* __restrict__ A, int64_t sam,
                                                    int64_t sak, const uint16_t* __restrict__ B,
                                                    int64_t sbk, int64_t sbn, float* __restrict__ C,
-                                                   int64_t ldc, int M, int N, int K, int acc) {
+                                                   int64_t ldc, int M, int N, int K, int acc,
+                                                   int kc, int64_t zs) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
+    A += (int64_t)kbeg * sak;
+    B += (int64_t)kbeg * sbk;
+    K = kend - kbeg;
+    C += blockIdx.z * zs;
     const int m0 = blockIdx.y * 64 + (wave >> 1) * 32, n0 = blockIdx.x * 64 + (wave & 1) * 32;
     const int r16 = lane & 15, kq = (lane >> 4) * 4;
     v4f d[2][2];
@@ -79,6 +87,18 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t* __restrict__ 
                     *c = acc ? *c + d[i][j][q] : d[i][j][q];
                 }
             }
+}
+
+// C[m][n] (+)= sum over the S split-K partials P[s][m][n], in order.
+__global__ void k_sum_partials(const float* __restrict__ P, float* __restrict__ C, int64_t ldc,
+                               int M, int N, int S, int acc) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)M * N) return;
+    const int m = (int)(e / N), n = (int)(e % N);
+    float v = 0.f;
+    for (int s = 0; s < S; s++) v += P[(int64_t)s * M * N + e];
+    float* c = C + (int64_t)m * ldc + n;
+    *c = acc ? *c + v : v;
 }
 
 // col[(p, z, y, x)][((dz*3 + dy)*3 + dx)*Cin + c] = X[p][z+dz][y+dy][x+dx][c]
@@ -216,8 +236,28 @@ int cnn_gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int
     C2_TRY(cnn_check(A && B && C && M >= 0 && N >= 0 && K >= 0 && ldc >= N, "gemm"));
     C2_TRY(ensure_init());
     if (M == 0 || N == 0) return CUBE2_OK;
+    hipStream_t s = ctx().stream;
     dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
-    k_gemm_bf16<<<grid, 256, 0, ctx().stream>>>(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc);
+    // few output tiles and a long K (the weight gradients, K = rows): split K
+    // over blocks into partial products, then sum them in a fixed order
+    const int64_t tiles = (int64_t)grid.x * grid.y;
+    int S = 1;
+    if (tiles < 256 && K >= 4096) S = (int)std::min<int64_t>(std::min<int64_t>(512 / tiles, K / 2048), 64);
+    if (S <= 1) {
+        k_gemm_bf16<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc, K, 0);
+        C2_HIP(hipGetLastError());
+        return CUBE2_OK;
+    }
+    const int kc = (int)(((K + S - 1) / S + 15) / 16 * 16);
+    S = (K + kc - 1) / kc;
+    static DevBuf partials;
+    C2_TRY(partials.reserve((size_t)S * M * N * sizeof(float)));
+    grid.z = (unsigned)S;
+    k_gemm_bf16<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, partials.as<float>(), N, M, N, K, 0,
+                                     kc, (int64_t)M * N);
+    C2_HIP(hipGetLastError());
+    k_sum_partials<<<blocks_for((int64_t)M * N), 256, 0, s>>>(partials.as<float>(), C, ldc, M, N, S,
+                                                              acc);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

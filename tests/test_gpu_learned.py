@@ -28,8 +28,10 @@ def _bf16_bits(a):
 
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(16, 16, 16, 0, 0), (37, 50, 29, 0, 0), (64, 3, 864, 0, 0),
-                                         (45, 33, 70, 1, 0), (20, 70, 40, 0, 1), (130, 96, 17, 1, 1)])
+                                         (45, 33, 70, 1, 0), (20, 70, 40, 0, 1), (130, 96, 17, 1, 1),
+                                         (864, 32, 20011, 1, 0), (27, 32, 9000, 1, 0)])
 def test_gemm_bf16_exact_on_integers(torch_lib, M, N, K, ta, tb):
+    """(the last two shapes take the split-K path: few output tiles, long K)"""
     torch, _lib = torch_lib
     rng = np.random.default_rng(M * 1000 + N + K)
     A = rng.integers(-8, 9, (M, K)).astype(np.float32)
