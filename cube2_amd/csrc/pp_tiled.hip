@@ -82,7 +82,7 @@ __device__ inline float sqrt_rn(float x) {
 struct FraK {
     float a;        // softening diameter
     double inv_a;   // 1/a in FP64
-    double half_a;  // a/2 in FP64
+    double half_a2;  // (a/2)^2 in FP64
     float c;        // (1/35)/(a*a), as basic_functions.f08:103 rounds it
     float r2max;    // a^2 (1 + 2^-17): for r^2 above it, RN(2r/a) > 2 for certain
 };
@@ -92,13 +92,12 @@ struct FraK {
 // 1/(r*r)).  Evaluated without divergence: the powers are shared, both
 // polynomials are formed in the reference's operation order, and one is
 // selected.  RN(12/e2) is 12 * (FP64 reciprocal), rounded once to float;
-// the reciprocal refines y0 = (a/2 * 1/r)^2, which is within a few float
+// the reciprocal refines y0 = (a/2)^2 (1/r)^2, which is within a few float
 // roundings of 1/e2 (ep = RN(2r/a), e2 = RN(ep*ep)).
-__device__ inline float fra_ref_poly(float ep, const FraK& K, double inv_r) {
+__device__ inline float fra_ref_poly(float ep, const FraK& K, double inv_r2) {
 #pragma clang fp contract(off)
     const float e2 = ep * ep, e3 = e2 * ep, e4 = e3 * ep, e5 = e4 * ep, e6 = e5 * ep;
-    const double q = K.half_a * inv_r;
-    const float t12 = (float)(12.0 * recip_refine((double)e2, q * q));
+    const float t12 = (float)(12.0 * recip_refine((double)e2, K.half_a2 * inv_r2));
     const float hi = K.c * (t12 - 224.0f + 896.0f * ep - 840.0f * e2 + 224.0f * e3 + 70.0f * e4 -
                             48.0f * e5 + 7.0f * e6);
     const float lo = K.c * (224.0f * ep - 224.0f * e3 + 70.0f * e4 + 48.0f * e5 - 21.0f * e6);
@@ -137,17 +136,18 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
         const float rm = sqrt_rn(e.w);
         const double inv_rm = recip64((double)rm);
         // RN(1/(r*r)): the reciprocal of RN(r*r) refined from (1/r)^2
-        const float frr = (float)recip_refine((double)(rm * rm), inv_rm * inv_rm);
+        const double inv_rm2 = inv_rm * inv_rm;
+        const float frr = (float)recip_refine((double)(rm * rm), inv_rm2);
         // range term: r < range, so ep <= 2 except by rounding at the cutoff
         const float epr = div_by(2.0f * rm, Kr.inv_a);
-        const float fr = epr > 2.0f ? frr : fra_ref_poly(epr, Kr, inv_rm);
+        const float fr = epr > 2.0f ? frr : fra_ref_poly(epr, Kr, inv_rm2);
         // softening term: 1/(r*r) unless r is within app; the polynomial runs
         // only when some lane of the wavefront needs it
         float fa = frr;
         if (Ka.a != 0.0f && __ballot(e.w <= Ka.r2max)) {
             const float epa = div_by(2.0f * rm, Ka.inv_a);
             if (__ballot(epa <= 2.0f)) {
-                const float pa = fra_ref_poly(epa, Ka, inv_rm);
+                const float pa = fra_ref_poly(epa, Ka, inv_rm2);
                 if (epa <= 2.0f) fa = pa;
             }
         }
@@ -275,11 +275,11 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         FraK Ka, Kr;
         Ka.a = A.app;
         Ka.inv_a = A.app > 0.0f ? 1.0 / (double)A.app : 0.0;
-        Ka.half_a = 0.5 * (double)A.app;
+        Ka.half_a2 = 0.25 * (double)A.app * (double)A.app;
         Ka.r2max = (float)((double)A.app * A.app * (1.0 + 0x1p-17));
         Kr.a = range;
         Kr.inv_a = 1.0 / (double)range;
-        Kr.half_a = 0.5 * (double)range;
+        Kr.half_a2 = 0.25 * (double)range * (double)range;
         Kr.r2max = (float)((double)range * range * (1.0 + 0x1p-17));
         {
 #pragma clang fp contract(off)
@@ -523,7 +523,7 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
 // r and the length l in [-20, 20]); seed == 0: every float with exponent in
 // [-40, 40] (n = 81 * 2^23) as b and r, with a = l = 1.  Checked: a/b,
 // 12/b, |1 - b*recip64(b)| <= 2^-51, 1/RN(r*r) refined from (1/r)^2, and
-// 12/e2 for ep = RN(2r/l), e2 = RN(ep*ep) refined from (l/2 * 1/r)^2, and
+// 12/e2 for ep = RN(2r/l), e2 = RN(ep*ep) refined from (l/2)^2 (1/r)^2, and
 // sqrt_rn(b) against IEEE sqrtf.
 __global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -567,8 +567,8 @@ __global__ void k_selftest_div(uint64_t seed, int n, unsigned long long* bad) {
     const double inv_r = recip64((double)r);
     const float rr = (float)recip_refine((double)p, inv_r * inv_r);
     const float ep2 = div_by(2.0f * r, 1.0 / (double)l);
-    const double q = 0.5 * (double)l * inv_r;
-    const float t12 = (float)(12.0 * recip_refine((double)e2, q * q));
+    const float t12 =
+        (float)(12.0 * recip_refine((double)e2, 0.25 * (double)l * (double)l * (inv_r * inv_r)));
     if (__float_as_uint(ieee) != __float_as_uint(mine) ||
         __float_as_uint(ieee12) != __float_as_uint(twelve) || !resid ||
         __float_as_uint(ieee_rr) != __float_as_uint(rr) || __float_as_uint(ep) != __float_as_uint(ep2) ||
