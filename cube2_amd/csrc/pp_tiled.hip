@@ -101,7 +101,9 @@ __device__ inline float fra_ref_poly(float ep, const FraK& K, double inv_r2) {
     const float hi = K.c * (t12 - 224.0f + 896.0f * ep - 840.0f * e2 + 224.0f * e3 + 70.0f * e4 -
                             48.0f * e5 + 7.0f * e6);
     const float lo = K.c * (224.0f * ep - 224.0f * e3 + 70.0f * e4 + 48.0f * e5 - 21.0f * e6);
-    return ep > 1.0f ? hi : (ep > 0.0f ? lo : 0.0f);
+    // (the reference's F_ra is 0 at ep = 0; pairs here have r^2 >= 2^-28, so
+    // ep > 0)
+    return ep > 1.0f ? hi : lo;
 }
 
 __device__ inline float fra_fast2(float ep, float ir2, float c) {
