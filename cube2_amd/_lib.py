@@ -89,6 +89,9 @@ SIGNATURES = {
     "cube2_kick_graph_dev": (C.c_int, [C.c_int] + [vp_] * 7 + [C.c_int64, vp_] + [C.c_float] * 3
                              + [vp_] * 5 + [C.POINTER(KickResultC), vp_]),
     "cube2_pp_debug_celllist": (C.c_int, [C.c_int] + [vp_] * 6),
+    "cube2_pp_count_tiles": (C.c_int, [C.c_int]),
+    "cube2_pp_debug_tile_pairs": (C.c_int, [C.c_int, vp_]),
+    "cube2_pp_debug_pair_forces": (C.c_int, [C.c_int, vp_, C.c_int, C.c_float, vp_]),
     "cube2_pm3_debug_job": (C.c_int, [C.c_int] + [vp_] * 6 + [C.c_int64, vp_, C.c_float] + [vp_] * 4),
     "cube2_fft3d_dev": (C.c_int, [vp_, C.c_int, C.c_int, C.c_int]),
     "cube2_selftest_div": (C.c_int, [C.c_uint64, C.c_int, vp_]),
@@ -183,3 +186,26 @@ def profile_stats() -> dict:
 
 def synchronize() -> None:
     check(lib().cube2_synchronize())
+
+
+def pp_count_tiles(on: bool = True) -> None:
+    check(lib().cube2_pp_count_tiles(int(on)))
+
+
+def pp_tile_pairs(ntile: int) -> np.ndarray:
+    """{P_cand, P_acc} per parent tile of the last PP call (counting on)."""
+    out = np.zeros((ntile, 2), np.uint64)
+    check(lib().cube2_pp_debug_tile_pairs(int(ntile), out.ctypes.data))
+    return out
+
+
+def pp_pair_forces(mode: int, d: np.ndarray, ranges: np.ndarray, app: float) -> np.ndarray:
+    """The pair kernel's force for pairs d[n][3] with per-pair ranges."""
+    n = d.shape[0]
+    pairs = np.zeros((n, 4), np.float32)
+    pairs[:, :3] = d
+    pairs[:, 3] = ranges
+    out = np.zeros((n, 3), np.float32)
+    check(lib().cube2_pp_debug_pair_forces(int(mode), pairs.ctypes.data, n, float(app),
+                                           out.ctypes.data))
+    return out

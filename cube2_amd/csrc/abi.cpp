@@ -5,11 +5,12 @@
 
 #include "common.h"
 #include "engine.h"
+#include "pp_kernels.h"
 #include "profile.h"
 
 using namespace cube2;
 
-static std::recursive_mutex g_mx;
+static std::recursive_mutex& g_mx = api_mutex();  // shared with context.cpp
 
 static int64_t n_used_of(const Geom& g, const int64_t* idx_b_r) {
     return idx_b_r[(size_t)g.ntile * g.nte * g.nte - 1];
@@ -311,6 +312,25 @@ int cube2_selftest_div(uint64_t seed, int n, unsigned long long* mismatches) {
         return CUBE2_ERR_ARG;
     }
     return selftest_div(seed, n, mismatches);
+}
+
+int cube2_pp_count_tiles(int on) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    if (pp_count_tiles() != (on != 0)) {
+        pp_count_tiles() = on != 0;
+        buffers_changed();  // a recorded kick records again with the new kernel arguments
+    }
+    return CUBE2_OK;
+}
+
+int cube2_pp_debug_tile_pairs(int ntile, uint64_t* counts) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return pp_debug_tile_pairs(ntile, counts);
+}
+
+int cube2_pp_debug_pair_forces(int mode, const float* pairs, int n, float app, float* forces) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return pair_forces_debug(mode, pairs, n, app, forces);
 }
 
 int cube2_pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys_sorted,

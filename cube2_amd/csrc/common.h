@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -104,6 +105,8 @@ struct Context {
 };
 Context& ctx();
 int ensure_init();
+// Serialises every C-ABI entry point (recursive: entry points call each other).
+std::recursive_mutex& api_mutex();
 
 // Per-call scalars that kernels read from device memory, so that a recorded
 // graph replays with new values: {mass, a_mid, dt, 0}.  The upload copies

@@ -233,12 +233,17 @@ int sync_and_flush() {
 
 using namespace cube2;
 
-static std::mutex g_api_mutex;
+namespace cube2 {
+std::recursive_mutex& api_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+}  // namespace cube2
 
 extern "C" {
 
 int cube2_set_params(const cube2_params_t* p) {
-    std::lock_guard<std::mutex> l(g_api_mutex);
+    std::lock_guard<std::recursive_mutex> l(api_mutex());
     if (!p || p->ng <= 0 || p->nnt <= 0 || p->nns <= 0 || p->ratio_cs <= 0 || p->ngb <= 0 ||
         p->ng % p->ratio_cs || (p->ng / p->ratio_cs) % p->nnt ||
         (p->ng / p->ratio_cs / p->nnt) % p->nns || p->ngb % p->ratio_cs || p->app < 0) {
@@ -267,7 +272,7 @@ int cube2_set_tuning(int key, int value) {
         set_error("tuning key %d value %d out of range", key, value);
         return CUBE2_ERR_ARG;
     }
-    std::lock_guard<std::mutex> l(g_api_mutex);
+    std::lock_guard<std::recursive_mutex> l(api_mutex());
     Context& c = ctx();
     if (c.tune[key] != value) {
         c.tune[key] = value;
@@ -289,7 +294,7 @@ int cube2_get_tuning(int key, int* value) {
 }
 
 int cube2_get_params(cube2_params_t* p) {
-    std::lock_guard<std::mutex> l(g_api_mutex);
+    std::lock_guard<std::recursive_mutex> l(api_mutex());
     Context& c = ctx();
     if (!c.params_set) {
         c.g = make_geom(default_params());
@@ -304,6 +309,7 @@ int cube2_set_pp_mode(int mode) {
         set_error("unknown PP mode %d", mode);
         return CUBE2_ERR_ARG;
     }
+    std::lock_guard<std::recursive_mutex> l(api_mutex());
     ctx().pp_mode = mode;
     return CUBE2_OK;
 }

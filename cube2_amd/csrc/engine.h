@@ -18,6 +18,7 @@ struct PPPlan {
     size_t cap = 1;    // capacity of the sorted arrays (n_used)
     int mode = 0;      // CUBE2_PP_*
     int grid_pairs = 0;  // blocks of the persistent pair kernel
+    bool count_tiles = false;  // count pairs per parent tile
 };
 int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf);
 int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
@@ -124,5 +125,7 @@ void drift_release();
 // ---------------------------------------------------------------- misc
 int fill_zero(void* d, size_t bytes, hipStream_t s);  // a kernel, not a memset
 int selftest_div(uint64_t seed, int n, unsigned long long* mismatches);
+int pair_forces_debug(int mode, const float* pairs, int n, float app, float* forces);
+int pp_debug_tile_pairs(int ntile, uint64_t* counts);
 
 }  // namespace cube2

@@ -40,13 +40,17 @@ int pp_tiled_grid(int mode);
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
     DevBuf chunk_flags, chunks, ilist, big, dense;
-    HostBuf h_res;  // PPRed, per-location counts, tile_first (pinned)
+    HostBuf h_res;  // PPRed, per-location counts, per-tile pairs, tile_first (pinned)
     std::vector<PPTile> h_tiles;
+    std::vector<uint64_t> tile_pairs;  // [ntile][2] of the last call (when counted)
+    bool count_tiles = false;
     int64_t c_tot = 0;
     int n_tot = 0;
     bool valid = false;
 };
 static PPState g_pp;
+
+bool& pp_count_tiles() { return g_pp.count_tiles; }
 
 // ---------------------------------------------------------------- kernels
 
@@ -486,6 +490,12 @@ __global__ void k_tile_first(const int* __restrict__ cell_start, const PPTile* _
 
 // ------------------------------------------------------------ host side
 
+// The PP reduction block: PPRed, then physical particles per subtile
+// location [ns3], then {P_cand, P_acc} per parent tile [ntile][2].
+static size_t red_bytes(const Geom& g, int ntile) {
+    return sizeof(PPRed) + ((size_t)g.ns3 + 2 * (size_t)ntile) * sizeof(unsigned long long);
+}
+
 static int location_maps(const Geom& g, const int* ires, const int* ixyz3, std::vector<int>& iloc,
                          std::vector<int>& owner) {
     iloc.assign(g.ns3, 0);
@@ -501,9 +511,14 @@ static int location_maps(const Geom& g, const int* ires, const int* ixyz3, std::
             return CUBE2_ERR_ARG;
         }
         int loc = (((tz * nnt + ty) * nnt + tx) * nns + sz) * nns * nns + sy * nns + sx;
+        if (owner[loc] >= 0) {
+            set_error("subtiles %d and %d have the same location in ixyz3", owner[loc], it);
+            return CUBE2_ERR_ARG;
+        }
         iloc[loc] = ires[it];
         owner[loc] = it;
     }
+    // ns3 distinct in-range locations out of ns3: every location is filled
     return CUBE2_OK;
 }
 
@@ -600,7 +615,7 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.pos4.reserve(P.cap * sizeof(float4)));
     C2_TRY(g_pp.slot.reserve(P.cap * sizeof(uint32_t)));
     C2_TRY(g_pp.cellid.reserve(P.cap * sizeof(uint32_t)));
-    C2_TRY(g_pp.red.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long)));
+    C2_TRY(g_pp.red.reserve(red_bytes(g, P.ntile)));
     C2_TRY(g_pp.tile_first.reserve((P.ntile + 1) * sizeof(int)));
     C2_TRY(g_pp.chunk_flags.reserve((P.cap + 1) * sizeof(int)));
     C2_TRY(g_pp.chunks.reserve((P.cap + 1) * sizeof(int)));
@@ -609,8 +624,8 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.dense.reserve((P.cap / 64 + 1) * sizeof(int)));
     C2_TRY(g_pp.scan_tmp.reserve(scan_tmp_ints(std::max<int64_t>(P.c_tot, (int64_t)P.cap)) * sizeof(int)));
     C2_TRY(nzero_reserve());
-    C2_TRY(g_pp.h_res.reserve(sizeof(PPRed) + g.ns3 * sizeof(unsigned long long) +
-                              (P.ntile + 1) * sizeof(int)));
+    C2_TRY(g_pp.h_res.reserve(red_bytes(g, P.ntile) + (P.ntile + 1) * sizeof(int)));
+    P.count_tiles = g_pp.count_tiles;
     C2_HIP(hipMemcpyAsync(g_pp.tiles.ptr, T.data(), P.ntile * sizeof(PPTile), hipMemcpyHostToDevice, s));
     C2_HIP(hipMemcpyAsync(g_pp.ires_loc.ptr, P.iloc.data(), g.ns3 * sizeof(int), hipMemcpyHostToDevice, s));
     C2_HIP(hipMemcpyAsync(g_pp.apm3.ptr, apm3, 7 * sizeof(float), hipMemcpyHostToDevice, s));
@@ -628,7 +643,7 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     unsigned long long* ploc = reinterpret_cast<unsigned long long*>(red + 1);
     int* counts = g_pp.counts.as<int>();
     const int* n_sorted = counts + P.c_tot;
-    C2_TRY(fill_zero(red, sizeof(PPRed) + g.ns3 * sizeof(unsigned long long), s));
+    C2_TRY(fill_zero(red, red_bytes(g, P.ntile), s));
     const int64_t* nzp = nullptr;
     C2_TRY(compute_nzero(d_rhoc, d_idx, &nzp, s));
     if (P.ncells_ext > 0) {
@@ -690,6 +705,7 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     A.dv64 = dv64;
     A.vp = d_vp;
     A.red = red;
+    A.tile_pairs = P.count_tiles ? ploc + g.ns3 : nullptr;
     if (P.mode == CUBE2_PP_EXACT) {
         C2_TRY(launch_pp_pairs(A, P.mode, s));
     } else {
@@ -717,10 +733,9 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     }
     // results into pinned host memory
     char* h = g_pp.h_res.as<char>();
-    C2_HIP(hipMemcpyAsync(h, red, sizeof(PPRed) + g.ns3 * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
-    C2_HIP(hipMemcpyAsync(h + sizeof(PPRed) + g.ns3 * sizeof(unsigned long long),
-                          g_pp.tile_first.ptr, (P.ntile + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(h, red, red_bytes(g, P.ntile), hipMemcpyDeviceToHost, s));
+    C2_HIP(hipMemcpyAsync(h + red_bytes(g, P.ntile), g_pp.tile_first.ptr,
+                          (P.ntile + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
     return CUBE2_OK;
 }
 
@@ -731,7 +746,7 @@ int pp_finish(const PPPlan& P, float* f2max, float* vmax, double* ptotal, double
     PPRed r;
     std::memcpy(&r, h, sizeof(PPRed));
     const unsigned long long* hp = reinterpret_cast<const unsigned long long*>(h + sizeof(PPRed));
-    const int* tf = reinterpret_cast<const int*>(h + sizeof(PPRed) + g.ns3 * sizeof(unsigned long long));
+    const int* tf = reinterpret_cast<const int*>(h + red_bytes(g, P.ntile));
     if (r.err & 2) {
         set_error("rhoc and idx_b_r describe slots outside [0, n_used=%lld)", (long long)P.n_used);
         return CUBE2_ERR_ARG;
@@ -761,9 +776,24 @@ int pp_finish(const PPPlan& P, float* f2max, float* vmax, double* ptotal, double
             ttotal[P.owner[t * P.nsub3]] = wall_s * share;
         }
     }
+    if (P.count_tiles) g_pp.tile_pairs.assign(hp + g.ns3, hp + g.ns3 + 2 * (size_t)P.ntile);
+    else g_pp.tile_pairs.clear();
     g_pp.c_tot = P.c_tot;
     g_pp.n_tot = n_tot;
     g_pp.valid = true;
+    return CUBE2_OK;
+}
+
+int pp_debug_tile_pairs(int ntile, uint64_t* counts) {
+    if (!g_pp.valid || g_pp.tile_pairs.empty()) {
+        set_error("no per-tile pair counts: enable them (cube2_pp_count_tiles) before a PP call");
+        return CUBE2_ERR_STATE;
+    }
+    if (!counts || (size_t)ntile * 2 != g_pp.tile_pairs.size()) {
+        set_error("per-tile pair counts: %zu tiles, not %d", g_pp.tile_pairs.size() / 2, ntile);
+        return CUBE2_ERR_ARG;
+    }
+    std::copy(g_pp.tile_pairs.begin(), g_pp.tile_pairs.end(), counts);
     return CUBE2_OK;
 }
 

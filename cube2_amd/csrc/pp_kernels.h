@@ -34,9 +34,15 @@ struct PPArgs {
     const double* dv64;        // pending PM3 Delta-v (FP64) or nullptr
     float* vp;
     PPRed* red;
+    // per parent tile {P_cand, P_acc} [ntile][2], or nullptr (counted only
+    // when requested: cube2_pp_count_tiles)
+    unsigned long long* tile_pairs;
 };
 
 int launch_pp_pairs(const PPArgs& a, int mode, hipStream_t s);
+
+// Whether PP calls count pairs per parent tile (off by default).
+bool& pp_count_tiles();
 
 // log2 of the pair kernel's chunk size in PP cells of >= 64 particles
 inline int pp_dense_lg() {

@@ -167,6 +167,11 @@ __global__ __launch_bounds__(256) void k_pp_pairs(PPArgs A) {
         }
         pc -= 1;  // the particle itself
         pp_update(A, a, ax, ay, az, f2, v0, v1, v2);
+        if (A.tile_pairs) {  // per-tile counts (on request only)
+            const int ti = find_tile(A.tiles, A.ntile, cid);
+            atomicAdd(&A.tile_pairs[2 * ti], pc);
+            atomicAdd(&A.tile_pairs[2 * ti + 1], pa);
+        }
     }
     pp_reduce(A, f2, v0, v1, v2, pc, pa);
 }

@@ -87,6 +87,19 @@ struct FraK {
     float r2max;    // a^2 (1 + 2^-17): for r^2 above it, RN(2r/a) > 2 for certain
 };
 
+__device__ inline FraK make_frak(float a) {
+    FraK K;
+    K.a = a;
+    K.inv_a = a > 0.0f ? 1.0 / (double)a : 0.0;
+    K.half_a2 = 0.25 * (double)a * (double)a;
+    K.r2max = (float)((double)a * a * (1.0 + 0x1p-17));
+    {
+#pragma clang fp contract(off)
+        K.c = 1.0f / 35.0f / (a * a);
+    }
+    return K;
+}
+
 // F_ra(r, a) of basic_functions.f08:95-109 with the reference's rounding, for
 // ep = RN(2r/a) <= 2 (the caller handles a = 0 and ep > 2, where F_ra is
 // 1/(r*r)).  Evaluated without divergence: the powers are shared, both
@@ -262,7 +275,9 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const bool vi = islot < ni;
         const int ia = grouped ? A.ilist[a0 + (vi ? islot : 0)] : a0 + (vi ? islot : 0);
         const float4 p = A.pos4[ia];
-        const PPTile T = A.tiles[uni(find_tile2(A.tiles, A.ntile, cid))];
+        const int ti = uni(find_tile2(A.tiles, A.ntile, cid));
+        const PPTile T = A.tiles[ti];
+        const unsigned long long pc0 = pc, pa0 = pa;
         const long long key = (long long)cid - T.cell_off;
         const int S = T.S;
         const int x = (int)(key % S), y = (int)((key / S) % S), z = (int)(key / ((long long)S * S));
@@ -274,20 +289,7 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
             R = (int)ceilf(range * (float)T.rcp / (float)A.ratio_cs);
             pp2 = range * range;
         }
-        FraK Ka, Kr;
-        Ka.a = A.app;
-        Ka.inv_a = A.app > 0.0f ? 1.0 / (double)A.app : 0.0;
-        Ka.half_a2 = 0.25 * (double)A.app * (double)A.app;
-        Ka.r2max = (float)((double)A.app * A.app * (1.0 + 0x1p-17));
-        Kr.a = range;
-        Kr.inv_a = 1.0 / (double)range;
-        Kr.half_a2 = 0.25 * (double)range * (double)range;
-        Kr.r2max = (float)((double)range * range * (1.0 + 0x1p-17));
-        {
-#pragma clang fp contract(off)
-            Ka.c = 1.0f / 35.0f / (A.app * A.app);
-            Kr.c = 1.0f / 35.0f / (range * range);
-        }
+        const FraK Ka = make_frak(A.app), Kr = make_frak(range);
         for (int q = lane; q < 64 * 3; q += 64) (&L.acc[0][0])[q] = 0;
         wave_lds_sync();
         const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -463,6 +465,13 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
             v1 = fmaxf(v1, fabsf(nv[1]));
             v2 = fmaxf(v2, fabsf(nv[2]));
         }
+        if (A.tile_pairs) {  // per-tile counts (on request only)
+            const unsigned long long dc = wave_sum_u64(pc - pc0), da = wave_sum_u64(pa - pa0);
+            if (lane == 0) {
+                atomicAdd(&A.tile_pairs[2 * ti], dc);
+                atomicAdd(&A.tile_pairs[2 * ti + 1], da);
+            }
+        }
         wave_lds_sync();  // acc and the queue are reused by the next chunk
     }
     const float mx[4] = {f2, v0, v1, v2};
@@ -516,6 +525,80 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
     else
         k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
     C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+// --- per-pair check: the force of each given pair (dx, dy, dz) with the
+// given range, by the pair kernel's own pair_force<mode>, with its
+// acceptance test (0 < r^2 < range^2; others give 0).
+template <int kMode>
+__global__ void k_pair_forces(const float4* __restrict__ in, int n, float app,
+                              float* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+    float range = 0.0f;
+    if (t < n) {
+        const float4 q = in[t];
+        e.x = q.x;
+        e.y = q.y;
+        e.z = q.z;
+        range = q.w;
+    }
+    bool ok;
+    {
+#pragma clang fp contract(off)
+        if (kMode == kModePrecise)
+            e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
+        else
+            e.w = e.x * e.x + e.y * e.y + e.z * e.z;
+        ok = t < n && e.w > 0.0f && e.w < range * range;
+    }
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (ok) {
+        const FraK Ka = make_frak(app), Kr = make_frak(range);
+        pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
+    }
+    if (t < n) {
+        out[3 * t] = fx;
+        out[3 * t + 1] = fy;
+        out[3 * t + 2] = fz;
+    }
+}
+
+int pair_forces_debug(int mode, const float* pairs, int n, float app, float* forces) {
+    C2_TRY(ensure_init());
+    if (n < 0 || (n > 0 && (!pairs || !forces))) {
+        set_error("pair check: bad arguments");
+        return CUBE2_ERR_ARG;
+    }
+    if (mode != CUBE2_PP_REF && mode != CUBE2_PP_FAST && mode != CUBE2_PP_PRECISE) {
+        set_error("pair check: mode %d has no pair_force", mode);
+        return CUBE2_ERR_ARG;
+    }
+    if (n == 0) return CUBE2_OK;
+    hipStream_t s = ctx().stream;
+    float4* d_in = nullptr;
+    float* d_out = nullptr;
+    C2_HIP(hipMalloc(&d_in, (size_t)n * sizeof(float4)));
+    if (hipMalloc(&d_out, (size_t)n * 3 * sizeof(float)) != hipSuccess) {
+        (void)hipFree(d_in);
+        set_error("pair check: out of device memory");
+        return CUBE2_ERR_HIP;
+    }
+    hipError_t e = hipMemcpyAsync(d_in, pairs, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        const unsigned blocks = ceil_div((size_t)n, 256);
+        if (mode == CUBE2_PP_REF) k_pair_forces<kModeRef><<<blocks, 256, 0, s>>>(d_in, n, app, d_out);
+        else if (mode == CUBE2_PP_FAST) k_pair_forces<kModeFast><<<blocks, 256, 0, s>>>(d_in, n, app, d_out);
+        else k_pair_forces<kModePrecise><<<blocks, 256, 0, s>>>(d_in, n, app, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(forces, d_out, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    C2_HIP(e);
     return CUBE2_OK;
 }
 

@@ -316,6 +316,22 @@ int cube2_kick_graph_dev(int flags, const int* ixyz3, const float* apm3, const i
 int cube2_pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys_sorted,
                             uint32_t* slot, int32_t* cell_start);
 
+/* Per-parent-tile pair counts.  After cube2_pp_count_tiles(1), each PP call
+ * also counts P_cand and P_acc per parent tile (the tile totals of
+ * pp_force_kernel.c's per-tile runs, :281-325); cube2_pp_debug_tile_pairs
+ * copies them from the last PP call: counts[ntile][2] = {P_cand, P_acc}.
+ * Off by default (it adds atomics to the pair kernel). */
+int cube2_pp_count_tiles(int on);
+int cube2_pp_debug_tile_pairs(int ntile, uint64_t* counts);
+
+/* The pair kernel's pair force for given pairs (host arrays):
+ * pairs[n][4] = {dx, dy, dz, range}, forces[n][3] = the force of that pair
+ * in mode CUBE2_PP_REF / FAST / PRECISE with softening app, evaluated by the
+ * same device function and acceptance test (0 < r^2 < range^2, else 0) as
+ * the pair kernel.  In REF mode it equals (F_ra(r,app) - F_ra(r,range))*d/r
+ * of pp_force_kernel.c:308-318 bit for bit. */
+int cube2_pp_debug_pair_forces(int mode, const float* pairs, int n, float app, float* forces);
+
 /* One PM3 subtile job's stage outputs from a fresh computation:
  * rho[n^3] (density incl. mass), phi[n^3] (potential / n^3),
  * force[3][(nfp+2)^3]; host pointers, any may be NULL. */

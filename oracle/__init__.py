@@ -47,6 +47,7 @@ def lib():
         L = C.CDLL(str(path))
         L.or_F_ra.restype = C.c_float
         L.or_F_ra.argtypes = [C.c_float, C.c_float]
+        L.or_pair_forces.argtypes = [f32p, f32p, C.c_float, C.c_int, f32p]
         L.or_stable_sort_u32_pair.argtypes = [u32p, u32p, C.c_int, C.c_uint32]
         L.or_pp_tile_sizes.argtypes = [C.POINTER(OrGeom), i32p, i32p, i32p, i32p, C.c_int,
                                        C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -61,6 +62,8 @@ def lib():
         L.or_fft3d_r2c.argtypes = [f32p, f64p, C.c_int]
         L.or_fft3d_c2r.argtypes = [f64p, f64p, C.c_int]
         L.or_fft1d.argtypes = [f64p, C.c_int, C.c_int]
+        L.or_potential_to_force.argtypes = [f64p, C.c_int, C.c_int, C.c_int, f64p,
+                                            C.POINTER(C.c_double)]
         L.or_green_3d.argtypes = [f32p, C.c_int, C.c_float, C.c_float, C.c_float]
         L.or_pm3_job.restype = C.c_int
         L.or_pm3_job.argtypes = [C.POINTER(OrGeom), C.c_int, i32p, i32p, i32p, i32p, i64p, i16p,
@@ -101,6 +104,15 @@ def F_ra(r, apm):
     r = np.asarray(r, dtype=np.float32).reshape(-1)
     a = np.broadcast_to(np.asarray(apm, dtype=np.float32), r.shape)
     return np.array([L.or_F_ra(float(x), float(y)) for x, y in zip(r, a)], dtype=np.float32)
+
+
+def pair_forces(d, ranges, app):
+    """(F_ra(r,app) - F_ra(r,range)) d / r per pair, the reference's FP32 rounding."""
+    d = _c(d, np.float32).reshape(-1, 3)
+    rg = _c(ranges, np.float32).reshape(-1)
+    out = np.zeros_like(d)
+    lib().or_pair_forces(d, rg, float(app), d.shape[0], out)
+    return out
 
 
 def stable_sort(keys, vals, bound):
@@ -202,6 +214,18 @@ def green_3d(n, apm_soft, apm_range, gridsize):
     g = np.zeros((n, n, n // 2 + 1), np.float32)
     lib().or_green_3d(g, int(n), float(apm_soft), float(apm_range), float(gridsize))
     return g
+
+
+def potential_to_force(phi, nfb, nfp):
+    """potential_to_force of an (n, n, n) potential [z][y][x]: returns
+    (F[3][nfp+2]^3, f2max over the interior)."""
+    phi = _c(phi, np.float64)
+    n = phi.shape[0]
+    nf = nfp + 2
+    F = np.zeros((3, nf, nf, nf), np.float64)
+    f2 = C.c_double()
+    lib().or_potential_to_force(phi.reshape(-1), n, int(nfb), int(nfp), F.reshape(-1), C.byref(f2))
+    return F, f2.value
 
 
 def pm3_job(geom, it, ires, ixyz3, ratio_sf, rhoc, idx_b_r, xp, green, mass,

@@ -392,13 +392,26 @@ static void run_job(const or_geom* g, const derived* dp, job_t jb, const int* rh
     const double inv = 1.0 / ((double)n * n * n);
     for (size_t q = 0; q < (size_t)n * n * n; q++) phi[q] *= inv;
     if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)n * n * n);
-    /* potential_to_force: force index f in [0, nfp+1] <-> phi storage f+nfb-1 */
+    const int nf = nfp + 2;
+    double* F = force_out ? force_out : (double*)malloc(sizeof(double) * 3 * (size_t)nf * nf * nf);
+    double f2 = 0.0;
+    or_potential_to_force(phi, n, nfb, nfp, F, &f2);
+    free(phi);
+    if (f2max_job) *f2max_job = f2;
+    if (!force_out) free(F);
+}
+
+/* potential_to_force (kick.f90:545-568) with the weights of
+ * parameters.f90:49-50: F_d = sum_m w_m phi(x + o_m e_d), o = (-2,-1,1,2),
+ * w = ((alpha-1)/4, -alpha/2, alpha/2, (1-alpha)/4) in real(4), alpha = 4/3,
+ * over force indices f in [0, nfp+1]^3 <-> phi storage f + nfb - 1 of the
+ * n^3 potential; *f2max = max |F|^2 over the interior [1, nfp]^3. */
+void or_potential_to_force(const double* phi, int n, int nfb, int nfp, double* F, double* f2max) {
     const float af = 4.0f / 3.0f;
     const double w[4] = {(double)((af - 1.0f) / 4.0f), (double)(-af / 2.0f), (double)(af / 2.0f),
                          (double)((1.0f - af) / 4.0f)};
     const int off[4] = {-2, -1, 1, 2};
     const int nf = nfp + 2;
-    double* F = force_out ? force_out : (double*)malloc(sizeof(double) * 3 * (size_t)nf * nf * nf);
     double f2 = 0.0;
 #pragma omp parallel for reduction(max : f2) schedule(static)
     for (int k0 = 0; k0 < nf; k0++)
@@ -420,9 +433,7 @@ static void run_job(const or_geom* g, const derived* dp, job_t jb, const int* rh
                     if (s > f2) f2 = s;
                 }
             }
-    free(phi);
-    if (f2max_job) *f2max_job = f2;
-    if (!force_out) free(F);
+    *f2max = f2;
 }
 
 int or_pm3_job(const or_geom* g, int it, const int* ires, const int* ixyz3, const int* ratio_sf,

@@ -68,6 +68,24 @@ float or_F_ra(float r, float apm) {
     return 0.0f;
 }
 
+/* The pair force of pp_force_kernel.c:308-318 for given difference vectors:
+ * f = (F_ra(r,app) - F_ra(r,range)) * d / r for 0 < r^2 < range^2, else 0. */
+void or_pair_forces(const float* d, const float* range, float app, int n, float* f) {
+    for (int q = 0; q < n; q++) {
+        float ddx = d[3 * q], ddy = d[3 * q + 1], ddz = d[3 * q + 2];
+        float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        float pp2 = range[q] * range[q];
+        f[3 * q] = f[3 * q + 1] = f[3 * q + 2] = 0.0f;
+        if (r2 > 0.0f && r2 < pp2) {
+            float rm = sqrtf(r2);
+            float fpp = or_F_ra(rm, app) - or_F_ra(rm, range[q]);
+            f[3 * q] = fpp * ddx / rm;
+            f[3 * q + 1] = fpp * ddy / rm;
+            f[3 * q + 2] = fpp * ddz / rm;
+        }
+    }
+}
+
 void or_stable_sort_u32_pair(uint32_t* keys, uint32_t* vals, int n, uint32_t key_bound) {
     uint32_t* cnt = (uint32_t*)calloc((size_t)key_bound + 1, sizeof(uint32_t));
     uint32_t* k2 = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));

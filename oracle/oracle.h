@@ -44,6 +44,8 @@ typedef struct {
 
 /* ---- PP (Q1-Q7) ---- */
 float or_F_ra(float r, float apm);
+/* Pair forces of the reference's pair loop for difference vectors d[n][3]. */
+void or_pair_forces(const float* d, const float* range, float app, int n, float* f);
 
 /* Stable counting sort of u32 keys (values permuted alongside). */
 void or_stable_sort_u32_pair(uint32_t* keys, uint32_t* vals, int n, uint32_t key_bound);
@@ -83,6 +85,9 @@ void or_fft1d(double* re_im /* 2n interleaved */, int n, int sign);
 void or_green_3d(float* Gk /* (n/2+1)*n*n */, int n, float apm_soft, float apm_range,
                  float gridsize);
 
+/* potential_to_force (kick.f90:545-568) on an n^3 potential phi (x fastest):
+ * F[3][nfp+2]^3 at phi storage offset nfb-1, f2max over the interior. */
+void or_potential_to_force(const double* phi, int n, int nfb, int nfp, double* F, double* f2max);
 /* One subtile job, stage outputs for stage-wise checks (any may be NULL):
  * rho[n^3] (float, fine-mesh density incl. mass), phi[n^3] (double,
  * unnormalised C2R(G*R2C(rho)) / n^3), force[3][(nfp+2)^3] (double). */

@@ -30,10 +30,10 @@ def lib():
     return _lib
 
 
-@pytest.mark.parametrize("n", [48, 56, 80, 112, 144, 160, 240])
+@pytest.mark.parametrize("n", [48, 56, 80, 112, 144, 160, 240, 320, 480])
 def test_fft_r2c_c2r_match_numpy(lib, n):
     import torch
-    nb = 2
+    nb = 2 if n <= 240 else 1
     rng = np.random.default_rng(n)
     x = rng.standard_normal((nb, n, n, n)).astype(np.float32)
     buf = np.zeros((nb, n, n, n + 2), np.float32)

@@ -25,18 +25,12 @@ BOUND = {"ref": 1e-5, "fast": 1e-5, "precise": 4e-7}
 
 
 @pytest.fixture(scope="module")
-def c1():
+def c1(c1_case):
     import torch
     assert torch.cuda.is_available()
     from cube2_amd import _lib
-    from cube2_amd.geometry import C1
-    from cube2_amd.kick import select_levels
-    from cube2_amd.layout import build_layout
-    from cube2_amd.synth import config1
     _lib.lib()
-    pos, _ = config1(seed=1)
-    lay = build_layout(pos, None, C1)
-    ires, _, _ = select_levels(C1, lay.rhoc)
+    pos, lay, ires, _ = c1_case
     return pos, lay, ires
 
 

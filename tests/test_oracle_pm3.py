@@ -5,7 +5,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from cube2_amd.geometry import C0, RATIO_SF, Geometry, ixyz3
+from cube2_amd.geometry import APM3F, C0, RATIO_SF, Geometry, ixyz3
 from cube2_amd.layout import build_layout
 
 
@@ -141,24 +141,54 @@ def test_tsc_deposit_invariants(oracle_lib, green_c0):
     assert abs(o["rho"].sum() - 1.0) < 1e-6
 
 
-def test_gradient_of_a_plane_wave(oracle_lib):
-    # potential_to_force applied to phi = cos(k x) gives
-    # (8 sin k - sin 2k)/6 * (-sin(k x)) with alpha = 4/3 (exact up to the
-    # real(4) weights), checked through a job whose Green's table is the
-    # transform of the plane wave itself.
-    g = Geometry(ng=32, nnt=1, nns=1, ratio_cs=4, ngb=8, app=0.05)
-    n = g.nft(2)
-    k = 2 * np.pi * 3 / n
-    # zero particles: deposit nothing; instead feed rho via a Green table of
-    # ones and check the FD operator on a known rho (one particle-free job
-    # gives phi = 0, so build the check on the formula directly).
-    xs = np.arange(n)
-    phi = np.cos(k * xs)
-    af = np.float32(4) / np.float32(3)
-    w = np.array([(af - 1) / 4, -af / 2, af / 2, (1 - af) / 4], dtype=np.float32).astype(np.float64)
-    fd = sum(w[m] * np.roll(phi, -o) for m, o in enumerate((-2, -1, 1, 2)))
-    want = -(8 * np.sin(k) - np.sin(2 * k)) / 6 * np.sin(k * xs)
-    assert np.allclose(fd, want, atol=2e-7)
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_gradient_of_a_plane_wave(oracle_lib, axis):
+    """The oracle's potential_to_force on phi = cos(k x_d) (x_d the storage
+    index along axis d): the 4-point stencil with alpha = 4/3 gives
+    F_d = -(8 sin k - sin 2k)/6 sin(k (f + nfb - 1)) at force index f, the
+    other components 0, and f2max its largest square over the interior."""
+    n, nfb, nfp = 48, 8, 32
+    k = 2 * np.pi * 5 / n
+    x = np.arange(n, dtype=np.float64)
+    shape = [1, 1, 1]
+    shape[2 - axis] = n                     # storage [z][y][x]
+    phi = np.broadcast_to(np.cos(k * x).reshape(shape), (n, n, n))
+    F, f2 = oracle_lib.potential_to_force(phi, nfb, nfp)
+    f = np.arange(nfp + 2) + nfb - 1
+    amp = -(8 * np.sin(k) - np.sin(2 * k)) / 6
+    want = np.broadcast_to((amp * np.sin(k * f)).reshape([1, 1, 1][:2 - axis] + [nfp + 2] +
+                                                           [1] * axis), F[axis].shape)
+    assert np.allclose(F[axis], want, rtol=0, atol=1e-6 * abs(amp))
+    for other in {0, 1, 2} - {axis}:
+        assert np.abs(F[other]).max() < 1e-12
+    inner = want[1:-1, 1:-1, 1:-1] if axis == 0 else want
+    assert abs(f2 - (np.abs(amp * np.sin(k * f[1:-1])) ** 2).max()) < 1e-6 * amp * amp
+    del inner
+
+
+@pytest.mark.parametrize("n,rsf", [(48, 2), (80, 2), (96, 4)])
+def test_green_matches_independent_alias_sum(oracle_lib, n, rsf):
+    """or_green_3d against an independent numpy FP64 evaluation of
+    Green.f90:22-71 at sampled k; the same evaluation without the alias sum
+    (n_int = 0) or with the CIC sinc power (4 instead of 2p+2 = 6) misses
+    the table by far more than the tolerance, so the check responds to both."""
+    from conftest import green_at
+    soft, rng_a = float(APM3F), float(APM3F * rsf)
+    gs = float(np.float32(1) / np.float32(rsf))
+    G = oracle_lib.green_3d(n, soft, rng_a, gs)
+    rng = np.random.default_rng(n)
+    idx = np.stack([rng.integers(0, n // 2 + 1, 400), rng.integers(0, n, 400),
+                    rng.integers(0, n, 400)], 1)
+    idx = np.concatenate([idx, [[0, 0, 0], [n // 2, 0, n // 2], [1, 0, 0], [n // 2, n - 1, 3]]])
+    got = G[idx[:, 2], idx[:, 1], idx[:, 0]].astype(np.float64)
+    want = green_at(n, soft, rng_a, gs, idx)
+    scale = np.abs(want).max()
+    assert np.allclose(got, want, rtol=3e-7, atol=1e-7 * scale)
+    tol = 3e-7 * np.abs(want) + 1e-7 * scale
+    for kw in (dict(n_int=0), dict(sinc_pow=4)):
+        alt = green_at(n, soft, rng_a, gs, idx, **kw)
+        missed = np.abs(got - alt) > tol
+        assert missed.mean() > 0.1, (kw, missed.mean())
 
 
 def test_integer_fine_floor_equals_fp64_floor():

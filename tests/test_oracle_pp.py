@@ -142,3 +142,32 @@ def test_cell_binned_fp64_direct_sum_equals_all_pairs(oracle_lib):
     a = oracle_lib.direct_pp_f64(pos, 32.0, tg, rg, 0.05)
     b = oracle_lib.direct_pp_f64(pos, 32.0, tg, rg, 0.05, cells=True)
     assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+
+
+def test_pair_forces_compose_the_reference_f_ra(oracle_lib):
+    """or_pair_forces (the checker of the GPU pair law) against the pair
+    expression of pp_force_kernel.c:308-318 evaluated in float32 numpy with
+    F_ra taken from the compiled reference (oracle/_ref; where it is not
+    built, from the oracle's F_ra, which the golden table pins)."""
+    from conftest import pair_cases
+    if oracle_lib.ref_available():
+        R = oracle_lib.ref()
+        fra = lambda r, a: np.float32(R.F_ra(float(r), float(a)))  # noqa: E731
+    else:
+        fra = lambda r, a: oracle_lib.F_ra([r], a)[0]  # noqa: E731
+    rng = np.random.default_rng(31)
+    for app in (0.05, 0.0):
+        d, rg = pair_cases(rng, APM3[1:6:2], app)
+        sel = rng.choice(d.shape[0], 3000, replace=False)
+        d, rg = d[sel], rg[sel]
+        got = oracle_lib.pair_forces(d, rg, app)
+        f32 = np.float32
+        for q in range(d.shape[0]):
+            x, y, z = d[q]
+            r2 = f32(f32(f32(x * x) + f32(y * y)) + f32(z * z))
+            want = np.zeros(3, np.float32)
+            if f32(0) < r2 < f32(rg[q] * rg[q]):
+                rm = np.sqrt(r2)
+                fpp = f32(fra(rm, app) - fra(rm, rg[q]))
+                want = np.array([f32(fpp * c) / rm for c in (x, y, z)], np.float32)
+            assert np.array_equal(got[q].view(np.uint32), want.view(np.uint32)), (q, d[q], rg[q])
