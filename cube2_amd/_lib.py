@@ -104,12 +104,27 @@ SIGNATURES = {
 }
 
 
+def _torch_runtime_first() -> None:
+    """PyTorch-ROCm wheels carry their own HIP and HSA runtimes, so a Python
+    process that uses torch holds two.  torch's runtime must open the device
+    before the library's: with the library loaded first, its
+    hipGetDeviceCount reports no device once torch has initialised.  Loading
+    torch and querying its device first keeps the working order (the query
+    returns False where there is no GPU)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def lib():
     global _LIB
     if _LIB is None:
         if not LIB_PATH.exists():
             raise FileNotFoundError(
                 f"{LIB_PATH} is not built; run __graft_entry__.build() or python cube2_amd/build.py")
+        _torch_runtime_first()
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

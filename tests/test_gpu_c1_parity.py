@@ -78,17 +78,22 @@ def test_c1_fine_level_job_matches_oracle(lib, oracle_lib, c1_case, level):
     phi = np.zeros((n, n, n), np.float32)
     force = np.zeros((3, nf, nf, nf), np.float32)
     f2 = np.zeros(1, np.float32)
-    lib.check(lib.lib().cube2_pm3_debug_job(it, ires.ctypes.data, ixyz3(g).ctypes.data,
+    ix = ixyz3(g)  # held: the library reads it through its address
+    lib.check(lib.lib().cube2_pm3_debug_job(it, ires.ctypes.data, ix.ctypes.data,
                                             RATIO_SF.ctypes.data, d["rhoc"].data_ptr(),
                                             d["idx"].data_ptr(), d["xp"].data_ptr(), lay.n_used,
                                             G.data_ptr(), 1.0, rho.ctypes.data, phi.ctypes.data,
                                             force.ctypes.data, f2.ctypes.data))
     del d, G
-    o = oracle_lib.pm3_job(g, it, ires, ixyz3(g), RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp, Gh, 1.0)
+    o = oracle_lib.pm3_job(g, it, ires, ix, RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp, Gh, 1.0)
     e_rho, e_phi, e_f = rel_rms(rho, o["rho"]), rel_rms(phi, o["phi"]), rel_rms(force, o["force"])
     print(f"level {level}: rho {e_rho:.2e} phi {e_phi:.2e} force {e_f:.2e} "
           f"f2max {f2[0]:.6e} vs {o['f2max']:.6e}", flush=True)
-    assert e_rho < 1e-6
+    # The oracle sums each cell's FP32 TSC weights sequentially in FP32, as
+    # the reference's real(4) density does (kick.f90:439-445); the device sums
+    # them in 32.32 fixed point.  Cells in the clumps of the finest levels
+    # take thousands of weights, so the oracle's own rounding reaches ~1e-6.
+    assert e_rho < 5e-6
     assert e_phi < RTOL
     assert e_f < RTOL
     assert abs(f2[0] - o["f2max"]) < 3e-5 * o["f2max"]
