@@ -70,6 +70,8 @@ SIGNATURES = {
     "cube2_green_file_read_dev": (C.c_int, [vp_, C.c_int, C.c_char_p]),
     "cube2_gemm_bf16_dev": (C.c_int, [vp_, C.c_int64, C.c_int64, vp_, C.c_int64, C.c_int64, vp_,
                                       C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cube2_gemm_bf16_noalloc_dev": (C.c_int, [vp_, C.c_int64, C.c_int64, vp_, C.c_int64, C.c_int64,
+                                              vp_, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cube2_cnn_im2col_dev": (C.c_int, [vp_, vp_, C.c_int, C.c_int, C.c_int]),
     "cube2_cnn_col2im_dev": (C.c_int, [vp_, vp_, C.c_int, C.c_int, C.c_int]),
     "cube2_cnn_bias_act_dev": (C.c_int, [vp_, vp_, C.c_int64, C.c_int, C.c_int]),

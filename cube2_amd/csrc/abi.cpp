@@ -130,6 +130,7 @@ void c_gpu_velocity_finalize(void) {
     kick_release();
     layout_release();
     drift_release();
+    cnn_release();
     Context& C = ctx();
     C.dv64.release();
     C.d_vp.release();
@@ -190,6 +191,11 @@ int cube2_gemm_bf16_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint1
                         int64_t sbk, int64_t sbn, float* C, int64_t ldc, int M, int N, int K,
                         int acc) {
     return cnn_call([&] { return cnn_gemm(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc); });
+}
+int cube2_gemm_bf16_noalloc_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+                                int64_t sbk, int64_t sbn, float* C, int64_t ldc, int M, int N,
+                                int K, int acc) {
+    return cnn_call([&] { return cnn_gemm_noalloc(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc); });
 }
 int cube2_cnn_im2col_dev(const float* X, uint16_t* col, int B, int D, int Cin) {
     return cnn_call([&] { return cnn_im2col(X, col, B, D, Cin); });

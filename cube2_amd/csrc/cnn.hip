@@ -231,36 +231,106 @@ static int cnn_check(bool ok, const char* what) {
 
 static unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-int cnn_gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
-             int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc) {
-    C2_TRY(cnn_check(A && B && C && M >= 0 && N >= 0 && K >= 0 && ldc >= N, "gemm"));
-    C2_TRY(ensure_init());
-    if (M == 0 || N == 0) return CUBE2_OK;
-    hipStream_t s = ctx().stream;
-    dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
-    // few output tiles and a long K (the weight gradients, K = rows): split K
-    // over blocks into partial products, then sum them in a fixed order
-    const int64_t tiles = (int64_t)grid.x * grid.y;
+// Split-K schedule of a product: S partial products over K ranges of kc
+// (a multiple of 16); S = 1 for no split.  Few output tiles and a long K
+// (the weight gradients, K = rows) are split over blocks, and the partials
+// summed in a fixed order.  S * M * N <= 2^21 for every split shape (tiles
+// >= M N / 4096 and S <= 512 / tiles).
+static void gemm_split(int M, int N, int K, int* S_out, int* kc_out) {
+    const int64_t tiles = (int64_t)((N + 63) / 64) * ((M + 63) / 64);
     int S = 1;
     if (tiles < 256 && K >= 4096) S = (int)std::min<int64_t>(std::min<int64_t>(512 / tiles, K / 2048), 64);
+    int kc = K;
+    if (S > 1) {
+        kc = (int)(((K + S - 1) / S + 15) / 16 * 16);
+        S = (K + kc - 1) / kc;
+    }
+    *S_out = S;
+    *kc_out = kc;
+}
+
+// The product on stream s, split-K partials (S > 1) in part[S * M * N].
+static int gemm_launch(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+                       int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc, int S,
+                       int kc, float* part, hipStream_t s) {
+    dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
     if (S <= 1) {
         k_gemm_bf16<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc, K, 0);
         C2_HIP(hipGetLastError());
         return CUBE2_OK;
     }
-    const int kc = (int)(((K + S - 1) / S + 15) / 16 * 16);
-    S = (K + kc - 1) / kc;
-    static DevBuf partials;
-    C2_TRY(partials.reserve((size_t)S * M * N * sizeof(float)));
     grid.z = (unsigned)S;
-    k_gemm_bf16<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, partials.as<float>(), N, M, N, K, 0,
-                                     kc, (int64_t)M * N);
+    k_gemm_bf16<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, part, N, M, N, K, 0, kc,
+                                     (int64_t)M * N);
     C2_HIP(hipGetLastError());
-    k_sum_partials<<<blocks_for((int64_t)M * N), 256, 0, s>>>(partials.as<float>(), C, ldc, M, N, S,
-                                                              acc);
+    k_sum_partials<<<blocks_for((int64_t)M * N), 256, 0, s>>>(part, C, ldc, M, N, S, acc);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
+
+static DevBuf g_partials;  // split-K partials of cnn_gemm (released by the finalizer)
+
+int cnn_gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+             int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc) {
+    C2_TRY(cnn_check(A && B && C && M >= 0 && N >= 0 && K >= 0 && ldc >= N, "gemm"));
+    C2_TRY(ensure_init());
+    if (M == 0 || N == 0) return CUBE2_OK;
+    int S, kc;
+    gemm_split(M, N, K, &S, &kc);
+    if (S > 1) C2_TRY(g_partials.reserve((size_t)S * M * N * sizeof(float)));
+    return gemm_launch(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc, S, kc, g_partials.as<float>(),
+                       ctx().stream);
+}
+
+// ------------------------------------------------ allocation-free product
+// cube2_gemm_bf16_noalloc_dev keeps its split-K partials in this file-scope
+// device array (4 MiB; every GPU the library runs on holds its own copy
+// until the process ends).  Host code never passes the array's name to a
+// kernel: in host code it is a stand-in, not a device address.  The device
+// address comes from hipGetSymbolAddress, once per device with that device
+// current, and is kept per device.  Calls that overlap on one GPU share the
+// array and race; the partials need no clearing (every call writes all
+// S * M * N of them before they are summed).
+constexpr size_t kScratchFloats = (size_t)1 << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+constexpr int kMaxDevices = 64;
+static float* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_address(float** out) {
+    int dev = 0;
+    C2_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d outside the scratch table", dev);
+        return CUBE2_ERR_STATE;
+    }
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        C2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CUBE2_OK;
+}
+
+int cnn_gemm_noalloc(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+                     int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc) {
+    C2_TRY(cnn_check(A && B && C && M >= 0 && N >= 0 && K >= 0 && ldc >= N, "gemm_noalloc"));
+    C2_TRY(ensure_init());
+    float* part = nullptr;
+    C2_TRY(scratch_address(&part));
+    if (M == 0 || N == 0) return CUBE2_OK;
+    int S, kc;
+    gemm_split(M, N, K, &S, &kc);
+    if (S > 1 && (size_t)S * M * N > kScratchFloats) {
+        set_error("gemm_noalloc: M=%d N=%d K=%d needs %zu bytes of split-K scratch; the static "
+                  "scratch holds %zu", M, N, K, (size_t)S * M * N * sizeof(float),
+                  kScratchFloats * sizeof(float));
+        return CUBE2_ERR_CAPACITY;
+    }
+    return gemm_launch(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc, S, kc, part, ctx().stream);
+}
+
+void cnn_release() { g_partials.release(); }
 
 int cnn_im2col(const float* X, uint16_t* col, int B, int D, int Cin) {
     C2_TRY(cnn_check(X && col && B >= 0 && D >= 3 && Cin >= 1, "im2col"));

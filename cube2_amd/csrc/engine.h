@@ -78,6 +78,9 @@ int green_file_write(const float* d_G, int n, const char* path);
 int green_file_read(float* d_G, int n, const char* path);
 int cnn_gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
              int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc);
+int cnn_gemm_noalloc(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+                     int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc);
+void cnn_release();
 int cnn_im2col(const float* X, uint16_t* col, int B, int D, int Cin);
 int cnn_col2im(const float* dcol, float* dX, int B, int D, int Cin);
 int cnn_bias_act(float* Y, const float* bias, int64_t rows, int C, int relu);

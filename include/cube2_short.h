@@ -188,6 +188,15 @@ int cube2_green_file_read_dev(float* d_G, int n, const char* path);
 int cube2_gemm_bf16_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
                         int64_t sbk, int64_t sbn, float* C, int64_t ldc, int M, int N, int K,
                         int acc);
+/* The same product, allocating nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Split-K partials live in
+ * a static 4 MiB device array per GPU; a split shape needing more returns
+ * CUBE2_ERR_CAPACITY (S*M*N*4 bytes, S the split count).  Results equal
+ * cube2_gemm_bf16_dev's bit for bit.  Calls that overlap on one GPU race. */
+int cube2_gemm_bf16_noalloc_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+                                int64_t sbk, int64_t sbn, float* C, int64_t ldc, int M, int N,
+                                int K, int acc);
 /* col[(p,z,y,x)][((dz*3+dy)*3+dx)*Cin + c] = X[p][z+dz][y+dy][x+dx][c] (BF16),
  * for the (D-2)^3 outputs of each of B inputs of D^3 x Cin. */
 int cube2_cnn_im2col_dev(const float* X, uint16_t* col, int B, int D, int Cin);
