@@ -93,6 +93,53 @@ def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s
                         f"OpenMP threads {ncpu}"))
 
 
+def time_dropin(lay, k, greens_host, mass, amid, dt, steps):
+    """The Fortran host's two calls per kick through the drop-in C ABI on host
+    arrays (kick.f90:144-146 then 202-205): c_pm3_force_kernel uploads the
+    positions and leaves the mesh Delta-v pending, c_pp_force_kernel uploads
+    the velocities, applies PM3 + PP and downloads them.  Wall ms per kick,
+    PCIe transfers included (pageable host memory, as a Fortran caller
+    hands it over)."""
+    import ctypes as C
+
+    import torch
+
+    from cube2_amd import _lib
+    from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
+    g = lay.geom
+    L = _lib.lib()
+    _lib.set_params(g)
+    ires = np.ascontiguousarray(k.ires, np.int32)
+    isort = np.ascontiguousarray(k.isort, np.int32)
+    ix = ixyz3(g)
+    rhoc = np.ascontiguousarray(lay.rhoc, np.int32)
+    idx = np.ascontiguousarray(lay.idx_b_r, np.int64)
+    xp = np.ascontiguousarray(lay.xp, np.int16)
+    vp = np.ascontiguousarray(lay.vp, np.float32).copy()
+    gk = [np.ascontiguousarray(greens_host[Lv], np.float32) if Lv in greens_host
+          else np.zeros(1, np.float32) for Lv in range(2, 7)]
+    f2 = np.zeros(1, np.float32)
+    vm = np.zeros(3, np.float32)
+    pt = np.zeros(g.ns3)
+    tt = np.zeros(g.ns3)
+    a = lambda x: x.ctypes.data  # noqa: E731
+    apm3 = np.ascontiguousarray(APM3, np.float32)
+    rsf = np.ascontiguousarray(RATIO_SF, np.int32)
+    times = []
+    for s in range(steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.c_pm3_force_kernel(a(isort), a(ires), a(ix), a(rsf), a(rhoc), a(idx), a(xp),
+                             *[a(x) for x in gk], a(vp), mass, amid, dt, a(f2), a(vm), a(pt))
+        L.c_pp_force_kernel(a(isort), a(ires), a(ix), a(apm3), a(rsf), a(rhoc), a(idx), a(xp),
+                            a(vp), mass, amid, dt, a(f2), a(vm), a(pt), a(tt))
+        times.append(1e3 * (time.perf_counter() - t0))
+        log(f"drop-in kick {s}: {times[-1]:.1f} ms")
+    L.c_pm3_fft_finalize()
+    return {"ms": round(min(times[1:]), 3), "first_call_ms": round(times[0], 3),
+            "bytes_over_pcie_per_kick": int(xp.nbytes + 2 * vp.nbytes + 2 * (rhoc.nbytes + idx.nbytes))}
+
+
 def profiled_traffic(stage: str):
     """HBM bytes per launch of a stage's kernel at this workload, from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (None if absent)."""
@@ -140,7 +187,7 @@ def bench_c2(args) -> int:
     log(f"C2 inputs: {n} particles in {C2.ntile} tiles (median {int(np.median(cnt))}, "
         f"max {int(cnt.max())}), {n_used} slots ({time.perf_counter() - t0:.1f} s)")
     _lib.set_pp_mode({"ref": _lib.PP_REF, "exact": _lib.PP_EXACT, "fast": _lib.PP_FAST,
-                     "precise": _lib.PP_PRECISE}[args.pp_mode])
+                     "precise": _lib.PP_PRECISE, "plummer": _lib.PP_PLUMMER}[args.pp_mode])
     k = ShortRangeKick.from_device(C2, rhoc, idx, xp, vp, n_used)
     levels = sorted(set(int(x) for x in k.ires))
     for L in levels:
@@ -174,7 +221,7 @@ def bench_c2(args) -> int:
     ms = 1e3 * (time.perf_counter() - t1) / args.steps
     assert (rg.p_cand, rg.p_acc) == (res.p_cand, res.p_acc)
     pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq",
-               "precise": "pp_pairs_precise"}[args.pp_mode]
+               "precise": "pp_pairs_precise", "plummer": "pp_pairs_plummer"}[args.pp_mode]
     pp_ms = stats[pp_name]["ms"] / args.steps
     stages = {nm: round(v["ms"] / args.steps, 3) for nm, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
     log(f"C2 kick {ms:.1f} ms (graph), first call {cold_ms:.1f} ms; stages {stages}")
@@ -198,39 +245,42 @@ def bench_c2(args) -> int:
 
 
 def bench_c4(args) -> int:
-    """Learned stand-in C4: training pairs from the plain PP on four C1-style
-    boxes, 1500 Adam steps of the CNN on the BF16-MFMA kernels (timed), then
-    the held-out box's sparse cells predicted (timed) and scored against plain
-    PP.  One JSON line; not the headline metric."""
+    """Learned stand-in C4: 2M training samples (the particles the network
+    stands in for) from four C1-geometry boxes at z = 10, 3, 1, 0 kicked by
+    the plain PP; Adam training on the matrix-core kernels (timed); then on
+    the 256^3 headline set the plain kick and the learned kick (graph
+    replays, timed) and the learned kick's error against the plain one: on
+    the substituted particles' PP acceleration and on every particle's
+    short-range (PM3 + PP) Delta-v; and the substituted error on a held-out
+    box (seed 99, z = 0).  One JSON line; not the headline metric."""
     import torch
     from cube2_amd import _lib
-    from cube2_amd.learned import CHANNELS, PATCH, LearnedPP, evaluate, pp_pairs, train, training_box
-    xs, ys = [], []
-    for seed, z in ((10, 3.0), (11, 1.0), (12, 0.0), (13, 0.0)):
-        g, pos = training_box(seed, z, npd=32)
-        p, y, _, _ = pp_pairs(g, pos)
-        xs.append(p)
-        ys.append(y)
-    X, Y = np.concatenate(xs), np.concatenate(ys)
-    g, pos = training_box(99, 0.0, npd=32)
-    hp, hy, inv, acc = pp_pairs(g, pos)
-    log(f"C4: {X.shape[0]} training pairs, {hp.shape[0]} held-out sparse cells")
+    from cube2_amd.geometry import C1
+    from cube2_amd.kick import ShortRangeKick
+    from cube2_amd.layout import build_layout_device
+    from cube2_amd.learned import CHANNELS, PATCH, LearnedPP, TrainingSet, box_samples, c1_style_box, train
+    from cube2_amd.synth import config1
+
+    t0 = time.perf_counter()
+    sets = []
+    for seed, z in ((10, 10.0), (11, 3.0), (12, 1.0), (13, 0.0)):
+        ts, k = box_samples(C1, c1_style_box(seed, z), max_samples=args.c4_samples // 4, seed=seed)
+        del k
+        sets.append(ts)
+        log(f"C4 box seed {seed} z {z}: {ts.n} samples ({time.perf_counter() - t0:.0f} s)")
+    ts = TrainingSet.concat(sets)
+    del sets
+    torch.cuda.empty_cache()
     net = LearnedPP(seed=0)
-    steps, batch = 1500, 256
+    steps, batch = args.c4_steps, 256
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    hist = train(net, X, Y, steps=steps, batch=batch)
+    t1 = time.perf_counter()
+    hist = train(net, ts, steps=steps, batch=batch, log=log)
     torch.cuda.synchronize()
-    train_ms = 1e3 * (time.perf_counter() - t0) / steps
-    net.predict(hp[:4096])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    net.predict(hp)
-    torch.cuda.synchronize()
-    infer_ms = 1e3 * (time.perf_counter() - t0)
-    r = evaluate(net, hp, hy, inv, acc)
-    # GEMM flops per patch: forward sum over layers of rows*K*N*2; backward twice
-    # that less layer 0's input gradient
+    train_ms = 1e3 * (time.perf_counter() - t1) / steps
+    del ts
+    torch.cuda.empty_cache()
+    # flops of a training step: forward convolutions, backward dW and dX
     fwd, bwd, D = 0, 0, PATCH
     for l in range(4):
         P = D - 2
@@ -239,21 +289,95 @@ def bench_c4(args) -> int:
         bwd += f * (2 if l else 1)
         D = P
     tflops = batch * (fwd + bwd) / (train_ms * 1e-3) / 1e12
+    log(f"C4 training: {train_ms:.2f} ms/step, {tflops:.2f} TFLOP/s")
+
+    # held-out box: substituted particles' error against plain PP
+    hold, kh = box_samples(C1, c1_style_box(99, 0.0), max_samples=200000, seed=99)
+    pred = net.predict(hold).double()
+    tgt = hold.target.double()
+    held_err = float(torch.sqrt(((pred - tgt) ** 2).sum() / (tgt ** 2).sum()))
+    del hold, kh
+    torch.cuda.empty_cache()
+
+    # the headline 256^3 set: plain and learned kicks
+    pos, vel = config1(seed=args.seed)
+    rhoc, idx, xp, vp, n_used = build_layout_device(pos, vel, C1)
+    del pos, vel
+    k = ShortRangeKick.from_device(C1, rhoc, idx, xp, vp, n_used)
+    mass, amid, dt = 1.0, 0.02, 1e-3
+    vp0 = k.d_vp.clone()
+    levels = sorted(set(int(x) for x in k.ires))
+    for L in levels:
+        k.green(L)
+    net.load()
+
+    def timed(learned):
+        k.d_vp.copy_(vp0)
+        k.kick_graph(mass, amid, dt, learned=learned)
+        ms = []
+        for _ in range(args.steps):
+            k.d_vp.copy_(vp0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = k.kick_graph(mass, amid, dt, learned=learned)
+            torch.cuda.synchronize()
+            ms.append(1e3 * (time.perf_counter() - t))
+        return min(ms), r, k.d_vp.clone()
+
+    plain_ms, rp, vp_plain = timed(False)
+    learned_ms, rl, vp_learned = timed(True)
+    # stage profile of one eager learned kick (pair kernel, network stages)
+    k.d_vp.copy_(vp0)
+    _lib.profile(True)
+    _lib.profile_reset()
+    k.kick(mass, amid, dt, learned=True)
+    torch.cuda.synchronize()
+    st_l = _lib.profile_stats()
+    _lib.profile_reset()
+    k.d_vp.copy_(vp0)
+    k.kick(mass, amid, dt)
+    torch.cuda.synchronize()
+    st_p = _lib.profile_stats()
+    _lib.profile(False)
+    _lib.lib().cube2_learned_clear()
+    dv_p = (vp_plain - vp0).double()
+    dv_l = (vp_learned - vp0).double()
+    diff = (dv_l - dv_p)
+    moved = (diff != 0).any(1)
+    sr_err = float(torch.sqrt((diff ** 2).sum() / (dv_p ** 2).sum()))
+    sub_err = float(torch.sqrt((diff[moved] ** 2).sum() / (dv_p[moved] ** 2).sum())) if moved.any() else 0.0
+    pp_plain = st_p.get("pp_pairs_ref", {}).get("ms", float("nan"))
+    pp_dense = st_l.get("pp_pairs_ref", {}).get("ms", float("nan"))
+    net_ms = sum(v["ms"] for n_, v in st_l.items() if n_.startswith("learned_"))
+    lay = st_l.get("learned_layers", {})
+    conv_tflops = lay.get("flops", 0.0) / (lay.get("ms", float("nan")) * 1e-3) / 1e12
+    log(f"C4 kick: plain {plain_ms:.1f} ms, learned {learned_ms:.1f} ms; pair kernel {pp_plain:.1f} -> "
+        f"{pp_dense:.1f} ms, network {net_ms:.1f} ms ({conv_tflops:.1f} TFLOP/s in the layers); "
+        f"short-range error {sr_err:.3e}, substituted {sub_err:.3f}, held-out {held_err:.3f}")
     out = {
-        "metric": "C4 learned stand-in: training ms/step, held-out error vs plain PP",
-        "value": round(train_ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": steps, "warmup": 0,
-        "ms_per_step": round(train_ms, 3), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "bf16 (FP32 accumulation and master weights)",
-        "data": "synthetic (four C1-style boxes, 32^3 lattice + 40% Plummer spheres, seeds 10-13; held-out seed 99)",
-        "config": {"workload": "C4: 3-D CNN 4 x 3^3 convolutions, 32 channels, 9^3 count stencil -> PP "
-                               "acceleration of fine cells in PP cells of < 64 particles",
-                   "batch": batch, "training_pairs": int(X.shape[0])},
-        "train_gemm_tflops": round(tflops, 4),
-        "roofline": {"kernel": "k_gemm_bf16 (whole training step)", "bound": "mfma",
-                     "achieved": round(tflops, 4), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(tflops / 2500.0, 6), "traffic": None},
-        "inference_ms": round(infer_ms, 3), "inference_cells": int(hp.shape[0]),
-        "held_out_rel_rms_error": r, "without_network_rel_rms_error": 1.0,
+        "metric": "C4 learned stand-in: learned vs plain kick ms at 256^3, short-range error",
+        "value": round(learned_ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(learned_ms, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (FP32 accumulation; FP32 master weights)",
+        "data": "synthetic (four C1-geometry boxes at z = 10, 3, 1, 0, seeds 10-13; held-out seed 99; "
+                "kicks on the 256^3 headline set, seed 1)",
+        "config": {"workload": "C4: 3-D CNN 4 x 3^3 convolutions, 4 -> 32 -> 32 -> 32 -> 3 channels, "
+                               "fine-cell count + sub-cell offset input, CIC to particles; stands in "
+                               "for PP in level-2 PP cells of < 64 particles",
+                   "batch": batch, "training_samples": int(args.c4_samples), "steps": steps},
+        "kick_ms_plain": round(plain_ms, 3), "kick_ms_learned": round(learned_ms, 3),
+        "speedup": round(plain_ms / learned_ms, 4),
+        "pair_kernel_ms_plain": round(pp_plain, 3), "pair_kernel_ms_learned": round(pp_dense, 3),
+        "sparse_pp_time_share": round((pp_plain - pp_dense) / pp_plain, 4),
+        "network_ms_in_kick": round(net_ms, 3), "network_layers_tflops": round(conv_tflops, 2),
+        "p_acc_plain": int(rp.p_acc), "p_acc_learned": int(rl.p_acc),
+        "short_range_rel_rms_error": sr_err, "substituted_pp_rel_rms_error": sub_err,
+        "held_out_substituted_rel_rms_error": held_err, "without_network_substituted_error": 1.0,
+        "train_ms_per_step": round(train_ms, 3), "train_tflops": round(tflops, 3),
+        "roofline": {"kernel": "k_conv3 (learned layers in the kick)", "bound": "mfma",
+                     "achieved": round(conv_tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(conv_tflops / 2500.0, 5), "traffic": None},
         "loss_first_last": [round(float(np.mean(hist[:50])), 4), round(float(np.mean(hist[-100:])), 4)],
         "cpu_baseline": None,
     }
@@ -266,7 +390,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pp-mode", default="ref", choices=["ref", "exact", "fast", "precise"])
+    ap.add_argument("--pp-mode", default="ref", choices=["ref", "exact", "fast", "precise", "plummer"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
@@ -276,9 +400,13 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-pm2", action="store_true", help="skip the PM2 measurement")
     ap.add_argument("--no-drift", action="store_true", help="skip the drift measurement")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in C ABI timing on host arrays")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c4"],
                     help="c1: the headline kick at 256^3; c2: the 4096-tile batching workload; "
                          "c4: the learned stand-in (training and held-out error)")
+    ap.add_argument("--c4-samples", type=int, default=2_000_000, help="C4 training samples")
+    ap.add_argument("--c4-steps", type=int, default=6000, help="C4 Adam steps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM counters)")
     args = ap.parse_args(argv)
@@ -308,7 +436,7 @@ def main(argv=None) -> int:
     lay = build_layout(pos, vel, C1)
     log(f"inputs: {lay.n_particles} particles, {lay.n_used} slots ({time.perf_counter() - t0:.1f} s)")
     _lib.set_pp_mode({"ref": _lib.PP_REF, "exact": _lib.PP_EXACT, "fast": _lib.PP_FAST,
-                     "precise": _lib.PP_PRECISE}[args.pp_mode])
+                     "precise": _lib.PP_PRECISE, "plummer": _lib.PP_PLUMMER}[args.pp_mode])
     k = ShortRangeKick(lay)
     levels = sorted(set(int(x) for x in k.ires))
     for L in levels:
@@ -365,6 +493,14 @@ def main(argv=None) -> int:
         assert not rg.recorded and (rg.p_cand, rg.p_acc) == (res.p_cand, res.p_acc)
         graph_ms = 1e3 * wall_graph / args.steps
         ms_step = graph_ms
+
+    # the Fortran host's path: the two drop-in calls on host arrays, transfers included
+    dropin = None
+    if not args.no_dropin:
+        greens_host = {L: k.green(L).cpu().numpy() for L in levels}
+        dropin = time_dropin(lay, k, greens_host, mass, amid, dt, max(1, min(args.steps, 3)))
+        del greens_host
+        log(f"drop-in C ABI kick (host arrays): {dropin['ms']:.1f} ms")
 
     # the same layout built on the device from the positions (not part of the kick)
     from cube2_amd.layout import build_layout_device
@@ -449,7 +585,7 @@ def main(argv=None) -> int:
         log(f"drift: {drift['ms']:.1f} ms, next layout {drift['next_layout_ms']:.1f} ms")
 
     pp_name = {"ref": "pp_pairs_ref", "exact": "pp_pairs_exact", "fast": "pp_pairs_fastq",
-               "precise": "pp_pairs_precise"}[args.pp_mode]
+               "precise": "pp_pairs_precise", "plummer": "pp_pairs_plummer"}[args.pp_mode]
     pp_ms = stats[pp_name]["ms"] / args.steps
     flops = pp_flops(res.p_cand, res.p_acc)
     pair_rate = res.p_acc / (pp_ms * 1e-3)
@@ -518,6 +654,7 @@ def main(argv=None) -> int:
         "kick_ms_eager": round(eager_ms, 3),
         "kick_ms_first_call": round(cold_ms, 3),
         "layout_build_ms": round(min(lay_ms), 3),
+        "dropin_abi_kick": dropin,
         "pp_pair_interactions_per_s": float(f"{pair_rate:.4g}"),
         "pp_ms_per_step": round(pp_ms, 3),
         "p_cand": int(res.p_cand),

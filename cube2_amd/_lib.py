@@ -16,11 +16,11 @@ LIB_PATH = Path(__file__).resolve().parent / "libcube2_short.so"
 _LIB = None
 
 OK, ERR_ARG, ERR_GEOMETRY, ERR_CAPACITY, ERR_HIP, ERR_STATE = range(6)
-PP_EXACT, PP_FAST, PP_REF, PP_PRECISE = 0, 1, 2, 3
-TUNE_PP_BLOCKS_PER_CU, TUNE_PP_DIRECT_LG, TUNE_FFT_MIN_LGC, TUNE_INTERP_SEG, TUNE_PP_DENSE_LG = \
-    0, 1, 2, 3, 4
+PP_EXACT, PP_FAST, PP_REF, PP_PRECISE, PP_PLUMMER = 0, 1, 2, 3, 4
+(TUNE_PP_BLOCKS_PER_CU, TUNE_PP_DIRECT_LG, TUNE_FFT_MIN_LGC, TUNE_INTERP_SEG, TUNE_PP_DENSE_LG,
+ TUNE_KICK_OVERLAP, TUNE_PP_OVERLAP_BLOCKS) = range(7)
 TUNE_NAMES = {"pp_blocks_per_cu": 0, "pp_direct_lg": 1, "fft_min_lgc": 2, "interp_seg": 3,
-              "pp_dense_lg": 4}
+              "pp_dense_lg": 4, "kick_overlap": 5, "pp_overlap_blocks": 6}
 
 
 class Cube2Error(RuntimeError):
@@ -45,7 +45,7 @@ class KickResultC(C.Structure):
                 ("napm", C.c_int * 7)]
 
 
-KICK_PM3, KICK_PP, KICK_KEEP_LEVELS, KICK_PM2 = 1, 2, 4, 8
+KICK_PM3, KICK_PP, KICK_KEEP_LEVELS, KICK_PM2, KICK_LEARNED = 1, 2, 4, 8, 16
 
 vp_ = C.c_void_p
 i32 = C.POINTER(C.c_int)
@@ -72,6 +72,14 @@ SIGNATURES = {
                                       C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cube2_gemm_bf16_noalloc_dev": (C.c_int, [vp_, C.c_int64, C.c_int64, vp_, C.c_int64, C.c_int64,
                                               vp_, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cube2_cnn_conv_pack_dev": (C.c_int, [vp_, vp_, C.c_int, C.c_int]),
+    "cube2_cnn_conv3_dev": (C.c_int, [vp_, C.c_int, vp_, vp_, vp_, C.c_int] + [C.c_int] * 5),
+    "cube2_learned_set": (C.c_int, [vp_, vp_, C.c_float, C.c_int]),
+    "cube2_learned_clear": (C.c_int, []),
+    "cube2_learned_inputs_dev": (C.c_int, [vp_, C.c_int, vp_, vp_]),
+    "cube2_learned_samples_dev": (C.c_int, [C.c_int, C.c_int, vp_, vp_, vp_, vp_, vp_]),
+    "cube2_learned_debug_last": (C.c_int, [vp_, vp_, C.c_int, vp_, vp_, vp_, vp_]),
+    "cube2_learned_gather_dev": (C.c_int, [vp_, C.c_int, vp_, vp_, C.c_int, vp_]),
     "cube2_cnn_im2col_dev": (C.c_int, [vp_, vp_, C.c_int, C.c_int, C.c_int]),
     "cube2_cnn_col2im_dev": (C.c_int, [vp_, vp_, C.c_int, C.c_int, C.c_int]),
     "cube2_cnn_bias_act_dev": (C.c_int, [vp_, vp_, C.c_int64, C.c_int, C.c_int]),
@@ -80,6 +88,7 @@ SIGNATURES = {
     "cube2_cnn_adam_dev": (C.c_int, [vp_, vp_, vp_, vp_, vp_, C.c_int64, C.c_float, C.c_float,
                                      C.c_float, C.c_float, C.c_int]),
     "cube2_cnn_mse_dev": (C.c_int, [vp_, vp_, vp_, vp_, C.c_int64]),
+    "cube2_cnn_cic_mse_dev": (C.c_int, [vp_, vp_, vp_, C.c_int, vp_, vp_, vp_]),
     "cube2_select_levels": (C.c_int, [vp_] * 5),
     "cube2_layout_build_dev": (C.c_int, [vp_, vp_, C.c_int64] + [vp_] * 4 + [C.c_int64, vp_]),
     "cube2_drift_dev": (C.c_int, [vp_] * 4 + [C.c_int64, C.c_float, C.c_float] + [vp_] * 4 +

@@ -1,7 +1,9 @@
 // C-ABI entry points: the reference's four bind(C) symbols (host pointers,
 // fatal errors) and the cube2_* device-resident API (status codes).
+#include <chrono>
 #include <cstdlib>
 #include <mutex>
+#include <string>
 
 #include "common.h"
 #include "engine.h"
@@ -64,6 +66,12 @@ void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* rati
                        float dt, float* f2max, float* vmax, double* ptotal, double* ttotal) {
     (void)isort;  // PP updates are disjoint per subtile; order is irrelevant
     std::lock_guard<std::recursive_mutex> l(g_mx);
+    const auto t0 = std::chrono::steady_clock::now();
+    // CUBE_PP_TIMING (pp_force_kernel.c:35-43): 1 a summary line, 2 also one
+    // line per parent tile
+    const char* tenv = std::getenv("CUBE_PP_TIMING");
+    const int timing = tenv ? std::atoi(tenv) : 0;
+    const bool prof = timing >= 1 && call_profile_begin("CUBE_PP_TIMING");
     if (pp_validate(ires, ixyz3, ratio_sf) != CUBE2_OK) die("[PP]", false);
     if (ensure_init() != CUBE2_OK) die("[PP]", false);
     Context& C = ctx();
@@ -84,6 +92,35 @@ void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* rati
         die("[PP]", false);
     }
     C.staged_xp_host = nullptr;  // a new kick re-uploads positions
+    if (prof) {
+        const Geom& g = C.g;
+        const std::string st = call_profile_end();
+        const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        double tsum = 0.0, psum = 0.0;
+        for (int it = 0; it < g.ns3; it++) {
+            tsum += ttotal[it];
+            psum += ptotal[it];
+        }
+        if (timing >= 2) {
+            const int nsub3 = g.p.nns * g.p.nns * g.p.nns;
+            for (int t = 0; t < g.ntile; t++) {
+                double pt = 0.0, tt = 0.0;
+                for (int it = 0; it < g.ns3; it++) {
+                    const int* x = ixyz3 + 6 * it;
+                    if (((x[5] - 1) * g.p.nnt + x[4] - 1) * g.p.nnt + x[3] - 1 != t) continue;
+                    pt += ptotal[it];
+                    tt += ttotal[it];
+                }
+                std::fprintf(stderr, "[PP-GPU-TILE] itile=%d subtiles=%d ncount=%.0f total=%.6fs\n", t,
+                             nsub3, pt, tt);
+            }
+        }
+        std::fprintf(stderr,
+                     "[PP-GPU] total=%.6fs tiles_sum=%.6fs pcount_sum=%.0f f2max=%.6e "
+                     "vmax=(%.6e, %.6e, %.6e) stages:%s\n",
+                     wall, tsum, psum, (double)*f2max, (double)vmax[0], (double)vmax[1],
+                     (double)vmax[2], st.c_str());
+    }
 }
 
 void c_pm3_force_kernel(int* isort, int* ires, int* ixyz3, int* ratio_sf, int* rhoc,
@@ -92,11 +129,19 @@ void c_pm3_force_kernel(int* isort, int* ires, int* ixyz3, int* ratio_sf, int* r
                         float* vmax, double* ptotal) {
     (void)vp;  // the Delta-v stays pending until c_pp_force_kernel (pm3_fft_cuda.cu:519)
     std::lock_guard<std::recursive_mutex> l(g_mx);
+    const auto t0 = std::chrono::steady_clock::now();
+    // CUBE_PM3_PROFILE (pm3_fft_cuda.cu:522-523,592-619): one line of stage times
+    const bool prof = call_profile_begin("CUBE_PM3_PROFILE");
     if (ensure_init() != CUBE2_OK) die("PM3", true);
     Context& C = ctx();
     const Geom& g = C.g;
     int64_t n = 0;
-    if (stage_inputs(rhoc, idx_b_r, xp, false, &n) != CUBE2_OK) die("PM3", true);
+    if (prof) {
+        Stage st("xp_h2d", 0.0);
+        if (stage_inputs(rhoc, idx_b_r, xp, false, &n) != CUBE2_OK) die("PM3", true);
+    } else if (stage_inputs(rhoc, idx_b_r, xp, false, &n) != CUBE2_OK) {
+        die("PM3", true);
+    }
     const float* host_g[5] = {g2, g4, g6, g8, g12};
     const float* dev_g[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     bool need[5] = {false, false, false, false, false};
@@ -113,6 +158,21 @@ void c_pm3_force_kernel(int* isort, int* ires, int* ixyz3, int* ratio_sf, int* r
                   C.d_xp.as<int16_t>(), n, dev_g, mass, amid, dt, f2max, ptotal) != CUBE2_OK)
         die("PM3", true);
     vmax[0] = vmax[1] = vmax[2] = 0.0f;  // pm3_fft_cuda.cu:552
+    if (prof) {
+        const std::string st = call_profile_end();
+        int napm[7] = {0, 0, 0, 0, 0, 0, 0};
+        double pt = 0.0;
+        for (int it = 0; it < g.ns3; it++) {
+            if (ires[it] >= 1 && ires[it] <= 7) napm[ires[it] - 1]++;
+            pt += ptotal[it];
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr,
+                     "[PM3 profile] subtiles L2..L6=%d/%d/%d/%d/%d particles=%.0f slots=%lld "
+                     "total=%.2fms f2max=%.6e stages:%s\n",
+                     napm[1], napm[2], napm[3], napm[4], napm[5], pt, (long long)n, ms,
+                     (double)*f2max, st.c_str());
+    }
 }
 
 void c_pm3_fft_finalize(void) {
@@ -131,6 +191,7 @@ void c_gpu_velocity_finalize(void) {
     layout_release();
     drift_release();
     cnn_release();
+    learned_release();
     Context& C = ctx();
     C.dv64.release();
     C.d_vp.release();
@@ -197,6 +258,98 @@ int cube2_gemm_bf16_noalloc_dev(const uint16_t* A, int64_t sam, int64_t sak, con
                                 int K, int acc) {
     return cnn_call([&] { return cnn_gemm_noalloc(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, acc); });
 }
+int cube2_cnn_conv_pack_dev(const float* W, uint16_t* Wp, int cin, int cout) {
+    return cnn_call([&] { return ensure_init() != CUBE2_OK ? CUBE2_ERR_HIP : cnn_conv_pack(W, Wp, cin, cout, ctx().stream); });
+}
+int cube2_cnn_conv3_dev(const void* X, int x_bf16, const uint16_t* Wp, const float* bias, void* Y,
+                        int y_bf16, int B, int D, int cin, int cout, int relu) {
+    return cnn_call([&] {
+        C2_TRY(ensure_init());
+        return cnn_conv3(X, x_bf16, Wp, bias, Y, y_bf16, B, D, cin, cout, relu, ctx().stream);
+    });
+}
+int cube2_learned_set(const float* const* W, const float* const* b, float y_scale, int occupancy) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    return learned_model_load(W, b, y_scale, occupancy);
+}
+
+int cube2_learned_clear(void) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    learned_release();
+    buffers_changed();
+    return CUBE2_OK;
+}
+
+int cube2_learned_inputs_dev(float* d_X, int cap_jobs, int* njobs, int* DI) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    const PPPlan* P = nullptr;
+    PPArgs A{};
+    const int* tf = nullptr;
+    C2_TRY(pp_last(&P, &A, &tf));
+    LearnedPlan Lp;
+    C2_TRY(learned_prepare(ctx().g, P->iloc, P->ratio_sf.data(), P->apm3.data(), Lp, 1));
+    if (njobs) *njobs = (int)Lp.jobs.size();
+    if (DI) *DI = Lp.DI;
+    if (!d_X) return CUBE2_OK;  // a size query
+    if (cap_jobs < (int)Lp.jobs.size()) {
+        set_error("learned inputs: %zu level-2 subtiles, room for %d", Lp.jobs.size(), cap_jobs);
+        return CUBE2_ERR_CAPACITY;
+    }
+    C2_TRY(learned_training_inputs(Lp, A, tf, d_X, ctx().stream));
+    C2_HIP(hipStreamSynchronize(ctx().stream));
+    return CUBE2_OK;
+}
+
+int cube2_learned_samples_dev(int occupancy, int cap, int* d_job, int* d_base, float* d_frac,
+                              uint32_t* d_slot, int* n) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    if (occupancy < 1 || cap < 0 || !n || (cap > 0 && (!d_job || !d_base || !d_frac || !d_slot))) {
+        set_error("learned samples: bad arguments");
+        return CUBE2_ERR_ARG;
+    }
+    const PPPlan* P = nullptr;
+    PPArgs A{};
+    const int* tf = nullptr;
+    C2_TRY(pp_last(&P, &A, &tf));
+    LearnedPlan Lp;
+    C2_TRY(learned_prepare(ctx().g, P->iloc, P->ratio_sf.data(), P->apm3.data(), Lp, occupancy));
+    DevBuf cnt;
+    C2_TRY(cnt.reserve(sizeof(int)));
+    C2_TRY(learned_training_samples(Lp, A, cap, cnt.as<int>(), d_job, d_base, d_frac, d_slot,
+                                    ctx().stream));
+    C2_HIP(hipMemcpyAsync(n, cnt.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx().stream));
+    C2_HIP(hipStreamSynchronize(ctx().stream));
+    cnt.release();
+    if (*n > cap) {
+        set_error("learned samples: %d substitutable particles, room for %d", *n, cap);
+        return CUBE2_ERR_CAPACITY;
+    }
+    return CUBE2_OK;
+}
+
+int cube2_learned_debug_last(float* d_X, float* d_O, int cap_jobs, int* j0, int* nj, int* DI,
+                             int* PO) {
+    std::lock_guard<std::recursive_mutex> l(g_mx);
+    if (!j0 || !nj || !DI || !PO) {
+        set_error("learned debug: null argument");
+        return CUBE2_ERR_ARG;
+    }
+    C2_TRY(ensure_init());
+    return learned_debug_last(d_X, d_O, cap_jobs, j0, nj, DI, PO, ctx().stream);
+}
+
+int cube2_learned_gather_dev(const float* d_X, int DI, const int* d_job, const int* d_base, int n,
+                             float* d_out) {
+    return cnn_call([&] {
+        C2_TRY(ensure_init());
+        if (!d_X || DI < 10 || n < 0 || (n > 0 && (!d_job || !d_base || !d_out))) {
+            set_error("learned gather: bad arguments");
+            return (int)CUBE2_ERR_ARG;
+        }
+        return learned_gather(d_X, DI, d_job, d_base, n, d_out, ctx().stream);
+    });
+}
+
 int cube2_cnn_im2col_dev(const float* X, uint16_t* col, int B, int D, int Cin) {
     return cnn_call([&] { return cnn_im2col(X, col, B, D, Cin); });
 }
@@ -219,6 +372,10 @@ int cube2_cnn_adam_dev(float* w, const float* g, float* m, float* v, uint16_t* w
 }
 int cube2_cnn_mse_dev(const float* out, const float* target, float* dout, float* loss, int64_t n) {
     return cnn_call([&] { return cnn_mse(out, target, dout, loss, n); });
+}
+int cube2_cnn_cic_mse_dev(const float* out8, const float* frac, const float* target, int n,
+                          float* pred, float* dout8, float* loss) {
+    return cnn_call([&] { return cnn_cic_mse(out8, frac, target, n, pred, dout8, loss); });
 }
 
 int cube2_fft3d_dev(float* d_data, int n, int nb, int dir) {

@@ -223,6 +223,221 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out,
     if (threadIdx.x == 0) *loss = red[0] / (float)n;
 }
 
+// ------------------------------------------------------ 3^3 convolution
+// Y[b][z][y][x][o] = act(bias[o] + sum over taps t = (dz, dy, dx) and input
+// channels c of X[b][z+dz][y+dy][x+dx][c] * W[t*CIN + c][o]) for the
+// P^3 = (D-2)^3 outputs of each of B inputs of D^3 x CIN cells
+// (channels-last), on v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32
+// accumulation) with no materialised im2col.
+//
+// A workgroup (4 wavefronts) is persistent over output blocks of
+// 4 (x) x 4 (y) x 8 (z) positions.  For each block it stages the 6 x 6 x 10
+// input cells as BF16 in LDS; wavefront w computes the z-planes 2w and 2w+1
+// (two 16-row tiles, the 4 x 4 positions of a plane) against every output
+// column.  The packed weights Wp[tap][out][in] (BF16) are loaded into LDS
+// once per workgroup.  MFMA operands: lane l holds row (position) l % 16 of
+// A and column (output channel) l % 16 of B with K indices 8 (l / 16) + 0..7;
+// the result lane holds rows 4 (l / 16) + 0..3 of column l % 16.  K runs over
+// (tap, channel) with the channel fastest: for CIN = 32 one K step is one
+// tap; for CIN = 4 a step is 8 taps (taps 27..31 are zeros).
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+constexpr int kConvBx = 4, kConvBy = 4, kConvBz = 8;
+constexpr int kConvIx = kConvBx + 2, kConvIy = kConvBy + 2, kConvIz = kConvBz + 2;
+constexpr int kConvCells = kConvIx * kConvIy * kConvIz;  // 360 staged input cells
+
+template <int CIN>
+constexpr int conv_taps_padded() { return CIN == 32 ? 27 : 32; }
+
+template <int CIN, int COUTP>
+constexpr size_t conv_lds_bytes() {
+    return (size_t)conv_taps_padded<CIN>() * COUTP * CIN * 2 + (size_t)kConvCells * CIN * 2;
+}
+
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <class T>
+__device__ inline float load_f32(const T* p);
+template <>
+__device__ inline float load_f32<float>(const float* p) { return *p; }
+template <>
+__device__ inline float load_f32<uint16_t>(const uint16_t* p) { return bf16_to_f32(*p); }
+
+template <class T>
+__device__ inline void store_act(T* p, float v);
+template <>
+__device__ inline void store_act<float>(float* p, float v) { *p = v; }
+template <>
+__device__ inline void store_act<uint16_t>(uint16_t* p, float v) { *p = to_bf16(v); }
+
+template <int CIN, int COUTP, class TIn, class TOut>
+__global__ __launch_bounds__(256) void k_conv3(const TIn* __restrict__ X,
+                                               const uint16_t* __restrict__ Wp,
+                                               const float* __restrict__ bias,
+                                               TOut* __restrict__ Y, int B, int D, int cout,
+                                               int relu) {
+    extern __shared__ __align__(16) uint16_t conv_lds[];
+    constexpr int TAPS = conv_taps_padded<CIN>();
+    uint16_t* wl = conv_lds;                         // [TAPS][COUTP][CIN]
+    uint16_t* xl = conv_lds + TAPS * COUTP * CIN;    // [cell][CIN]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(Wp);
+        uint4* dst = reinterpret_cast<uint4*>(wl);
+        for (int i = tid; i < TAPS * COUTP * CIN / 8; i += 256) dst[i] = src[i];
+    }
+    const int P = D - 2;
+    const int nbx = (P + kConvBx - 1) / kConvBx, nby = (P + kConvBy - 1) / kConvBy,
+              nbz = (P + kConvBz - 1) / kConvBz;
+    const int64_t per = (int64_t)nbx * nby * nbz, nblk = per * B;
+    const int r16 = lane & 15, kq = lane >> 4;
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int b = (int)(blk / per);
+        const int64_t rr = blk - (int64_t)b * per;
+        const int bx = (int)(rr % nbx), by = (int)((rr / nbx) % nby), bz = (int)(rr / ((int64_t)nbx * nby));
+        const int x0 = bx * kConvBx, y0 = by * kConvBy, z0 = bz * kConvBz;
+        __syncthreads();  // the previous block's reads of xl are done (and wl is loaded)
+        // stage the input cells as BF16 (zeros outside the grid), 4 channels per item
+        for (int i = tid; i < kConvCells * (CIN / 4); i += 256) {
+            const int cell = i / (CIN / 4), q = i % (CIN / 4);
+            const int cx = cell % kConvIx, cy = (cell / kConvIx) % kConvIy, cz = cell / (kConvIx * kConvIy);
+            const int gx = x0 + cx, gy = y0 + cy, gz = z0 + cz;
+            uint16_t h[4] = {0, 0, 0, 0};
+            if (gx < D && gy < D && gz < D) {
+                const TIn* src = X + ((((int64_t)b * D + gz) * D + gy) * D + gx) * CIN + 4 * q;
+#pragma unroll
+                for (int j = 0; j < 4; j++) h[j] = to_bf16(load_f32<TIn>(src + j));
+            }
+            uint2 v;
+            v.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+            v.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+            *reinterpret_cast<uint2*>(xl + cell * CIN + 4 * q) = v;
+        }
+        __syncthreads();
+        v4f acc[2][COUTP / 16];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int ct = 0; ct < COUTP / 16; ct++) acc[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+        const int xr = r16 & 3, yr = r16 >> 2;
+        constexpr int STEPS = TAPS * CIN / 32;
+        for (int s = 0; s < STEPS; s++) {
+            v8bf a[2], bm[COUTP / 16];
+            if (CIN == 32) {
+                const int t = s, dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++) {
+                    const int zl = 2 * wave + rt;
+                    const int cell = ((zl + dz) * kConvIy + yr + dy) * kConvIx + xr + dx;
+                    a[rt] = *reinterpret_cast<const v8bf*>(xl + cell * CIN + 8 * kq);
+                }
+#pragma unroll
+                for (int ct = 0; ct < COUTP / 16; ct++)
+                    bm[ct] = *reinterpret_cast<const v8bf*>(wl + ((t * COUTP + ct * 16 + r16) * CIN + 8 * kq));
+            } else {  // CIN == 4: taps 8s + 2kq and 8s + 2kq + 1, four channels each
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int t = 8 * s + 2 * kq + h;
+                    const bool tv = t < 27;
+                    const int tt = tv ? t : 0;
+                    const int dz = tt / 9, dy = (tt / 3) % 3, dx = tt % 3;
+#pragma unroll
+                    for (int rt = 0; rt < 2; rt++) {
+                        const int zl = 2 * wave + rt;
+                        const int cell = ((zl + dz) * kConvIy + yr + dy) * kConvIx + xr + dx;
+                        v4bf q = *reinterpret_cast<const v4bf*>(xl + cell * CIN);
+                        if (!tv) q = v4bf{};
+#pragma unroll
+                        for (int j = 0; j < 4; j++) a[rt][4 * h + j] = q[j];
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < COUTP / 16; ct++) {
+                        const v4bf q = *reinterpret_cast<const v4bf*>(wl + (t * COUTP + ct * 16 + r16) * CIN);
+#pragma unroll
+                        for (int j = 0; j < 4; j++) bm[ct][4 * h + j] = q[j];
+                    }
+                }
+            }
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < COUTP / 16; ct++)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], bm[ct], acc[rt][ct], 0, 0, 0);
+        }
+        // epilogue: bias, activation, store the valid positions and channels
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++) {
+            const int gz = z0 + 2 * wave + rt;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = 4 * kq + i;
+                const int gx = x0 + (r & 3), gy = y0 + (r >> 2);
+                if (gx >= P || gy >= P || gz >= P) continue;
+                TOut* out = Y + ((((int64_t)b * P + gz) * P + gy) * P + gx) * cout;
+#pragma unroll
+                for (int ct = 0; ct < COUTP / 16; ct++) {
+                    const int o = ct * 16 + r16;
+                    if (o >= cout) continue;
+                    float v = acc[rt][ct][i] + bias[o];
+                    if (relu) v = fmaxf(v, 0.f);
+                    store_act<TOut>(out + o, v);
+                }
+            }
+        }
+    }
+}
+
+// Wp[t][o][c] = BF16(W[t*cin + c][o]) for t < 27, o < cout, zero elsewhere
+// ([taps_padded][coutp][cin]).
+__global__ void k_conv_pack(const float* __restrict__ W, uint16_t* __restrict__ Wp, int cin,
+                            int cout, int taps, int coutp) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= taps * coutp * cin) return;
+    const int c = e % cin, o = (e / cin) % coutp, t = e / (cin * coutp);
+    Wp[e] = (t < 27 && o < cout) ? to_bf16(W[(t * cin + c) * cout + o]) : (uint16_t)0;
+}
+
+// Cloud-in-cell combination of the 2^3 output cells of each sample with its
+// fractions f (pred = sum over corners of prod_d (f_d or 1 - f_d) out), the
+// mean squared error against target over n*3 values and its gradient with
+// respect to the 2^3 outputs.  One block.
+__global__ __launch_bounds__(256) void k_cic_mse(const float* __restrict__ out8,
+                                                 const float* __restrict__ frac,
+                                                 const float* __restrict__ target, int n,
+                                                 float* __restrict__ pred,
+                                                 float* __restrict__ dout8,
+                                                 float* __restrict__ loss) {
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const float f0 = frac[3 * i], f1 = frac[3 * i + 1], f2 = frac[3 * i + 2];
+        float w[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            w[c] = ((c & 1) ? f0 : 1.f - f0) * ((c & 2) ? f1 : 1.f - f1) * ((c & 4) ? f2 : 1.f - f2);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            float p = 0.f;
+#pragma unroll
+            for (int c = 0; c < 8; c++) p += w[c] * out8[(8 * i + c) * 3 + d];
+            if (pred) pred[3 * i + d] = p;
+            const float r = p - target[3 * i + d];
+            s += r * r;
+            const float g = 2.f * r / (3.f * (float)n);
+#pragma unroll
+            for (int c = 0; c < 8; c++) dout8[(8 * i + c) * 3 + d] = w[c] * g;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = red[0] / (3.f * (float)n);
+}
+
 static int cnn_check(bool ok, const char* what) {
     if (ok) return CUBE2_OK;
     set_error("cnn: bad arguments to %s", what);
@@ -332,6 +547,70 @@ int cnn_gemm_noalloc(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t
 
 void cnn_release() { g_partials.release(); }
 
+// ---------------------------------------------------------- convolution
+int cnn_conv_pack(const float* W, uint16_t* Wp, int cin, int cout, hipStream_t s) {
+    C2_TRY(cnn_check(W && Wp && (cin == 4 || cin == 32) && (cout == 32 || (cout == 3 && cin == 32)),
+                     "conv_pack"));
+    const int taps = cin == 32 ? 27 : 32, coutp = cout == 3 ? 16 : 32;
+    const int n = taps * coutp * cin;
+    k_conv_pack<<<blocks_for(n), 256, 0, s>>>(W, Wp, cin, cout, taps, coutp);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+size_t cnn_conv_packed_count(int cin, int cout) {
+    return (size_t)(cin == 32 ? 27 : 32) * (cout == 3 ? 16 : 32) * cin;
+}
+
+template <int CIN, int COUTP, class TIn, class TOut>
+static int conv_launch(const void* X, const uint16_t* Wp, const float* bias, void* Y, int B, int D,
+                       int cout, int relu, hipStream_t s) {
+    auto kern = k_conv3<CIN, COUTP, TIn, TOut>;
+    const size_t lds = conv_lds_bytes<CIN, COUTP>();
+    static int per_cu = 0;  // resident workgroups per CU (one value per instantiation)
+    if (!per_cu) {
+        C2_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        int per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, lds) != hipSuccess || per < 1)
+            per = 1;
+        per_cu = per;
+    }
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    const int P = D - 2;
+    const int64_t nblk = (int64_t)B * ((P + kConvBx - 1) / kConvBx) * ((P + kConvBy - 1) / kConvBy) *
+                         ((P + kConvBz - 1) / kConvBz);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ncu * per_cu));
+    kern<<<grid, 256, lds, s>>>(static_cast<const TIn*>(X), Wp, bias, static_cast<TOut*>(Y), B, D,
+                                cout, relu);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+// x_bf16 / y_bf16: activations stored as BF16 (else FP32).
+int cnn_conv3(const void* X, int x_bf16, const uint16_t* Wp, const float* bias, void* Y, int y_bf16,
+              int B, int D, int cin, int cout, int relu, hipStream_t s) {
+    C2_TRY(cnn_check(X && Wp && bias && Y && B >= 0 && D >= 3, "conv3"));
+    if (B == 0) return CUBE2_OK;
+    if (cin == 4 && cout == 32 && !x_bf16)
+        return y_bf16 ? conv_launch<4, 32, float, uint16_t>(X, Wp, bias, Y, B, D, cout, relu, s)
+                      : conv_launch<4, 32, float, float>(X, Wp, bias, Y, B, D, cout, relu, s);
+    if (cin == 32 && cout == 32)
+        return x_bf16 ? (y_bf16 ? conv_launch<32, 32, uint16_t, uint16_t>(X, Wp, bias, Y, B, D, cout, relu, s)
+                                : conv_launch<32, 32, uint16_t, float>(X, Wp, bias, Y, B, D, cout, relu, s))
+                      : (y_bf16 ? conv_launch<32, 32, float, uint16_t>(X, Wp, bias, Y, B, D, cout, relu, s)
+                                : conv_launch<32, 32, float, float>(X, Wp, bias, Y, B, D, cout, relu, s));
+    if (cin == 32 && cout == 3 && !y_bf16)
+        return x_bf16 ? conv_launch<32, 16, uint16_t, float>(X, Wp, bias, Y, B, D, cout, relu, s)
+                      : conv_launch<32, 16, float, float>(X, Wp, bias, Y, B, D, cout, relu, s);
+    set_error("conv3: no kernel for %d -> %d channels (x %s, y %s)", cin, cout,
+              x_bf16 ? "bf16" : "f32", y_bf16 ? "bf16" : "f32");
+    return CUBE2_ERR_ARG;
+}
+
 int cnn_im2col(const float* X, uint16_t* col, int B, int D, int Cin) {
     C2_TRY(cnn_check(X && col && B >= 0 && D >= 3 && Cin >= 1, "im2col"));
     C2_TRY(ensure_init());
@@ -385,6 +664,15 @@ int cnn_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t
     C2_TRY(ensure_init());
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
     if (n) k_adam<<<blocks_for(n), 256, 0, ctx().stream>>>(w, g, m, v, wb, n, lr, b1, b2, eps, c1, c2);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
+int cnn_cic_mse(const float* out8, const float* frac, const float* target, int n, float* pred,
+                float* dout8, float* loss) {
+    C2_TRY(cnn_check(out8 && frac && target && dout8 && loss && n >= 1, "cic_mse"));
+    C2_TRY(ensure_init());
+    k_cic_mse<<<1, 256, 0, ctx().stream>>>(out8, frac, target, n, pred, dout8, loss);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

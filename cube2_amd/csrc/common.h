@@ -87,6 +87,13 @@ struct Context {
     Geom g{};
     int device = -1;
     hipStream_t stream = nullptr;
+    // second library stream: PP's pass beside PM3 in an overlapped kick
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // stream the stage profiler records on (nullptr: stream)
+    hipStream_t stage_stream = nullptr;
+    // set while one kick's PM3 and PP share a cell-base table computed up front
+    bool nzero_shared = false;
     int pp_mode = CUBE2_PP_REF;
     // velocity accumulator (dv64, FP64, 3 per slot) shared by PM3 and PP
     DevBuf dv64;
@@ -98,7 +105,7 @@ struct Context {
     int64_t staged_xp_n = -1;
     DevBuf d_scal;
     HostBuf h_scal;
-    int tune[CUBE2_TUNE_COUNT] = {0, 0, 0, 0, 0};  // 0: built-in choice
+    int tune[CUBE2_TUNE_COUNT] = {};  // 0: built-in choice
     // true while work is being recorded into a HIP graph: no host syncs,
     // allocations, memsets or profiler events may be issued
     bool capturing = false;
@@ -113,6 +120,17 @@ std::recursive_mutex& api_mutex();
 // from a pinned host block, which a replay reads at launch time.
 int scalars_upload(float mass, float a_mid, float dt, hipStream_t s, const float** dev);   // params + HIP device + library stream
 void ensure_params();  // params only (no HIP call)
+
+// Blocks per CU of the pair kernel while PM3 runs beside it (built-in
+// choice of CUBE2_TUNE_PP_OVERLAP_BLOCKS, capped at the occupancy maximum).
+// Measured at 256^3 (kick ms, graph replay; serial 145.5): 1: 248.5, 2: 205.4,
+// 3: 161.4, 4: 136.8, 5: 126.2, 6 (the maximum): 125.1.  The mesh kernels'
+// workgroups take the slots of pair-kernel workgroups as those retire, so
+// leaving slots empty only slows the pair kernel.
+constexpr int kPPOverlapBlocks = 16;
+
+// Whether the kick driver overlaps PP with PM3 (CUBE2_TUNE_KICK_OVERLAP).
+inline bool kick_overlap() { return ctx().tune[CUBE2_TUNE_KICK_OVERLAP] != 2; }
 
 // ------------------------------------------------------------- helpers
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

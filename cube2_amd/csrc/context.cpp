@@ -1,6 +1,7 @@
 // Library state: geometry, stream, errors, grow-only device buffers and
 // the per-stage event profiler.
 #include <cstdarg>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 
@@ -144,7 +145,14 @@ int ensure_init() {
             return CUBE2_ERR_HIP;
         }
         C2_HIP(hipGetDevice(&c.device));
-        C2_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        // Blocking streams (not hipStreamNonBlocking): work on them is ordered
+        // after work a caller queued on the default (null) stream, as a
+        // PyTorch caller's tensor writes are, and default-stream work after
+        // theirs; the two library streams still run beside each other.
+        C2_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamDefault));
+        C2_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamDefault));
+        C2_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+        C2_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     }
     return CUBE2_OK;
 }
@@ -184,7 +192,8 @@ Stage::Stage(const char* name, double bytes, double flops) {
     bytes_ = bytes;
     flops_ = flops;
     a_ = get_event();
-    (void)hipEventRecord(a_, ctx().stream);
+    s_ = ctx().stage_stream ? ctx().stage_stream : ctx().stream;
+    (void)hipEventRecord(a_, s_);
 }
 
 void Stage::set_work(double bytes, double flops) {
@@ -195,7 +204,7 @@ void Stage::set_work(double bytes, double flops) {
 Stage::~Stage() {
     if (!on_) return;
     hipEvent_t b = get_event();
-    (void)hipEventRecord(b, ctx().stream);
+    (void)hipEventRecord(b, s_);
     g_pending.push_back({name_, a_, b, bytes_, flops_});
 }
 
@@ -221,6 +230,35 @@ void profile_add(const char* stage, double ms, double bytes, double flops) {
     s.n += 1;
     s.bytes += bytes;
     s.flops += flops;
+}
+
+// Stage profile of one drop-in call, switched on by an environment variable
+// as the reference's CUBE_PM3_PROFILE / CUBE_PP_TIMING: the caller's
+// accumulated stage statistics are set aside for the call and restored.
+static std::map<std::string, Stat> g_saved_stats;
+static bool g_saved_prof = false;
+
+bool call_profile_begin(const char* env) {
+    if (!std::getenv(env)) return false;
+    g_saved_prof = g_prof;
+    g_saved_stats.swap(g_stats);
+    g_stats.clear();
+    g_prof = true;
+    return true;
+}
+
+std::string call_profile_end() {
+    profile_flush();
+    std::string out;
+    char buf[96];
+    for (const auto& kv : g_stats) {
+        std::snprintf(buf, sizeof(buf), " %s=%.3fms", kv.first.c_str(), kv.second.ms);
+        out += buf;
+    }
+    g_stats.swap(g_saved_stats);
+    g_saved_stats.clear();
+    g_prof = g_saved_prof;
+    return out;
 }
 
 int sync_and_flush() {
@@ -266,8 +304,8 @@ int cube2_set_params(const cube2_params_t* p) {
 }
 
 int cube2_set_tuning(int key, int value) {
-    static const int lo[CUBE2_TUNE_COUNT] = {1, 1, 1, 1, 3},
-                     hi[CUBE2_TUNE_COUNT] = {16, 7, 6, 32, 6};
+    static const int lo[CUBE2_TUNE_COUNT] = {1, 1, 1, 1, 3, 1, 1},
+                     hi[CUBE2_TUNE_COUNT] = {16, 7, 6, 32, 6, 2, 16};
     if (key < 0 || key >= CUBE2_TUNE_COUNT || (value != 0 && (value < lo[key] || value > hi[key]))) {
         set_error("tuning key %d value %d out of range", key, value);
         return CUBE2_ERR_ARG;
@@ -286,7 +324,7 @@ int cube2_get_tuning(int key, int* value) {
         set_error("bad tuning key %d", key);
         return CUBE2_ERR_ARG;
     }
-    static const int builtin[CUBE2_TUNE_COUNT] = {0, 2, 0, 4, 4};
+    static const int builtin[CUBE2_TUNE_COUNT] = {0, 2, 0, 4, 4, 1, kPPOverlapBlocks};
     const int v = ctx().tune[key];
     *value = v ? v : builtin[key];
     if (key == CUBE2_TUNE_PP_BLOCKS_PER_CU && !v) *value = 0;  // occupancy-derived at launch
@@ -305,7 +343,7 @@ int cube2_get_params(cube2_params_t* p) {
 }
 
 int cube2_set_pp_mode(int mode) {
-    if (mode < CUBE2_PP_EXACT || mode > CUBE2_PP_PRECISE) {
+    if (mode < CUBE2_PP_EXACT || mode > CUBE2_PP_PLUMMER) {
         set_error("unknown PP mode %d", mode);
         return CUBE2_ERR_ARG;
     }

@@ -6,11 +6,22 @@
 //              (no host synchronisation: recordable into a HIP graph);
 //   *_finish:  after the stream has completed, host-side results and errors.
 #pragma once
+#include <array>
+
 #include "common.h"
+#include "pp_kernels.h"
 
 namespace cube2 {
 
 // ----------------------------------------------------------------- PP
+// Host plan of the learned stand-in for one PP call (learned.hip): the
+// level-2 subtiles ("jobs"), the grid sizes (input DI^3, output PO^3 fine
+// cells per job) and the jobs per pass of the layers.
+struct LearnedPlan {
+    std::vector<LearnedJob> jobs;
+    LearnedArgs args{};
+    int nfs = 0, DI = 0, PO = 0, batch = 1;
+};
 struct PPPlan {
     std::vector<int> iloc, owner;  // subtile location -> level / -> subtile index
     int ntile = 0, nsub3 = 0;
@@ -19,18 +30,49 @@ struct PPPlan {
     int mode = 0;      // CUBE2_PP_*
     int grid_pairs = 0;  // blocks of the persistent pair kernel
     bool count_tiles = false;  // count pairs per parent tile
+    bool split = false;        // pair sums apart from the velocity update (pp_enqueue_apply)
+    bool learned = false;      // the network stands in for PP in sparse level-2 cells
+    LearnedPlan lp;
+    std::array<int, 7> ratio_sf{};
+    std::array<float, 7> apm3{};
 };
 int pp_validate(const int* ires, const int* ixyz3, const int* ratio_sf);
 int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
-               int64_t n_used, PPPlan& P);
+               int64_t n_used, PPPlan& P, bool split = false, bool learned = false);
 int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp,
                float* d_vp, const float* d_scal, const double* dv64, hipStream_t s);
+// After a split pp_enqueue and whatever else writes dv64 (PM3), on stream s:
+// the velocity update and vmax, then the result copies.
+int pp_enqueue_apply(const PPPlan& P, float* d_vp, const double* dv64, hipStream_t s);
 int pp_finish(const PPPlan& P, float* f2max, float* vmax, double* ptotal, double* ttotal,
               uint64_t* pair_counts, double wall_s);
 int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
              int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
-             double* ptotal, double* ttotal, uint64_t* pair_counts);
+             double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned = false);
+// Blocks of the persistent pair kernel (overlap: beside PM3).
+int pp_tiled_grid(int mode, bool overlap);
+// The last PP call's sorted particles and cell lists (for the learned
+// stand-in's training inputs); CUBE2_ERR_STATE if there is none.
+int pp_last(const PPPlan** P, PPArgs* A, const int** tile_first);
+
+// Learned stand-in (learned.hip).
+bool learned_model_set();
+int learned_model_load(const float* const* W, const float* const* b, float y_scale, int occupancy);
+int learned_model_occupancy();
+void learned_release();
+int learned_prepare(const Geom& g, const std::vector<int>& iloc, const int* ratio_sf,
+                    const float* apm3, LearnedPlan& Lp, int occupancy);
+int learned_reserve(const LearnedPlan& Lp);
+int learned_enqueue(const LearnedPlan& Lp, const PPArgs& A, const int* tile_first, hipStream_t s);
+int learned_training_inputs(const LearnedPlan& Lp, const PPArgs& A, const int* tile_first, float* X,
+                            hipStream_t s);
+int learned_training_samples(const LearnedPlan& Lp, const PPArgs& A, int cap, int* d_count,
+                             int* job, int* base, float* frac, uint32_t* slot, hipStream_t s);
+int learned_gather(const float* X, int DI, const int* job, const int* base, int n, float* out,
+                   hipStream_t s);
+int learned_debug_last(float* X, float* O, int cap_jobs, int* j0, int* nj, int* DI, int* PO,
+                       hipStream_t s);
 int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t* slot,
                       int32_t* cell_start);
 void pp_release();
@@ -81,6 +123,12 @@ int cnn_gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int
 int cnn_gemm_noalloc(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
                      int64_t sbn, float* C, int64_t ldc, int M, int N, int K, int acc);
 void cnn_release();
+// 3^3 valid convolution on the matrix cores (cin 4 or 32; cout 32, or 3
+// from 32); Wp packed by cnn_conv_pack (cnn_conv_packed_count entries).
+int cnn_conv_pack(const float* W, uint16_t* Wp, int cin, int cout, hipStream_t s);
+size_t cnn_conv_packed_count(int cin, int cout);
+int cnn_conv3(const void* X, int x_bf16, const uint16_t* Wp, const float* bias, void* Y, int y_bf16,
+              int B, int D, int cin, int cout, int relu, hipStream_t s);
 int cnn_im2col(const float* X, uint16_t* col, int B, int D, int Cin);
 int cnn_col2im(const float* dcol, float* dX, int B, int D, int Cin);
 int cnn_bias_act(float* Y, const float* bias, int64_t rows, int C, int relu);
@@ -90,6 +138,8 @@ int cnn_to_bf16(const float* x, uint16_t* y, int64_t n);
 int cnn_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n, float lr,
              float b1, float b2, float eps, int t);
 int cnn_mse(const float* out, const float* target, float* dout, float* loss, int64_t n);
+int cnn_cic_mse(const float* out8, const float* frac, const float* target, int n, float* pred,
+                float* dout8, float* loss);
 int fft3d(float* d, int n, int nb, int dir);
 void pm3_release();
 int green_cache_get(const float* host, size_t count, const float** dev);

@@ -572,23 +572,28 @@ struct GreenTiled {
 };
 static std::vector<GreenTiled> g_green_tiled;
 
+// The tiled copy is rewritten from G on every call (a plan's preparation):
+// a caller's table at a given device address can change (a freed table's
+// memory reused for another), so the copy is never trusted across calls;
+// only its allocation is kept per (address, n, tile width).
 int green_tiled(const float* G, int n, const float** out, hipStream_t s) {
     const int lgC = fft_cols_lgC(n), C = 1 << lgC;
-    for (auto& t : g_green_tiled)
-        if (t.src == G && t.n == n && t.lgC == lgC) {
-            *out = t.dev;
-            return CUBE2_OK;
-        }
     const int ntiles = (n / 2 + 1 + C - 1) / C;
     const size_t tot = (size_t)n * ntiles * n * C;
-    GreenTiled t{G, n, lgC, nullptr};
-    buffers_changed();
-    C2_HIP(hipMalloc(&t.dev, tot * sizeof(float)));
-    k_green_tile<<<ceil_div(tot, 256), 256, 0, s>>>(G, t.dev, n, lgC, ntiles);
+    GreenTiled* hit = nullptr;
+    for (auto& t : g_green_tiled)
+        if (t.src == G && t.n == n && t.lgC == lgC) hit = &t;
+    if (!hit) {
+        GreenTiled t{G, n, lgC, nullptr};
+        buffers_changed();
+        C2_HIP(hipMalloc(&t.dev, tot * sizeof(float)));
+        g_green_tiled.push_back(t);
+        hit = &g_green_tiled.back();
+    }
+    k_green_tile<<<ceil_div(tot, 256), 256, 0, s>>>(G, hit->dev, n, lgC, ntiles);
     C2_HIP(hipGetLastError());
     C2_HIP(hipStreamSynchronize(s));
-    g_green_tiled.push_back(t);
-    *out = t.dev;
+    *out = hit->dev;
     return CUBE2_OK;
 }
 

@@ -143,12 +143,74 @@ static double dt_force(float f2max, float a_mid) {
     return 0.5 * std::sqrt(1.0 / (std::sqrt((double)f2max) * (double)a_mid * g_grid));
 }
 
+// PM3 and PP of one kick on two streams.  Nothing couples them until the
+// velocity update: PM3 reads the particle arrays and writes the pending FP64
+// Delta-v, PP's cell lists and pair sums read the particle arrays and write
+// their own buffers.  So PP (a split pass) runs on the second stream beside
+// PM3, and the update (vp = RN_f(vp + dv64 + PP term), the order of
+// pp_force_kernel.cu:496-498,1043-1047: PM3's Delta-v first, then PP's)
+// follows on the main stream once both have finished.  The cell-base table
+// both read is computed once, before the fork.
+static int enqueue_pm3_pp(const PM3Plan& P3, const PPPlan& P, const int* d_rhoc,
+                          const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
+                          const float* d_scal, int accumulate, hipStream_t s) {
+    Context& C = ctx();
+    hipStream_t s2 = C.stream2;
+    double* dv64 = C.dv64.as<double>();
+    const int64_t* nz = nullptr;
+    C2_TRY(compute_nzero(d_rhoc, d_idx, &nz, s));
+    struct Shared {
+        Shared() { ctx().nzero_shared = true; }
+        ~Shared() { ctx().nzero_shared = false; }
+    } shared;
+    C2_HIP(hipEventRecord(C.ev_fork, s));
+    C2_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
+    {
+        StageStream on(s2);
+        C2_TRY(pp_enqueue(P, d_rhoc, d_idx, d_xp, d_vp, d_scal, nullptr, s2));
+    }
+    C2_TRY(pm3_enqueue(P3, d_rhoc, d_idx, d_xp, d_scal, accumulate, dv64, s));
+    C2_HIP(hipEventRecord(C.ev_join, s2));
+    C2_HIP(hipStreamWaitEvent(s, C.ev_join, 0));
+    return pp_enqueue_apply(P, d_vp, dv64, s);
+}
+
+// The overlapped PM3 + PP of the eager kick driver (prepare, enqueue, wait,
+// finish), in place of pm3_force then pp_force.
+static int pm3_pp_force(const int* isort, const int* ires, const int* ixyz3, const float* apm3,
+                        const int* ratio_sf, const int* d_rhoc, const int64_t* d_idx,
+                        const int16_t* d_xp, float* d_vp, int64_t n_used,
+                        const float* const* d_green, float mass, float a_mid, float dt,
+                        float* f2max_pm3, double* ptotal_pm3, float* f2max_pp, float* vmax,
+                        double* ptotal_pp, double* ttotal_pp, uint64_t* cnt, bool learned) {
+    Context& C = ctx();
+    const auto t0 = std::chrono::steady_clock::now();
+    if (C.dv_state == 1 && C.dv_n_used != n_used) {
+        set_error("particle count changed while a Delta-v is pending");
+        return CUBE2_ERR_STATE;
+    }
+    PM3Plan P3;
+    C2_TRY(pm3_prepare(isort, ires, ixyz3, ratio_sf, n_used, d_green, P3));
+    PPPlan P;
+    C2_TRY(pp_prepare(ires, ixyz3, apm3, ratio_sf, n_used, P, true, learned));
+    const int accumulate = C.dv_state == 1 ? 1 : 0;
+    const float* d_scal = nullptr;
+    C2_TRY(scalars_upload(mass, a_mid, dt, C.stream, &d_scal));
+    C2_TRY(enqueue_pm3_pp(P3, P, d_rhoc, d_idx, d_xp, d_vp, d_scal, accumulate, C.stream));
+    C2_TRY(sync_and_flush());
+    C.dv_state = 0;
+    C2_TRY(pm3_finish(P3, f2max_pm3, ptotal_pm3));
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return pp_finish(P, f2max_pp, vmax, ptotal_pp, ttotal_pp, cnt, wall);
+}
+
 int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, const int* d_rhoc,
          const int64_t* d_idx, const int16_t* d_xp, float* d_vp, int64_t n_used,
          const float* const* d_green, float mass, float a_mid, float dt, int* ires, int* isort,
          double* ptotal_pm3, double* ptotal_pp, double* ttotal_pp, cube2_kick_result_t* out) {
     if (!ixyz3 || !apm3 || !ratio_sf || !ires || !isort || !out ||
-        (flags & ~(CUBE2_KICK_PM3 | CUBE2_KICK_PP | CUBE2_KICK_KEEP_LEVELS | CUBE2_KICK_PM2)) ||
+        (flags & ~(CUBE2_KICK_PM3 | CUBE2_KICK_PP | CUBE2_KICK_KEEP_LEVELS | CUBE2_KICK_PM2 |
+                   CUBE2_KICK_LEARNED)) ||
         ((flags & CUBE2_KICK_PM2) && (!d_green || !d_green[5]))) {
         set_error("kick: null argument or unknown flag");
         return CUBE2_ERR_ARG;
@@ -182,6 +244,20 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
                          &out->f2max_pm2, pt2.data()));
         out->dt_pm2 = dt_force(out->f2max_pm2, a_mid);
     }
+    const bool both = (flags & CUBE2_KICK_PM3) && (flags & CUBE2_KICK_PP);
+    const bool learned = flags & CUBE2_KICK_LEARNED;
+    if (both && kick_overlap() && ctx().pp_mode != CUBE2_PP_EXACT) {
+        uint64_t cnt[2] = {0, 0};
+        C2_TRY(pm3_pp_force(isort, ires, ixyz3, apm3, ratio_sf, d_rhoc, d_idx, d_xp, d_vp, n_used,
+                            d_green, mass, a_mid, dt, &out->f2max_pm3, ptotal_pm3, &out->f2max_pp,
+                            out->vmax, ptotal_pp, ttotal_pp, cnt, learned));
+        for (int it = 0; it < g.ns3; it++) out->pt_pm3 += ptotal_pm3[it];
+        out->dt_pm3 = dt_force(out->f2max_pm3, a_mid);
+        out->pair_counts[0] = cnt[0];
+        out->pair_counts[1] = cnt[1];
+        for (int it = 0; it < g.ns3; it++) out->pt_pp += ptotal_pp[it];
+        out->dt_pp = dt_force(out->f2max_pp, a_mid);
+    } else {
     if (flags & CUBE2_KICK_PM3) {
         C2_TRY(pm3_force(isort, ires, ixyz3, ratio_sf, d_rhoc, d_idx, d_xp, n_used, d_green, mass,
                          a_mid, dt, &out->f2max_pm3, ptotal_pm3));
@@ -191,7 +267,7 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
     if (flags & CUBE2_KICK_PP) {
         uint64_t cnt[2] = {0, 0};
         C2_TRY(pp_force(ires, ixyz3, apm3, ratio_sf, d_rhoc, d_idx, d_xp, d_vp, n_used, mass,
-                        a_mid, dt, &out->f2max_pp, out->vmax, ptotal_pp, ttotal_pp, cnt));
+                        a_mid, dt, &out->f2max_pp, out->vmax, ptotal_pp, ttotal_pp, cnt, learned));
         out->pair_counts[0] = cnt[0];
         out->pair_counts[1] = cnt[1];
         for (int it = 0; it < g.ns3; it++) out->pt_pp += ptotal_pp[it];
@@ -199,6 +275,7 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
     } else if (flags & (CUBE2_KICK_PM3 | CUBE2_KICK_PM2)) {
         // no PP to apply the pending mesh Delta-v: apply it here
         C2_TRY(velocity_apply(d_rhoc, d_idx, d_vp, n_used, out->vmax));
+    }
     }
     if (!(flags & CUBE2_KICK_PM3)) out->dt_pm3 = INFINITY;
     if (!(flags & CUBE2_KICK_PP)) out->dt_pp = INFINITY;
@@ -214,6 +291,8 @@ int kick(int flags, const int* ixyz3, const float* apm3, const int* ratio_sf, co
 // graph refills from a pinned block at each launch.
 struct GraphKey {
     int flags = 0, mode = 0;
+    bool overlap = false;  // PM3 and PP on two streams (recorded as parallel branches)
+    bool learned = false;  // the network stands in for PP in sparse level-2 cells
     int64_t n_used = 0;
     const void* arrays[4] = {nullptr, nullptr, nullptr, nullptr};
     const float* green[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -222,7 +301,8 @@ struct GraphKey {
     int ratio_sf[7] = {0};
     cube2_params_t p{};
     bool same(const GraphKey& o) const {
-        return flags == o.flags && mode == o.mode && n_used == o.n_used &&
+        return flags == o.flags && mode == o.mode && overlap == o.overlap && learned == o.learned &&
+               n_used == o.n_used &&
                std::equal(arrays, arrays + 4, o.arrays) && std::equal(green, green + 5, o.green) &&
                ires == o.ires && isort == o.isort && ixyz3 == o.ixyz3 &&
                std::memcmp(apm3, o.apm3, sizeof(apm3)) == 0 &&
@@ -257,10 +337,13 @@ static int graph_enqueue(const GraphKey& K, const int* d_rhoc, const int64_t* d_
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, a_mid, dt, s, &d_scal));
     const bool pm3 = K.flags & CUBE2_KICK_PM3;
+    if (K.overlap)
+        return enqueue_pm3_pp(g_graph.pm3, g_graph.pp, d_rhoc, d_idx, d_xp, d_vp, d_scal, 0, s);
     if (pm3)
         C2_TRY(pm3_enqueue(g_graph.pm3, d_rhoc, d_idx, d_xp, d_scal, 0, C.dv64.as<double>(), s));
-    C2_TRY(pp_enqueue(g_graph.pp, d_rhoc, d_idx, d_xp, d_vp, d_scal,
-                      pm3 ? C.dv64.as<double>() : nullptr, s));
+    const double* dv = pm3 ? C.dv64.as<double>() : nullptr;
+    C2_TRY(pp_enqueue(g_graph.pp, d_rhoc, d_idx, d_xp, d_vp, d_scal, dv, s));
+    if (g_graph.pp.split) C2_TRY(pp_enqueue_apply(g_graph.pp, d_vp, dv, s));
     return CUBE2_OK;
 }
 
@@ -274,7 +357,9 @@ static int graph_record(const GraphKey& K, const int* ixyz3, const float* apm3,
     if (K.flags & CUBE2_KICK_PM3)
         C2_TRY(pm3_prepare(K.isort.data(), K.ires.data(), ixyz3, ratio_sf, K.n_used, d_green,
                            g_graph.pm3));
-    C2_TRY(pp_prepare(K.ires.data(), ixyz3, apm3, ratio_sf, K.n_used, g_graph.pp));
+    C2_TRY(pp_prepare(K.ires.data(), ixyz3, apm3, ratio_sf, K.n_used, g_graph.pp,
+                      K.overlap || K.learned, K.learned));
+    if (!K.overlap) g_graph.pp.grid_pairs = pp_tiled_grid(g_graph.pp.mode, false);
     C2_TRY(C.dv64.reserve((size_t)std::max<int64_t>(K.n_used, 1) * 3 * sizeof(double)));
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, a_mid, dt, s, &d_scal));  // reserves the scalar blocks
@@ -319,8 +404,9 @@ int kick_graph(int flags, const int* ixyz3, const float* apm3, const int* ratio_
                const int* ires, const int* isort, double* ptotal_pm3, double* ptotal_pp,
                double* ttotal_pp, cube2_kick_result_t* out, int* recorded) {
     if (!ixyz3 || !apm3 || !ratio_sf || !ires || !out || !d_rhoc || !d_idx || !d_xp || !d_vp ||
-        (flags != (CUBE2_KICK_PM3 | CUBE2_KICK_PP) && flags != CUBE2_KICK_PP)) {
-        set_error("recorded kick: null argument, or flags other than PM3|PP or PP");
+        ((flags & ~CUBE2_KICK_LEARNED) != (CUBE2_KICK_PM3 | CUBE2_KICK_PP) &&
+         (flags & ~CUBE2_KICK_LEARNED) != CUBE2_KICK_PP)) {
+        set_error("recorded kick: null argument, or flags other than PM3|PP or PP (with LEARNED)");
         return CUBE2_ERR_ARG;
     }
     if ((flags & CUBE2_KICK_PM3) && (!isort || !d_green)) {
@@ -345,6 +431,9 @@ int kick_graph(int flags, const int* ixyz3, const float* apm3, const int* ratio_
     GraphKey K;
     K.flags = flags;
     K.mode = C.pp_mode;
+    K.learned = flags & CUBE2_KICK_LEARNED;
+    K.flags = flags & ~CUBE2_KICK_LEARNED;
+    K.overlap = (flags & CUBE2_KICK_PM3) && kick_overlap() && C.pp_mode != CUBE2_PP_EXACT;
     K.n_used = n_used;
     K.arrays[0] = d_rhoc;
     K.arrays[1] = d_idx;

@@ -29,20 +29,23 @@
 #include "engine.h"
 #include "pp_kernels.h"
 #include "profile.h"
+#include "reduce.h"
 #include "scan.h"
 
 namespace cube2 {
 
 int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int grid, int mode,
                     hipStream_t s);
-int pp_tiled_grid(int mode);
+int pp_tiled_grid(int mode, bool overlap);
 
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
     DevBuf chunk_flags, chunks, ilist, big, dense;
+    DevBuf term;  // split pass: per sorted particle velocity term [cap][3]
     HostBuf h_res;  // PPRed, per-location counts, per-tile pairs, tile_first (pinned)
     std::vector<PPTile> h_tiles;
     std::vector<uint64_t> tile_pairs;  // [ntile][2] of the last call (when counted)
+    PPPlan last;                       // plan of the last completed call
     bool count_tiles = false;
     int64_t c_tot = 0;
     int n_tot = 0;
@@ -533,6 +536,10 @@ int compute_nzero(const int* d_rhoc, const int64_t* d_idx, const int64_t** out, 
     const Geom& g = ctx().g;
     const int64_t rows = (int64_t)g.ntile * g.nte * g.nte;
     C2_TRY(nzero_reserve());
+    if (ctx().nzero_shared) {  // computed once for both passes of an overlapped kick
+        *out = g_pp.nzero.as<int64_t>();
+        return CUBE2_OK;
+    }
     if (rows > 0) {
         Stage st("nzero", (double)rows * g.nte * 12 + rows * 8);
         k_nzero<<<ceil_div(rows, 256), 256, 0, s>>>(d_rhoc, d_idx, g_pp.nzero.as<int64_t>(), g.nte, rows);
@@ -565,7 +572,7 @@ __global__ void k_pp_check_total(const int* __restrict__ total, int64_t cap, PPR
 }
 
 int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
-               int64_t n_used, PPPlan& P) {
+               int64_t n_used, PPPlan& P, bool split, bool learned) {
     ensure_params();
     Context& C = ctx();
     const Geom& g = C.g;
@@ -590,6 +597,19 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     P.nsub3 = g.p.nns * g.p.nns * g.p.nns;
     P.n_used = n_used;
     P.mode = C.pp_mode;
+    if (split && P.mode == CUBE2_PP_EXACT) {
+        set_error("the exact-order PP mode has no split pass");
+        return CUBE2_ERR_ARG;
+    }
+    P.split = split;
+    if (learned && (!split || !learned_model_set())) {
+        set_error(split ? "learned kick without a model (cube2_learned_set)"
+                        : "the learned stand-in needs the split pair pass");
+        return split ? CUBE2_ERR_STATE : CUBE2_ERR_ARG;
+    }
+    P.learned = learned;
+    std::copy(ratio_sf, ratio_sf + 7, P.ratio_sf.begin());
+    std::copy(apm3, apm3 + 7, P.apm3.begin());
     std::vector<PPTile>& T = g_pp.h_tiles;
     T.assign(P.ntile, PPTile{});
     int64_t off = 0;
@@ -622,6 +642,11 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.ilist.reserve(P.cap * sizeof(int)));
     C2_TRY(g_pp.big.reserve(std::max<int64_t>(P.ncells_ext, 1) * sizeof(int)));
     C2_TRY(g_pp.dense.reserve((P.cap / 64 + 1) * sizeof(int)));
+    if (split) C2_TRY(g_pp.term.reserve(P.cap * 3 * sizeof(float)));
+    if (learned) {
+        C2_TRY(learned_prepare(g, P.iloc, ratio_sf, apm3, P.lp, learned_model_occupancy()));
+        C2_TRY(learned_reserve(P.lp));
+    }
     C2_TRY(g_pp.scan_tmp.reserve(scan_tmp_ints(std::max<int64_t>(P.c_tot, (int64_t)P.cap)) * sizeof(int)));
     C2_TRY(nzero_reserve());
     C2_TRY(g_pp.h_res.reserve(red_bytes(g, P.ntile) + (P.ntile + 1) * sizeof(int)));
@@ -631,10 +656,12 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_HIP(hipMemcpyAsync(g_pp.apm3.ptr, apm3, 7 * sizeof(float), hipMemcpyHostToDevice, s));
     C2_HIP(hipStreamSynchronize(s));  // host tables may go out of scope
     buffers_changed();                 // the device tables now describe this call
-    P.grid_pairs = pp_tiled_grid(P.mode);
+    P.grid_pairs = pp_tiled_grid(P.mode, split);
     g_pp.valid = false;
     return CUBE2_OK;
 }
+
+static int pp_results_to_host(const PPPlan& P, hipStream_t s);
 
 int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp,
                float* d_vp, const float* d_scal, const double* dv64, hipStream_t s) {
@@ -689,7 +716,7 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
                                                               g_pp.tile_first.as<int>());
         C2_HIP(hipGetLastError());
     }
-    PPArgs A;
+    PPArgs A{};
     A.pos4 = g_pp.pos4.as<float4>();
     A.slot = g_pp.slot.as<uint32_t>();
     A.cellid = g_pp.cellid.as<uint32_t>();
@@ -702,10 +729,13 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     A.ratio_cs = g.p.ratio_cs;
     A.app = (float)g.p.app;
     A.scal = d_scal;
-    A.dv64 = dv64;
-    A.vp = d_vp;
+    A.dv64 = P.split ? nullptr : dv64;
+    A.vp = P.split ? nullptr : d_vp;
+    A.term = P.split ? g_pp.term.as<float>() : nullptr;
     A.red = red;
     A.tile_pairs = P.count_tiles ? ploc + g.ns3 : nullptr;
+    A.learned_occ = P.learned ? P.lp.args.occupancy : 0;
+    A.learned_range = P.learned ? P.lp.args.range : 0.0f;
     if (P.mode == CUBE2_PP_EXACT) {
         C2_TRY(launch_pp_pairs(A, P.mode, s));
     } else {
@@ -730,8 +760,60 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             C2_HIP(hipGetLastError());
         }
         C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), flags + P.cap, P.grid_pairs, P.mode, s));
+        if (P.learned) C2_TRY(learned_enqueue(P.lp, A, g_pp.tile_first.as<int>(), s));
     }
-    // results into pinned host memory
+    if (P.split) return CUBE2_OK;  // pp_enqueue_apply updates vp and copies the results
+    return pp_results_to_host(P, s);
+}
+
+// Velocity update of a split pass, once the pair kernel and PM3 are done:
+// for every active sorted particle (range > 0), vp = RN_f(vp + dv64 +
+// term / pi) in FP64, the unsplit kernel's operation order; vmax over them.
+__global__ __launch_bounds__(256) void k_pp_apply(const float4* __restrict__ pos4,
+                                                 const uint32_t* __restrict__ slot,
+                                                 const float* __restrict__ term,
+                                                 const int* __restrict__ n_sorted, int cap,
+                                                 const double* __restrict__ dv64,
+                                                 float* __restrict__ vp, PPRed* red) {
+    float vm[3] = {0.f, 0.f, 0.f};
+    const bool ok = red->err == 0;  // a failed call leaves the sorted arrays unread
+    const int n = ok ? min(*n_sorted, cap) : 0;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
+        if (!(pos4[a].w > 0.0f)) continue;  // halo sources are not updated
+        const size_t ip = (size_t)slot[a] * 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double t = (double)term[3 * (size_t)a + c] / 3.14159265358979323846;
+            double v = (double)vp[ip + c];
+            if (dv64) v += dv64[ip + c];
+            const float nv = (float)(v + t);
+            vp[ip + c] = nv;
+            vm[c] = fmaxf(vm[c], fabsf(nv));
+        }
+    }
+    block_max_to<3>(red->vmax_bits, vm);
+}
+
+int pp_enqueue_apply(const PPPlan& P, float* d_vp, const double* dv64, hipStream_t s) {
+    if (!P.split) {
+        set_error("velocity apply of an unsplit PP pass");
+        return CUBE2_ERR_STATE;
+    }
+    {
+        Stage st("pp_apply", (double)P.n_used * (16 + 4 + 12 + 12 + 24 + 12));
+        const unsigned blocks = std::min<unsigned>(ceil_div(P.cap, 256), 8192);
+        k_pp_apply<<<blocks, 256, 0, s>>>(g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>(),
+                                          g_pp.term.as<float>(), g_pp.counts.as<int>() + P.c_tot,
+                                          (int)P.cap, dv64, d_vp, g_pp.red.as<PPRed>());
+        C2_HIP(hipGetLastError());
+    }
+    return pp_results_to_host(P, s);
+}
+
+// PP's reduction block and tile starts into pinned host memory.
+static int pp_results_to_host(const PPPlan& P, hipStream_t s) {
+    const Geom& g = ctx().g;
+    PPRed* red = g_pp.red.as<PPRed>();
     char* h = g_pp.h_res.as<char>();
     C2_HIP(hipMemcpyAsync(h, red, red_bytes(g, P.ntile), hipMemcpyDeviceToHost, s));
     C2_HIP(hipMemcpyAsync(h + red_bytes(g, P.ntile), g_pp.tile_first.ptr,
@@ -780,7 +862,32 @@ int pp_finish(const PPPlan& P, float* f2max, float* vmax, double* ptotal, double
     else g_pp.tile_pairs.clear();
     g_pp.c_tot = P.c_tot;
     g_pp.n_tot = n_tot;
+    g_pp.last = P;
     g_pp.valid = true;
+    return CUBE2_OK;
+}
+
+int pp_last(const PPPlan** P, PPArgs* A, const int** tile_first) {
+    if (!g_pp.valid || g_pp.last.mode == CUBE2_PP_EXACT) {
+        set_error("no PP cell lists: run a PP call in a tiled mode first");
+        return CUBE2_ERR_STATE;
+    }
+    const PPPlan& L = g_pp.last;
+    *P = &L;
+    PPArgs a{};
+    a.pos4 = g_pp.pos4.as<float4>();
+    a.slot = g_pp.slot.as<uint32_t>();
+    a.cellid = g_pp.cellid.as<uint32_t>();
+    a.cell_start = g_pp.counts.as<int>();
+    a.ilist = g_pp.ilist.as<int>();
+    a.tiles = g_pp.tiles.as<PPTile>();
+    a.n_sorted = g_pp.counts.as<int>() + L.c_tot;
+    a.ntile = L.ntile;
+    a.cap = (int)L.cap;
+    a.ratio_cs = ctx().g.p.ratio_cs;
+    a.red = g_pp.red.as<PPRed>();
+    *A = a;
+    *tile_first = g_pp.tile_first.as<int>();
     return CUBE2_OK;
 }
 
@@ -800,7 +907,7 @@ int pp_debug_tile_pairs(int ntile, uint64_t* counts) {
 int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
              int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
-             double* ptotal, double* ttotal, uint64_t* pair_counts) {
+             double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned) {
     ensure_params();
     Context& C = ctx();
     auto t0 = std::chrono::steady_clock::now();
@@ -813,11 +920,13 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
         return CUBE2_ERR_STATE;
     }
     PPPlan P;
-    C2_TRY(pp_prepare(ires, ixyz3, apm3, ratio_sf, n_used, P));
+    C2_TRY(pp_prepare(ires, ixyz3, apm3, ratio_sf, n_used, P, learned, learned));
+    P.grid_pairs = pp_tiled_grid(P.mode, false);  // nothing runs beside it here
     const double* dv = (C.dv_state == 1) ? C.dv64.as<double>() : nullptr;
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, a_mid, dt, C.stream, &d_scal));
     C2_TRY(pp_enqueue(P, d_rhoc, d_idx, d_xp, d_vp, d_scal, dv, C.stream));
+    if (P.split) C2_TRY(pp_enqueue_apply(P, d_vp, dv, C.stream));
     C2_TRY(sync_and_flush());
     C.dv_state = 0;
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -860,7 +969,7 @@ void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
                       &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks,
-                      &p.ilist, &p.big, &p.dense})
+                      &p.ilist, &p.big, &p.dense, &p.term})
         b->release();
     p.valid = false;
 }

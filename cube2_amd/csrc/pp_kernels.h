@@ -20,6 +20,28 @@ struct PPRed {
     int pad;
 };
 
+// Learned stand-in (learned.hip): a level-2 subtile whose fine cells the
+// network evaluates ("job"): its parent tile and its origin in fine cells
+// (tile-relative, level 2).
+struct LearnedJob {
+    int t;
+    int o[3];
+};
+
+// What the learned kernels need to pick the substituted particles (physical
+// particles of level-2 subtiles in PP cells of fewer than `occupancy`) and
+// place them in their job's grids.
+struct LearnedArgs {
+    const LearnedJob* jobs;
+    const int* job_of;  // [ntile * nns^3] subtile location -> job, or -1
+    float range;        // the level-2 PP range (apm3 of level 2)
+    int occupancy;
+    float rsf2;         // fine cells per standard cell at level 2
+    float sub_std;      // standard cells per subtile side
+    int nns;
+    float y_scale;      // network output unit (acceleration)
+};
+
 struct PPArgs {
     const float4* pos4;        // sorted (x, y, z, range); range 0 = halo source
     const uint32_t* slot;      // sorted -> xp/vp slot
@@ -33,10 +55,19 @@ struct PPArgs {
     const float* scal;         // {mass, a_mid, dt} (device)
     const double* dv64;        // pending PM3 Delta-v (FP64) or nullptr
     float* vp;
+    // split pass (non-null): the pair kernel stores each active particle's
+    // velocity term a * m^2 * a_mid * dt / 6 (FP32, before the division by
+    // pi) at term[3 * sorted index], and k_pp_apply updates vp and vmax
+    // after PM3 has joined; vp and dv64 are then unused by the pair kernel
+    float* term;
     PPRed* red;
     // per parent tile {P_cand, P_acc} [ntile][2], or nullptr (counted only
     // when requested: cube2_pp_count_tiles)
     unsigned long long* tile_pairs;
+    // learned kick: chunks of PP cells of fewer than learned_occ particles
+    // with range learned_range are left to the network (0: none)
+    int learned_occ;
+    float learned_range;
 };
 
 int launch_pp_pairs(const PPArgs& a, int mode, hipStream_t s);

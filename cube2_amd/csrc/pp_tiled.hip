@@ -29,6 +29,8 @@
 //   the reference; only the order of the per-i sums differs (<= ~4e-7
 //   relative RMS).
 // FAST: FMA/rsqrt arithmetic with F_ra in Horner form.
+// PLUMMER: Plummer softening r/(r^2 + app^2)^(3/2) in place of F_ra(r, app)
+//   (no counterpart in the reference), in PRECISE's arithmetic.
 // PRECISE: a cancellation-free force law and FP64 per-particle sums.  For
 //   1 < ep <= 2, 35 a^2 ep^2 (F_ra(r,a) - 1/r^2) = (ep-2)^5 (7ep^3 + 22ep^2
 //   + 10ep + 4), so 1/r^2 - F_ra = (2-ep)^5 (7ep^3+22ep^2+10ep+4)/(140 r^2)
@@ -130,7 +132,7 @@ __device__ inline float fra_fast2(float ep, float ir2, float c) {
     return ep > 1.0f ? hi : lo;
 }
 
-enum { kModeRef = 0, kModeFast = 1, kModePrecise = 2 };
+enum { kModeRef = 0, kModeFast = 1, kModePrecise = 2, kModePlummer = 3 };
 
 // 1/r^2 - F_ra(r, a) for r < a (ep = 2r/a in (0, 2]), cancellation-free.
 __device__ inline float short_term(float ep, float ir2, float c) {
@@ -141,6 +143,16 @@ __device__ inline float short_term(float ep, float ir2, float c) {
     const float lo =
         ir2 - c * ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
     return ep > 1.0f ? hi : lo;
+}
+
+// 1/r^2 - r / (r^2 + eps^2)^(3/2) (Plummer softening), cancellation-free:
+// with u = eps^2/r^2 and w = (1 + u)^(-1/2) it is (1 - w^3)/r^2, and
+// 1 - w = u w^2 / (1 + w), so (1/r^2) u w^2 (1 + w + w^2) / (1 + w).
+__device__ inline float plummer_term(float ir2, float eps2) {
+    const float u = eps2 * ir2;
+    const float w = __frsqrt_rn(1.0f + u);
+    const float w2 = w * w;
+    return ir2 * u * w2 * (1.0f + w + w2) / (1.0f + w);
 }
 
 template <int kMode>
@@ -180,6 +192,18 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
         fx = f * e.x;
         fy = f * e.y;
         fz = f * e.z;
+    } else if (kMode == kModePlummer) {
+        // pair = (1/r^2 - F_ra(r, range)) - (1/r^2 - r/(r^2 + app^2)^(3/2)),
+        // each piece cancellation-free (softening length eps = app)
+        const float r = sqrtf(e.w);
+        const float ir = 1.0f / r;
+        const float ir2 = ir * ir;
+        float f = short_term(r * (float)(2.0 * Kr.inv_a), ir2, Kr.c);
+        if (Ka.a > 0.0f) f -= plummer_term(ir2, Ka.a * Ka.a);
+        f *= ir;
+        fx = f * e.x;
+        fy = f * e.y;
+        fz = f * e.z;
     } else {
         // pair = (1/r^2 - F_ra(r, range)) - (1/r^2 - F_ra(r, app)), the second
         // term zero for r > app; each piece cancellation-free.
@@ -202,7 +226,8 @@ __device__ inline void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
 
 template <int kMode>
 struct WaveLDS {
-    using Acc = typename std::conditional<kMode == kModePrecise, double, float>::type;
+    using Acc = typename std::conditional<kMode == kModePrecise || kMode == kModePlummer, double,
+                                          float>::type;
     float4 q[kQ];           // queued chunks: accepted pairs (dx, dy, dz, r^2)
     unsigned char qi[kQ];   // and their i slot
     Acc acc[64][3];         // per-i force sums
@@ -264,9 +289,10 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const int a0 = uni(chunks[ck]);
         const uint32_t cid = (uint32_t)uni((int)A.cellid[a0]);
         const int cend = uni(A.cell_start[cid + 1]);
+        const int cbeg = uni(A.cell_start[cid]);
         // cells of >= 64 particles are visited in sub-cell order (ilist), so
         // that a chunk is spatially compact
-        const bool grouped = cend - uni(A.cell_start[cid]) >= 64;
+        const bool grouped = cend - cbeg >= 64;
         const int ni = min(grouped ? 1 << dense_lg : 64, cend - a0);
         int lgI = 0;
         while ((1 << lgI) < ni) lgI++;
@@ -282,6 +308,8 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const int S = T.S;
         const int x = (int)(key % S), y = (int)((key / S) % S), z = (int)(key / ((long long)S * S));
         const float range = uni(p.w);  // one subtile per PP cell: same range for the chunk
+        // learned kick: the network stands in for the sparse level-2 cells
+        if (A.learned_occ && range == A.learned_range && cend - cbeg < A.learned_occ) continue;
         int R;
         float pp2;
         {
@@ -386,7 +414,7 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                     e.x = qj.x - p.x;
                     e.y = qj.y - p.y;
                     e.z = qj.z - p.z;
-                    if (kMode == kModePrecise)
+                    if (kMode == kModePrecise || kMode == kModePlummer)
                         e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
                     else
                         e.w = e.x * e.x + e.y * e.y + e.z * e.z;
@@ -450,20 +478,25 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
             const float mass = A.scal[0], a_mid = A.scal[1], dt = A.scal[2];
             const float a0m = ax * mass * mass, a1m = ay * mass * mass, a2m = az * mass * mass;
             f2 = fmaxf(f2, a0m * a0m + a1m * a1m + a2m * a2m);
-            const size_t ip = (size_t)A.slot[a] * 3;
             const float acc[3] = {a0m, a1m, a2m};
-            float nv[3];
+            if (A.term) {  // split pass: k_pp_apply finishes the update
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double term = (double)(acc[c] * a_mid * dt / 6.0f) / 3.14159265358979323846;
-                double v = (double)A.vp[ip + c];
-                if (A.dv64) v += A.dv64[ip + c];
-                nv[c] = (float)(v + term);
-                A.vp[ip + c] = nv[c];
+                for (int c = 0; c < 3; c++) A.term[3 * (size_t)a + c] = acc[c] * a_mid * dt / 6.0f;
+            } else {
+                const size_t ip = (size_t)A.slot[a] * 3;
+                float nv[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double term = (double)(acc[c] * a_mid * dt / 6.0f) / 3.14159265358979323846;
+                    double v = (double)A.vp[ip + c];
+                    if (A.dv64) v += A.dv64[ip + c];
+                    nv[c] = (float)(v + term);
+                    A.vp[ip + c] = nv[c];
+                }
+                v0 = fmaxf(v0, fabsf(nv[0]));
+                v1 = fmaxf(v1, fabsf(nv[1]));
+                v2 = fmaxf(v2, fabsf(nv[2]));
             }
-            v0 = fmaxf(v0, fabsf(nv[0]));
-            v1 = fmaxf(v1, fabsf(nv[1]));
-            v2 = fmaxf(v2, fabsf(nv[2]));
         }
         if (A.tile_pairs) {  // per-tile counts (on request only)
             const unsigned long long dc = wave_sum_u64(pc - pc0), da = wave_sum_u64(pa - pa0);
@@ -482,11 +515,21 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
 
 // Blocks of the persistent pair kernel: the resident maximum over all CUs
 // (occupancy query, cached per mode).
-int pp_tiled_grid(int mode) {
-    static int cache[4] = {0, 0, 0, 0};
-    const int m = mode & 3;
+// overlap: the kernel runs beside PM3 (CUBE2_TUNE_PP_OVERLAP_BLOCKS per CU,
+// at most the occupancy maximum).
+int pp_tiled_grid(int mode, bool overlap) {
+    static int cache[8] = {};
+    const int m = mode & 7;
     const int forced = ctx().tune[CUBE2_TUNE_PP_BLOCKS_PER_CU];
     int dev = 0, ncu = 0, per = 0;
+    if (overlap) {
+        const int full = pp_tiled_grid(mode, false);
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
+        const int t = ctx().tune[CUBE2_TUNE_PP_OVERLAP_BLOCKS];
+        return std::min(full, std::max(1, ncu) * (t ? t : kPPOverlapBlocks));
+    }
     if (forced) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -502,6 +545,8 @@ int pp_tiled_grid(int mode) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModeRef>, 256, 0);
     else if (mode == CUBE2_PP_FAST)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModeFast>, 256, 0);
+    else if (mode == CUBE2_PP_PLUMMER)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModePlummer>, 256, 0);
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModePrecise>, 256, 0);
     if (e != hipSuccess || per < 1) per = 2;
@@ -511,8 +556,10 @@ int pp_tiled_grid(int mode) {
 
 int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int grid, int mode,
                     hipStream_t s) {
-    const char* name = mode == CUBE2_PP_REF ? "pp_pairs_ref"
-                       : mode == CUBE2_PP_FAST ? "pp_pairs_fastq" : "pp_pairs_precise";
+    const char* name = mode == CUBE2_PP_REF       ? "pp_pairs_ref"
+                       : mode == CUBE2_PP_FAST    ? "pp_pairs_fastq"
+                       : mode == CUBE2_PP_PLUMMER ? "pp_pairs_plummer"
+                                                  : "pp_pairs_precise";
     Stage st(name);
     const unsigned blocks = (unsigned)std::max(1, grid);
     // chunks of more than 2 particles sum directly unless tuned otherwise
@@ -522,6 +569,8 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
         k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
     else if (mode == CUBE2_PP_FAST)
         k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
+    else if (mode == CUBE2_PP_PLUMMER)
+        k_pp_tiled<kModePlummer><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
     else
         k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
     C2_HIP(hipGetLastError());
@@ -547,7 +596,7 @@ __global__ void k_pair_forces(const float4* __restrict__ in, int n, float app,
     bool ok;
     {
 #pragma clang fp contract(off)
-        if (kMode == kModePrecise)
+        if (kMode == kModePrecise || kMode == kModePlummer)
             e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
         else
             e.w = e.x * e.x + e.y * e.y + e.z * e.z;
@@ -571,7 +620,8 @@ int pair_forces_debug(int mode, const float* pairs, int n, float app, float* for
         set_error("pair check: bad arguments");
         return CUBE2_ERR_ARG;
     }
-    if (mode != CUBE2_PP_REF && mode != CUBE2_PP_FAST && mode != CUBE2_PP_PRECISE) {
+    if (mode != CUBE2_PP_REF && mode != CUBE2_PP_FAST && mode != CUBE2_PP_PRECISE &&
+        mode != CUBE2_PP_PLUMMER) {
         set_error("pair check: mode %d has no pair_force", mode);
         return CUBE2_ERR_ARG;
     }
@@ -590,6 +640,8 @@ int pair_forces_debug(int mode, const float* pairs, int n, float app, float* for
         const unsigned blocks = ceil_div((size_t)n, 256);
         if (mode == CUBE2_PP_REF) k_pair_forces<kModeRef><<<blocks, 256, 0, s>>>(d_in, n, app, d_out);
         else if (mode == CUBE2_PP_FAST) k_pair_forces<kModeFast><<<blocks, 256, 0, s>>>(d_in, n, app, d_out);
+        else if (mode == CUBE2_PP_PLUMMER)
+            k_pair_forces<kModePlummer><<<blocks, 256, 0, s>>>(d_in, n, app, d_out);
         else k_pair_forces<kModePrecise><<<blocks, 256, 0, s>>>(d_in, n, app, d_out);
         e = hipGetLastError();
     }

@@ -221,15 +221,16 @@ class ShortRangeKick:
         return arr
 
     def kick(self, mass: float, a_mid: float, dt: float, pm3: bool = True, pp: bool = True,
-             select: bool = False, pm2: bool = False) -> KickResult:
+             select: bool = False, pm2: bool = False, learned: bool = False) -> KickResult:
         """One short-range kick through the library's kick driver
         (cube2_kick_dev).  select=True re-selects levels on the device from
         rhoc (kick.f90:107-131); otherwise the current ires/isort are used.
-        pm2=True runs PM2 (kick.f90:76-105) first."""
+        pm2=True runs PM2 (kick.f90:76-105) first.  learned=True lets the
+        library's loaded network stand in for PP in sparse level-2 cells."""
         L = _lib.lib()
         _lib.set_params(self.geom)
         flags = ((_lib.KICK_PM3 if pm3 else 0) | (_lib.KICK_PP if pp else 0) |
-                 (_lib.KICK_PM2 if pm2 else 0))
+                 (_lib.KICK_PM2 if pm2 else 0) | (_lib.KICK_LEARNED if learned else 0))
         if select:
             self.ires = np.zeros(self.geom.ns3, np.int32)
             self.isort = np.zeros(self.geom.ns3, np.int32)
@@ -259,13 +260,15 @@ class ShortRangeKick:
                              dt_vmax=out.dt_vmax)
         return res
 
-    def kick_graph(self, mass: float, a_mid: float, dt: float, pm3: bool = True) -> KickResult:
+    def kick_graph(self, mass: float, a_mid: float, dt: float, pm3: bool = True,
+                   learned: bool = False) -> KickResult:
         """The same kick through cube2_kick_graph_dev: recorded into a HIP
         graph on the first call (res.recorded = True) and replayed while the
         levels, arrays and sizes stay the same."""
         L = _lib.lib()
         _lib.set_params(self.geom)
-        flags = (_lib.KICK_PM3 if pm3 else 0) | _lib.KICK_PP
+        flags = ((_lib.KICK_PM3 if pm3 else 0) | _lib.KICK_PP |
+                 (_lib.KICK_LEARNED if learned else 0))
         ns3 = self.geom.ns3
         pt3, ptp, ttp = np.zeros(ns3), np.zeros(ns3), np.zeros(ns3)
         out = _lib.KickResultC()

@@ -38,7 +38,13 @@ enum {
     CUBE2_ERR_STATE = 5     /* call order (velocity accumulator) violated */
 };
 
-enum { CUBE2_PP_EXACT = 0, CUBE2_PP_FAST = 1, CUBE2_PP_REF = 2, CUBE2_PP_PRECISE = 3 };
+enum {
+    CUBE2_PP_EXACT = 0,
+    CUBE2_PP_FAST = 1,
+    CUBE2_PP_REF = 2,
+    CUBE2_PP_PRECISE = 3,
+    CUBE2_PP_PLUMMER = 4
+};
 
 /* ---------------------------------------------------------------------
  * Drop-in entry points (host pointers; the Fortran caller owns every array)
@@ -100,7 +106,11 @@ int cube2_get_params(cube2_params_t* p);
  *   CUBE2_PP_FAST: FMA/rsqrt arithmetic, work-compacted (its own rounding;
  *     accuracy stated against an FP64 direct sum);
  *   CUBE2_PP_PRECISE: cancellation-free force law near the cutoff and FP64
- *     per-particle sums (accuracy stated against an FP64 direct sum). */
+ *     per-particle sums (accuracy stated against an FP64 direct sum);
+ *   CUBE2_PP_PLUMMER: Plummer softening, no counterpart in the reference:
+ *     the pair force r / (r^2 + app^2)^(3/2) - F_ra(r, range_i) (app is the
+ *     softening length eps), in PRECISE's arithmetic (accuracy stated
+ *     against an FP64 direct sum of the same law). */
 int cube2_set_pp_mode(int mode);
 
 /* Launch-shape tunables (no counterpart in the reference; set by the
@@ -115,7 +125,12 @@ int cube2_set_pp_mode(int mode);
  *   INTERP_SEG        1..32, particles per wavefront work item of the mesh
  *                     interpolation, in units of 64 (4);
  *   PP_DENSE_LG       3..6, log2 of the pair kernel's chunk size in PP
- *                     cells of >= 64 particles (4).
+ *                     cells of >= 64 particles (4);
+ *   KICK_OVERLAP      1: the kick driver runs PP's cell lists and pair sums
+ *                     on a second stream beside PM3, the velocity update
+ *                     after both (built-in); 2: PM3, then PP;
+ *   PP_OVERLAP_BLOCKS 1..16, blocks per CU of the pair kernel while PM3
+ *                     runs beside it (built-in: see the library).
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
@@ -123,7 +138,9 @@ enum {
     CUBE2_TUNE_FFT_MIN_LGC = 2,
     CUBE2_TUNE_INTERP_SEG = 3,
     CUBE2_TUNE_PP_DENSE_LG = 4,
-    CUBE2_TUNE_COUNT = 5
+    CUBE2_TUNE_KICK_OVERLAP = 5,
+    CUBE2_TUNE_PP_OVERLAP_BLOCKS = 6,
+    CUBE2_TUNE_COUNT = 7
 };
 int cube2_set_tuning(int key, int value);
 int cube2_get_tuning(int key, int* value); /* the value in use (built-in choice resolved) */
@@ -133,7 +150,10 @@ int cube2_get_tuning(int key, int* value); /* the value in use (built-in choice 
  * same path with particle and cell arrays already in HBM).  rhoc, idx_b_r,
  * xp, vp are device pointers; isort/ires/ixyz3/apm3/ratio_sf and outputs
  * f2max/vmax/ptotal/ttotal are host pointers.  n_used = idx_b_r[last].
- * Work runs on the library stream; calls return after it completes.
+ * Work runs on the library stream; calls return after it completes.  The
+ * library's streams are blocking streams: their work starts after work the
+ * caller queued earlier on the default (null) stream, such as a PyTorch
+ * caller's tensor writes.
  * ------------------------------------------------------------------- */
 int cube2_pm3_force_dev(const int* isort, const int* ires, const int* ixyz3, const int* ratio_sf,
                         const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp,
@@ -197,6 +217,46 @@ int cube2_gemm_bf16_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint1
 int cube2_gemm_bf16_noalloc_dev(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
                                 int64_t sbk, int64_t sbn, float* C, int64_t ldc, int M, int N,
                                 int K, int acc);
+/* Layer weights W[27*cin][cout] (k = tap*cin + c, tap = (dz*3+dy)*3+dx; FP32)
+ * packed as BF16 for cube2_cnn_conv3_dev: Wp holds T*Cp*cin entries, T = 27
+ * (cin 32) or 32 (cin 4), Cp = 32 (cout 32) or 16 (cout 3). */
+int cube2_cnn_conv_pack_dev(const float* W, uint16_t* Wp, int cin, int cout);
+/* Y[b][z][y][x][o] = act(bias[o] + sum_{tap, c} X[b][z+dz][y+dy][x+dx][c] W[tap*cin+c][o])
+ * for the (D-2)^3 outputs of each of B grids of D^3 x cin cells
+ * (channels-last), on the matrix cores (BF16 operands, FP32 accumulation),
+ * with no im2col.  act = ReLU if relu.  x_bf16 / y_bf16: X / Y hold BF16
+ * bit patterns (else FP32).  Kernels: cin 4 -> cout 32 (X FP32),
+ * 32 -> 32, 32 -> 3 (Y FP32). */
+int cube2_cnn_conv3_dev(const void* X, int x_bf16, const uint16_t* Wp, const float* bias, void* Y,
+                        int y_bf16, int B, int D, int cin, int cout, int relu);
+/* Learned stand-in in the kick (CUBE2_KICK_LEARNED).  The network: four
+ * valid 3^3 convolutions 4 -> 32 -> 32 -> 32 -> 3 channels, ReLU after the
+ * first three, over fine cells of the level-2 mesh; input channels
+ * log(1 + count) and the mean offset of the cell's particles from its centre
+ * (fine cells, x y z); a particle's acceleration is y_scale times the
+ * cloud-in-cell interpolation of the output at its position.  It stands in
+ * for PP for the physical particles of level-2 subtiles in PP cells of fewer
+ * than `occupancy` particles.  W[l]: 27*cin*cout FP32 (host, k = tap*cin +
+ * c), b[l]: cout FP32 (host). */
+int cube2_learned_set(const float* const* W /* [4] */, const float* const* b /* [4] */,
+                      float y_scale, int occupancy);
+int cube2_learned_clear(void);
+/* Training data after a PP call (tiled modes): the input grids of every
+ * level-2 subtile, d_X [njobs][DI][DI][DI][4] FP32 (d_X NULL: sizes only),
+ * and the particles the network would stand in for: their subtile (job), the
+ * origin of their 10^3 input patch in the job's grid (= the CIC lower corner
+ * in the job's output grid), the CIC fractions, and their slot. */
+int cube2_learned_inputs_dev(float* d_X, int cap_jobs, int* njobs, int* DI);
+int cube2_learned_samples_dev(int occupancy, int cap, int* d_job, int* d_base /* [cap][3] */,
+                              float* d_frac /* [cap][3] */, uint32_t* d_slot, int* n);
+/* The last learned kick's last pass of the layers (for checks): jobs
+ * [*j0, *j0 + *nj), their input grids d_X [nj][DI^3][4] and outputs d_O
+ * [nj][PO^3][3] (either may be NULL; both NULL: sizes only). */
+int cube2_learned_debug_last(float* d_X, float* d_O, int cap_jobs, int* j0, int* nj, int* DI,
+                             int* PO);
+/* d_out[i] = the 10^3 x 4 patch of d_X[d_job[i]] at d_base[i] (zeros outside). */
+int cube2_learned_gather_dev(const float* d_X, int DI, const int* d_job, const int* d_base, int n,
+                             float* d_out);
 /* col[(p,z,y,x)][((dz*3+dy)*3+dx)*Cin + c] = X[p][z+dz][y+dy][x+dx][c] (BF16),
  * for the (D-2)^3 outputs of each of B inputs of D^3 x Cin. */
 int cube2_cnn_im2col_dev(const float* X, uint16_t* col, int B, int D, int Cin);
@@ -214,6 +274,11 @@ int cube2_cnn_adam_dev(float* w, const float* g, float* m, float* v, uint16_t* w
                        float lr, float b1, float b2, float eps, int t);
 /* *loss = mean (out - target)^2 over n values, dout = 2 (out - target) / n. */
 int cube2_cnn_mse_dev(const float* out, const float* target, float* dout, float* loss, int64_t n);
+/* pred[i][d] = sum over the 2^3 corners c (x fastest) of the CIC weight of
+ * frac[i] times out8[i][c][d]; loss = mean over n*3 of (pred - target)^2;
+ * dout8 = its gradient with respect to out8.  pred may be NULL. */
+int cube2_cnn_cic_mse_dev(const float* out8, const float* frac, const float* target, int n,
+                          float* pred, float* dout8, float* loss);
 
 /* Level selection and job order of kick.f90:107-131 (host arrays). */
 int cube2_select_levels(const int* rhoc, const int* ixyz3, int* ires, int* isort, int* napm);
@@ -288,7 +353,9 @@ enum {
     CUBE2_KICK_PM3 = 1,          /* run PM3 */
     CUBE2_KICK_PP = 2,           /* run PP (applies PM3's pending Delta-v) */
     CUBE2_KICK_KEEP_LEVELS = 4,  /* use the caller's ires/isort instead of selecting them */
-    CUBE2_KICK_PM2 = 8           /* run PM2 first; d_green[5] is its table (Gk2) */
+    CUBE2_KICK_PM2 = 8,          /* run PM2 first; d_green[5] is its table (Gk2) */
+    CUBE2_KICK_LEARNED = 16      /* the learned stand-in replaces PP in sparse level-2 cells
+                                    (cube2_learned_set); PP modes other than EXACT */
 };
 
 /* flags: CUBE2_KICK_*.  ires/isort [ns3] are written (level selection of
@@ -335,7 +402,7 @@ int cube2_pp_debug_tile_pairs(int ntile, uint64_t* counts);
 
 /* The pair kernel's pair force for given pairs (host arrays):
  * pairs[n][4] = {dx, dy, dz, range}, forces[n][3] = the force of that pair
- * in mode CUBE2_PP_REF / FAST / PRECISE with softening app, evaluated by the
+ * in mode CUBE2_PP_REF / FAST / PRECISE / PLUMMER with softening app, evaluated by the
  * same device function and acceptance test (0 < r^2 < range^2, else 0) as
  * the pair kernel.  In REF mode it equals (F_ra(r,app) - F_ra(r,range))*d/r
  * of pp_force_kernel.c:308-318 bit for bit. */

@@ -81,9 +81,9 @@ def lib():
                                i32p, i64p, i16p, f32p, C.c_int64, C.POINTER(C.c_int64)]
         L.or_velocity_apply.argtypes = [C.POINTER(OrGeom), i32p, i64p, f32p, f64p, f32p]
         L.or_direct_pp_f64_cells.argtypes = [f64p, C.c_int, C.c_double, i32p, C.c_int, f64p,
-                                             C.c_double, f64p]
+                                             C.c_double, C.c_int, f64p]
         L.or_direct_pp_f64.argtypes = [f64p, C.c_int, C.c_double, i32p, C.c_int, f64p, C.c_double,
-                                       f64p]
+                                       C.c_int, f64p]
         _LIB = L
     return _LIB
 
@@ -313,14 +313,17 @@ def velocity_apply(geom, rhoc, idx_b_r, vp, dv64):
     return vp2, vm
 
 
-def direct_pp_f64(pos, box, targets, ranges, app, cells=False):
+def direct_pp_f64(pos, box, targets, ranges, app, cells=False, law="s2"):
     """FP64 direct sum for the targets; cells=True bins the particles so each
-    target scans its 27 neighbour cells (same pairs, other summation order)."""
+    target scans its 27 neighbour cells (same pairs, other summation order).
+    law "s2": F_ra(r, app) - F_ra(r, range) (pp_force_kernel.c:308-318);
+    "plummer": r / (r^2 + app^2)^(3/2) - F_ra(r, range)."""
     pos = _c(pos, np.float64)
     targets = _c(targets, np.int32)
     acc = np.zeros((targets.size, 3), np.float64)
     f = lib().or_direct_pp_f64_cells if cells else lib().or_direct_pp_f64
-    f(pos, pos.shape[0], float(box), targets, targets.size, _c(ranges, np.float64), float(app), acc)
+    f(pos, pos.shape[0], float(box), targets, targets.size, _c(ranges, np.float64), float(app),
+      {"s2": 0, "plummer": 1}[law], acc)
     return acc
 
 

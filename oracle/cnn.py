@@ -2,19 +2,21 @@
 ONLY: loaded by tests/ as the checker of cube2_amd/learned.py).
 
 The network has no counterpart in the reference.  Its definition: patches of
-a 9^3 fine-cell density stencil, channels-last [patch][z][y][x][c]; four 3^3
-valid convolutions 1 -> 32 -> 32 -> 32 -> 3 channels (9 -> 7 -> 5 -> 3 -> 1
-cells), ReLU after the first three; layer weights W[k][c_out] with
-k = ((dz*3 + dy)*3 + dx)*c_in + c, biases b[c_out].  Forward and backward are
-plain numpy in FP64; the tests pin the backward against central finite
-differences of the forward.
+10^3 fine cells with 4 channels (log(1 + count) and the mean offset of the
+cell's particles from its centre), channels-last [patch][z][y][x][c]; four
+3^3 valid convolutions 4 -> 32 -> 32 -> 32 -> 3 channels (10 -> 8 -> 6 -> 4
+-> 2 cells), ReLU after the first three; layer weights W[k][c_out] with
+k = ((dz*3 + dy)*3 + dx)*c_in + c, biases b[c_out].  A sample's prediction
+is the cloud-in-cell combination of its 2^3 outputs with its fractions
+(``cic``).  Forward and backward are plain numpy in FP64; the tests pin the
+backward against central finite differences of the forward.
 """
 from __future__ import annotations
 
 import numpy as np
 
-CHANNELS = (1, 32, 32, 32, 3)
-PATCH = 9
+CHANNELS = (4, 32, 32, 32, 3)
+PATCH = 10
 
 
 def im2col(x: np.ndarray) -> np.ndarray:
@@ -51,8 +53,25 @@ def init_params(seed: int = 0):
     return params
 
 
+def cic_weights(frac: np.ndarray) -> np.ndarray:
+    """[n][8] weights of the 2^3 corners (corner c = x + 2y + 4z) for
+    fractions frac [n][3]."""
+    f = np.asarray(frac, np.float64)
+    w = np.empty((f.shape[0], 8))
+    for c in range(8):
+        w[:, c] = ((f[:, 0] if c & 1 else 1 - f[:, 0]) * (f[:, 1] if c & 2 else 1 - f[:, 1])
+                   * (f[:, 2] if c & 4 else 1 - f[:, 2]))
+    return w
+
+
+def cic(out: np.ndarray, frac: np.ndarray) -> np.ndarray:
+    """Predictions [n][3] from outputs [n][2][2][2][3] (or [n][24])."""
+    o = np.asarray(out).reshape(-1, 8, 3)
+    return np.einsum("nc,ncd->nd", cic_weights(frac), o)
+
+
 def forward(params, x: np.ndarray):
-    """x [B][9][9][9][1] -> (out [B][3], cache for backward)."""
+    """x [B][D][D][D][c_in] -> (out [B][(D-8)^3 * 3], cache for backward)."""
     B = x.shape[0]
     acts, cols = [x], []
     h = x

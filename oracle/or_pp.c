@@ -565,8 +565,21 @@ static double F_ra64(double r, double apm) {
     return 0.0;
 }
 
+/* Plummer softening (no counterpart in the reference): the force of a unit
+ * mass at distance r is r / (r^2 + eps^2)^(3/2), minus the same F_ra(r,
+ * range) the mesh carries; eps = app. */
+static double F_plummer64(double r, double eps) {
+    const double q = r * r + eps * eps;
+    return r / (q * sqrt(q));
+}
+
+/* The softened pair force of the given law (0: S2 F_ra, 1: Plummer). */
+static double F_soft64(double r, double app, int law) {
+    return law == 1 ? F_plummer64(r, app) : F_ra64(r, app);
+}
+
 void or_direct_pp_f64(const double* pos, int n, double box, const int* targets, int nt,
-                      const double* range_t, double app, double* acc) {
+                      const double* range_t, double app, int law, double* acc) {
 #pragma omp parallel for schedule(dynamic, 4)
     for (int q = 0; q < nt; q++) {
         int i = targets[q];
@@ -581,7 +594,7 @@ void or_direct_pp_f64(const double* pos, int n, double box, const int* targets, 
             }
             if (r2 > 0 && r2 < rg * rg) {
                 double r = sqrt(r2);
-                double f = F_ra64(r, app) - F_ra64(r, rg);
+                double f = F_soft64(r, app, law) - F_ra64(r, rg);
                 for (int c = 0; c < 3; c++) a[c] += f * dd[c] / r;
             }
         }
@@ -596,14 +609,14 @@ void or_direct_pp_f64(const double* pos, int n, double box, const int* targets, 
  * cells within its own range (falls back to all pairs when fewer than 3
  * cells fit the box). */
 void or_direct_pp_f64_cells(const double* pos, int n, double box, const int* targets, int nt,
-                            const double* range_t, double app, double* acc) {
+                            const double* range_t, double app, int law, double* acc) {
     double rmin = 0.0;
     for (int q = 0; q < nt; q++)
         if (range_t[q] > 0.0 && (rmin == 0.0 || range_t[q] < rmin)) rmin = range_t[q];
     int nc = rmin > 0.0 ? (int)floor(box / rmin) : 1;
     if (nc > 256) nc = 256;
     if (nc < 3) {
-        or_direct_pp_f64(pos, n, box, targets, nt, range_t, app, acc);
+        or_direct_pp_f64(pos, n, box, targets, nt, range_t, app, law, acc);
         return;
     }
     const double cs = box / nc;
@@ -651,7 +664,7 @@ void or_direct_pp_f64_cells(const double* pos, int n, double box, const int* tar
                         }
                         if (r2 > 0 && r2 < rg * rg) {
                             const double r = sqrt(r2);
-                            const double f = F_ra64(r, app) - F_ra64(r, rg);
+                            const double f = F_soft64(r, app, law) - F_ra64(r, rg);
                             for (int d = 0; d < 3; d++) a[d] += f * dd[d] / r;
                         }
                     }

@@ -129,10 +129,13 @@ void or_velocity_apply(const or_geom* g, const int* rhoc, const int64_t* idx_b_r
 /* FP64 direct sum of the PP kernel over all pairs within range_i, with the
  * periodic minimum image, for a list of target particles.  pos in
  * standard cells [0, box), range per target. */
+/* law 0: the S2 F_ra(r, app) of basic_functions.f08:95-109; law 1: Plummer
+ * r / (r^2 + app^2)^(3/2) (an addition with no counterpart in the
+ * reference); minus F_ra(r, range_i) in both. */
 void or_direct_pp_f64_cells(const double* pos, int n, double box, const int* targets, int nt,
-                            const double* range_t, double app, double* acc);
+                            const double* range_t, double app, int law, double* acc);
 void or_direct_pp_f64(const double* pos /*[n][3]*/, int n, double box, const int* targets,
-                      int nt, const double* range_t, double app, double* acc /*[nt][3]*/);
+                      int nt, const double* range_t, double app, int law, double* acc /*[nt][3]*/);
 
 #ifdef __cplusplus
 }

@@ -221,3 +221,39 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
     finally:
         for key, _ in settings:
             lib.set_tuning(key, 0)
+
+
+@pytest.mark.parametrize("mode", ["ref", "fast", "precise"])
+def test_overlapped_kick_equals_serial_kick(lib, mixed_layout, mode):
+    """PP on the second stream beside PM3 (its pair sums apart from the
+    velocity update, which follows both) gives the serial kick's results bit
+    for bit: the update is vp = RN_f(vp + dv64 + PP term) in the same order,
+    and neither pass reads what the other writes.  Eager and recorded."""
+    import torch
+    from cube2_amd.kick import ShortRangeKick
+    lay, ires = mixed_layout
+    lib.set_pp_mode({"ref": lib.PP_REF, "fast": lib.PP_FAST, "precise": lib.PP_PRECISE}[mode])
+    k = ShortRangeKick(lay)
+    k.set_levels(ires)
+    vp0 = k.d_vp.clone()
+    runs = {}
+    try:
+        for ov, graph in ((2, False), (1, False), (1, True)):
+            lib.set_tuning(lib.TUNE_KICK_OVERLAP, ov)
+            k.d_vp.copy_(vp0)
+            torch.cuda.synchronize()
+            r = k.kick_graph(1.3, 0.05, 2e-3) if graph else k.kick(1.3, 0.05, 2e-3)
+            runs[(ov, graph)] = (r, k.vp_host())
+    finally:
+        lib.set_tuning(lib.TUNE_KICK_OVERLAP, 0)
+        lib.set_pp_mode(lib.PP_REF)
+    rs, vs = runs[(2, False)]
+    for key in ((1, False), (1, True)):
+        r, v = runs[key]
+        print(f"{mode} overlap={key}: vp differs in {int((v != vs).sum())} entries", flush=True)
+        assert np.array_equal(v.view(np.uint32), vs.view(np.uint32)), key
+        assert r.f2max_pm3 == rs.f2max_pm3 and r.f2max_pp == rs.f2max_pp
+        assert np.array_equal(r.vmax, rs.vmax)
+        assert (r.p_cand, r.p_acc) == (rs.p_cand, rs.p_acc)
+        assert np.array_equal(r.ptotal_pm3, rs.ptotal_pm3)
+        assert np.array_equal(r.ptotal_pp, rs.ptotal_pp)

@@ -77,10 +77,16 @@ def test_ref_pair_forces_on_golden_fra_points(lib, oracle_lib):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), float(ap)
 
 
-@pytest.mark.parametrize("mode,bound", [("fast", 5e-5), ("precise", 1e-6)])
+def _plummer64(r, eps):
+    """r / (r^2 + eps^2)^(3/2) in FP64 (Plummer softening; eps = 0: 1/r^2)."""
+    q = r * r + eps * eps
+    return r / (q * np.sqrt(q))
+
+
+@pytest.mark.parametrize("mode,bound", [("fast", 5e-5), ("precise", 1e-6), ("plummer", 1e-6)])
 def test_other_arithmetics_against_fp64_law(lib, mode, bound):
     from cube2_amd.geometry import APM3
-    m = {"fast": lib.PP_FAST, "precise": lib.PP_PRECISE}[mode]
+    m = {"fast": lib.PP_FAST, "precise": lib.PP_PRECISE, "plummer": lib.PP_PLUMMER}[mode]
     worst = 0.0
     for app in APPS:
         rng = np.random.default_rng(int(app * 1000) + 17)
@@ -90,7 +96,8 @@ def test_other_arithmetics_against_fp64_law(lib, mode, bound):
         r = np.sqrt((d64 ** 2).sum(1))
         r2f = (d * d).sum(1, dtype=np.float32)
         acc = (r2f > 0) & (r2f < rg * rg)
-        f = _fra64(r, app) - _fra64(r, rg)
+        soft = _plummer64(r, app) if mode == "plummer" else _fra64(r, app)
+        f = soft - _fra64(r, rg)
         want = np.where(acc[:, None], (f / r)[:, None] * d64, 0.0)
         err = np.abs(got - want).max(1) * r ** 2
         worst = max(worst, float(err[acc].max()))

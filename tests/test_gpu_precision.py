@@ -5,7 +5,9 @@ pair arithmetic (REF: the reference's FP32 rounding; FAST; PRECISE: FP64
 pair sums of a cancellation-free law) the PP Delta-v of a seeded 10^4
 particle subsample is compared with an FP64 direct sum of the same law
 over the same pairs (the oracle); relative RMS and the largest error
-(relative to the RMS force) are printed and bounded."""
+(relative to the RMS force) are printed and bounded.  The Plummer law
+(PLUMMER: r/(r^2 + eps^2)^(3/2) with eps the softening, in PRECISE's
+arithmetic) is held to an FP64 direct sum of the Plummer law at each eps."""
 from __future__ import annotations
 
 import dataclasses
@@ -21,7 +23,7 @@ EPS_FINE = [0.02, 0.05, 0.1]
 # chunks in dense cells: ref and fast about 2e-6, precise about 9e-8, at every
 # softening; 64-particle chunks, one FP32 partial sum per i, gave 9e-6 for
 # ref and fast)
-BOUND = {"ref": 1e-5, "fast": 1e-5, "precise": 4e-7}
+BOUND = {"ref": 1e-5, "fast": 1e-5, "precise": 4e-7, "plummer": 1e-6}
 
 
 @pytest.fixture(scope="module")
@@ -65,11 +67,14 @@ def test_precision_sweep_against_fp64(c1, oracle_lib):
     for eps in EPS_FINE:
         g = lay0.geom.with_(app=eps * 0.5)
         lay = dataclasses.replace(lay0, geom=g)
-        acc = oracle_lib.direct_pp_f64(pdec, g.ng, tg, ranges, g.app, cells=True)
-        truth = acc * amid * dt / (6 * math.pi)
-        scale = math.sqrt((truth ** 2).sum(1).mean())
+        truths = {}
+        for law in ("s2", "plummer"):
+            acc = oracle_lib.direct_pp_f64(pdec, g.ng, tg, ranges, g.app, cells=True, law=law)
+            truths[law] = acc * amid * dt / (6 * math.pi)
         for name, mode in (("ref", _lib.PP_REF), ("fast", _lib.PP_FAST),
-                           ("precise", _lib.PP_PRECISE)):
+                           ("precise", _lib.PP_PRECISE), ("plummer", _lib.PP_PLUMMER)):
+            truth = truths["plummer" if name == "plummer" else "s2"]
+            scale = math.sqrt((truth ** 2).sum(1).mean())
             _lib.set_pp_mode(mode)
             k = ShortRangeKick(lay)
             k.set_levels(ires)

@@ -27,18 +27,34 @@ def test_one_layer_is_a_direct_convolution():
     assert np.isclose(y[z, yy, xx, co], s, rtol=1e-12)
 
 
+def test_cic_weights_interpolate_linear_fields_exactly():
+    """The 2^3 combination reproduces a linear field sampled at the corners
+    (cloud-in-cell), and its weights sum to 1."""
+    rng = np.random.default_rng(5)
+    frac = rng.uniform(0, 1, (20, 3))
+    a, g = rng.normal(size=3), rng.normal(size=(3, 3))
+    corners = np.array([[c & 1, (c >> 1) & 1, c >> 2] for c in range(8)], np.float64)
+    out = (a[None, None, :] + corners[None, :, :] @ g[None, :, :]).repeat(20, 0)
+    assert np.allclose(cnn.cic_weights(frac).sum(1), 1.0)
+    assert np.allclose(cnn.cic(out, frac), a + frac @ g, rtol=1e-13)
+
+
 def test_backward_matches_central_differences():
+    """Through the cloud-in-cell combination of the 2^3 outputs, as trained."""
     rng = np.random.default_rng(2)
     params = cnn.init_params(3)
     params = [(W, rng.normal(0, 0.1, b.shape)) for W, b in params]
-    x = rng.uniform(0, 2, (3, 9, 9, 9, 1))
+    x = rng.uniform(0, 2, (3, 10, 10, 10, 4))
     t = rng.normal(size=(3, 3))
+    frac = rng.uniform(0, 1, (3, 3))
+    w = cnn.cic_weights(frac)
     out, cache = cnn.forward(params, x)
-    _, dout = cnn.mse(out, t)
+    _, dpred = cnn.mse(cnn.cic(out, frac), t)
+    dout = (w[:, :, None] * dpred[:, None, :]).reshape(3, 24)
     grads = cnn.backward(params, cache, dout)
 
     def loss(p):
-        return cnn.mse(cnn.forward(p, x)[0], t)[0]
+        return cnn.mse(cnn.cic(cnn.forward(p, x)[0], frac), t)[0]
 
     h = 1e-6
     for l in range(4):

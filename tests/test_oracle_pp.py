@@ -171,3 +171,37 @@ def test_pair_forces_compose_the_reference_f_ra(oracle_lib):
                 fpp = f32(fra(rm, app) - fra(rm, rg[q]))
                 want = np.array([f32(fpp * c) / rm for c in (x, y, z)], np.float32)
             assert np.array_equal(got[q].view(np.uint32), want.view(np.uint32)), (q, d[q], rg[q])
+
+
+def test_plummer_direct_sum_follows_its_potential(oracle_lib):
+    """The oracle's Plummer law (law="plummer"): for one pair at distance r,
+    the acceleration minus that of the unsoftened S2 law (app = 0) is
+    F_pl(r) - 1/r^2 along the pair, with the attraction F_pl = dPhi/dr of
+    Phi = -1/sqrt(r^2 + eps^2) taken here by central differences of the
+    potential; at eps -> 0 it
+    is the unsoftened law; the force peaks at r = eps/sqrt(2) with
+    2/(3 sqrt(3) eps^2)."""
+    eps, rg = 0.05, 1.75
+    box = 32.0
+    phi = lambda r: -1.0 / np.sqrt(r * r + eps * eps)  # noqa: E731
+    rs = np.concatenate([np.linspace(0.004, 0.3, 60), np.linspace(0.3, 1.7, 30)])
+    for r in rs:
+        u = np.array([0.6, -0.48, 0.64])
+        p = np.array([[10.0, 11.0, 12.0], [10.0, 11.0, 12.0] + r * u])
+        a_pl = oracle_lib.direct_pp_f64(p, box, [0], [rg], eps, law="plummer")[0]
+        a_0 = oracle_lib.direct_pp_f64(p, box, [0], [rg], 0.0)[0]
+        h = 1e-5 * r
+        f_fd = (phi(r + h) - phi(r - h)) / (2 * h)
+        want = (f_fd - 1.0 / r ** 2) * u
+        assert np.allclose(a_pl - a_0, want, rtol=1e-6, atol=1e-9 / r ** 2), r
+        a_tiny = oracle_lib.direct_pp_f64(p, box, [0], [rg], 1e-9, law="plummer")[0]
+        assert np.allclose(a_tiny, a_0, rtol=0, atol=1e-12 / r ** 2)
+    # peak of the softened force (the S2 range term is smooth there)
+    r = np.linspace(0.6, 0.8, 2001) * eps
+    f = r / (r * r + eps * eps) ** 1.5
+    assert abs(r[np.argmax(f)] - eps / np.sqrt(2)) < 2e-4 * eps
+    assert np.isclose(f.max(), 2 / (3 * np.sqrt(3) * eps * eps), rtol=1e-7)
+    p = np.array([[10.0, 11.0, 12.0], [10.0 + eps / np.sqrt(2), 11.0, 12.0]])
+    a_pl = oracle_lib.direct_pp_f64(p, box, [0], [rg], eps, law="plummer")[0]
+    a_0 = oracle_lib.direct_pp_f64(p, box, [0], [rg], 0.0)[0]
+    assert np.isclose(a_pl[0] - a_0[0], 2 / (3 * np.sqrt(3) * eps * eps) - 2 / eps ** 2, rtol=1e-12)
