@@ -121,13 +121,6 @@ std::recursive_mutex& api_mutex();
 int scalars_upload(float mass, float a_mid, float dt, hipStream_t s, const float** dev);   // params + HIP device + library stream
 void ensure_params();  // params only (no HIP call)
 
-// Blocks per CU of the pair kernel while PM3 runs beside it (built-in
-// choice of CUBE2_TUNE_PP_OVERLAP_BLOCKS, capped at the occupancy maximum).
-// Measured at 256^3 (kick ms, graph replay; serial 145.5): 1: 248.5, 2: 205.4,
-// 3: 161.4, 4: 136.8, 5: 126.2, 6 (the maximum): 125.1.  The mesh kernels'
-// workgroups take the slots of pair-kernel workgroups as those retire, so
-// leaving slots empty only slows the pair kernel.
-constexpr int kPPOverlapBlocks = 16;
 
 // Whether the kick driver overlaps PP with PM3 (CUBE2_TUNE_KICK_OVERLAP).
 inline bool kick_overlap() { return ctx().tune[CUBE2_TUNE_KICK_OVERLAP] != 2; }

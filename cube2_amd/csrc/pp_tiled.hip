@@ -527,8 +527,11 @@ int pp_tiled_grid(int mode, bool overlap) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             ncu = 256;
+        // explicit choice: may exceed the resident maximum, so that blocks
+        // retire throughout the launch and the mesh kernels' workgroups take
+        // their slots
         const int t = ctx().tune[CUBE2_TUNE_PP_OVERLAP_BLOCKS];
-        return std::min(full, std::max(1, ncu) * (t ? t : kPPOverlapBlocks));
+        return t ? std::max(1, ncu) * t : full;
     }
     if (forced) {
         if (hipGetDevice(&dev) != hipSuccess ||

@@ -129,8 +129,9 @@ int cube2_set_pp_mode(int mode);
  *   KICK_OVERLAP      1: the kick driver runs PP's cell lists and pair sums
  *                     on a second stream beside PM3, the velocity update
  *                     after both (built-in); 2: PM3, then PP;
- *   PP_OVERLAP_BLOCKS 1..16, blocks per CU of the pair kernel while PM3
- *                     runs beside it (built-in: see the library).
+ *   PP_OVERLAP_BLOCKS 1..64, blocks per CU of the pair kernel while PM3
+ *                     runs beside it, beyond the resident maximum if asked
+ *                     (built-in: the resident maximum).
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
