@@ -198,7 +198,7 @@ def bench_c2(args) -> int:
     cold_ms = 1e3 * (time.perf_counter() - t1)
     for _ in range(args.warmup):
         k.kick(mass, amid, dt)
-    tuning = None
+    tuning = apply_tune_args(args)
     if args.autotune:
         from cube2_amd.autotune import autotune
         tuning = autotune(k, mass, amid, dt, log=log)
@@ -242,6 +242,19 @@ def bench_c2(args) -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def apply_tune_args(args):
+    """Sets the tunables given by --tune NAME=VALUE; returns them, or None."""
+    if not args.tune:
+        return None
+    chosen = {}
+    for item in args.tune:
+        name, _, value = item.partition("=")
+        _lib.set_tuning(_lib.TUNE_NAMES[name], int(value))
+        chosen[name] = int(value)
+    log(f"tunables set: {chosen}")
+    return {"chosen": chosen, "source": "--tune"}
 
 
 def bench_c4(args) -> int:
@@ -326,8 +339,11 @@ def bench_c4(args) -> int:
 
     plain_ms, rp, vp_plain = timed(False)
     learned_ms, rl, vp_learned = timed(True)
-    # stage profile of one eager learned kick (pair kernel, network stages)
+    # stage profile of one eager learned kick and one plain kick, with PM3 and PP
+    # serial so that each stage's events time it alone (pair kernel, network)
+    _lib.set_tuning(_lib.TUNE_KICK_OVERLAP, 2)
     k.d_vp.copy_(vp0)
+    torch.cuda.synchronize()
     _lib.profile(True)
     _lib.profile_reset()
     k.kick(mass, amid, dt, learned=True)
@@ -335,10 +351,12 @@ def bench_c4(args) -> int:
     st_l = _lib.profile_stats()
     _lib.profile_reset()
     k.d_vp.copy_(vp0)
+    torch.cuda.synchronize()
     k.kick(mass, amid, dt)
     torch.cuda.synchronize()
     st_p = _lib.profile_stats()
     _lib.profile(False)
+    _lib.set_tuning(_lib.TUNE_KICK_OVERLAP, 0)
     _lib.lib().cube2_learned_clear()
     dv_p = (vp_plain - vp0).double()
     dv_l = (vp_learned - vp0).double()
@@ -393,6 +411,8 @@ def main(argv=None) -> int:
     ap.add_argument("--pp-mode", default="ref", choices=["ref", "exact", "fast", "precise", "plummer"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a library tunable (names as in cube2_amd._lib.TUNE_NAMES)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch-shape tunables on this workload before timing")
     ap.add_argument("--no-graph", action="store_true",
@@ -454,7 +474,7 @@ def main(argv=None) -> int:
         t1 = time.perf_counter()
         res = k.kick(mass, amid, dt)
         log(f"warmup {w}: {1e3 * (time.perf_counter() - t1):.1f} ms")
-    tuning = None
+    tuning = apply_tune_args(args)
     if args.autotune:
         from cube2_amd.autotune import autotune
         tuning = autotune(k, mass, amid, dt, log=log)

@@ -41,6 +41,7 @@ int pp_tiled_grid(int mode, bool overlap);
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
     DevBuf chunk_flags, chunks, ilist, big, dense;
+    DevBuf order, order_bin, order_hist;  // chunks heaviest first (k_chunk_cost/order)
     DevBuf term;  // split pass: per sorted particle velocity term [cap][3]
     HostBuf h_res;  // PPRed, per-location counts, per-tile pairs, tile_first (pinned)
     std::vector<PPTile> h_tiles;
@@ -410,6 +411,110 @@ __global__ void k_chunk_emit(const int* __restrict__ scanned, int n, int* __rest
     if (scanned[a + 1] != scanned[a]) chunks[scanned[a]] = a;
 }
 
+// Chunk order of the pair kernel: heaviest first.  Its wavefronts take chunks
+// by a static stride, so on a list sorted by decreasing cost two wavefronts'
+// totals differ by at most the cost of one chunk, where in cell order the
+// clumps' heavy chunks gather on some wavefronts (at 256^3 REF they ended
+// between 32 and 80 ms).  The cost is the chunk's candidate pairs, (its i
+// slots + 1) x (the j in its neighbour rows), as 8 steps per octave; the
+// sort is a stable counting sort over these 256 bins (k_chunk_cost, a scan,
+// k_chunk_order), so chunks of equal bin keep their cell order.
+constexpr int kOrderBins = 256;
+constexpr int kOrderBlocks = 1024;
+
+__device__ inline void order_segment(int n, int& beg, int& end) {
+    const int seg = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    beg = min(n, (int)blockIdx.x * seg);
+    end = min(n, beg + seg);
+}
+
+__global__ __launch_bounds__(kOrderBins) void k_chunk_cost(PPArgs A, const int* __restrict__ chunks,
+                                                        const int* __restrict__ nchunks_dev,
+                                                        int dense_lg, unsigned char* __restrict__ bin,
+                                                        int* __restrict__ hist) {
+    __shared__ int h[kOrderBins];
+    int beg, end;
+    order_segment(*nchunks_dev, beg, end);
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int ck = beg + (int)threadIdx.x; ck < end; ck += kOrderBins) {
+        const int a0 = chunks[ck];
+        const uint32_t cid = A.cellid[a0];
+        const int cbeg = A.cell_start[cid], cend = A.cell_start[cid + 1];
+        const int ni = min(cend - cbeg >= 64 ? 1 << dense_lg : 64, cend - a0);
+        int I = 1;
+        while (I < ni) I <<= 1;
+        int lo = 0, hi = A.ntile - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (A.tiles[mid].cell_off <= (long long)cid) lo = mid;
+            else hi = mid - 1;
+        }
+        const PPTile T = A.tiles[lo];
+        const long long key = (long long)cid - T.cell_off;
+        const int S = T.S;
+        const int x = (int)(key % S), y = (int)((key / S) % S), z = (int)(key / ((long long)S * S));
+        const int R = (int)ceilf(A.pos4[a0].w * (float)T.rcp / (float)A.ratio_cs);
+        const int xlo = max(x - R, 0), xhi = min(x + R, S - 1);
+        long long M = 0;
+        for (int zz = max(z - R, 0); zz <= min(z + R, S - 1); zz++)
+            for (int yy = max(y - R, 0); yy <= min(y + R, S - 1); yy++) {
+                const long long row = T.cell_off + ((long long)zz * S + yy) * S;
+                M += A.cell_start[row + xhi + 1] - A.cell_start[row + xlo];
+            }
+        const float cost = (float)M * (float)(I + 1);
+        const int k = min(kOrderBins - 1, (int)(8.0f * log2f(cost + 1.0f)));
+        const int b = kOrderBins - 1 - k;
+        bin[ck] = (unsigned char)b;
+        atomicAdd(&h[b], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];  // bin-major for the scan
+}
+
+// Stable scatter of the chunks by bin: within a block's segment, 256 chunks
+// at a time in order; a chunk's place is its bin's base, the chunks of its bin
+// in the earlier wavefronts of the step, and its rank among its wavefront's
+// lanes of the same bin (found with 8 ballots on the bin's bits).
+__global__ __launch_bounds__(kOrderBins) void k_chunk_order(const int* __restrict__ chunks,
+                                                         const unsigned char* __restrict__ bin,
+                                                         const int* __restrict__ nchunks_dev,
+                                                         const int* __restrict__ hist,
+                                                         int* __restrict__ out) {
+    __shared__ int base[kOrderBins];
+    __shared__ int wc[4][kOrderBins];
+    int beg, end;
+    order_segment(*nchunks_dev, beg, end);
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    base[t] = hist[t * gridDim.x + blockIdx.x];
+    wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
+    __syncthreads();
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int t0 = beg; t0 < end; t0 += kOrderBins) {  // block-uniform trip count
+        const int ck = t0 + t;
+        const bool valid = ck < end;
+        const int b = valid ? bin[ck] : 0;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const uint64_t m = __ballot(valid && ((b >> q) & 1));
+            peers &= ((b >> q) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wc[wave][b] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pos = base[b] + rank;
+            for (int w = 0; w < wave; w++) pos += wc[w][b];
+            out[pos] = chunks[ck];
+        }
+        __syncthreads();
+        base[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+        wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
+        __syncthreads();
+    }
+}
+
 // Spreads the low 3 bits of v to bits 0, 3, 6 (Morton interleave).
 __device__ inline int spread3(int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4); }
 
@@ -639,6 +744,11 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.tile_first.reserve((P.ntile + 1) * sizeof(int)));
     C2_TRY(g_pp.chunk_flags.reserve((P.cap + 1) * sizeof(int)));
     C2_TRY(g_pp.chunks.reserve((P.cap + 1) * sizeof(int)));
+    if (pp_order_chunks()) {
+        C2_TRY(g_pp.order.reserve((P.cap + 1) * sizeof(int)));
+        C2_TRY(g_pp.order_bin.reserve(P.cap));
+        C2_TRY(g_pp.order_hist.reserve(((size_t)kOrderBins * kOrderBlocks + 1) * sizeof(int)));
+    }
     C2_TRY(g_pp.ilist.reserve(P.cap * sizeof(int)));
     C2_TRY(g_pp.big.reserve(std::max<int64_t>(P.ncells_ext, 1) * sizeof(int)));
     C2_TRY(g_pp.dense.reserve((P.cap / 64 + 1) * sizeof(int)));
@@ -759,7 +869,20 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
                 g.p.ratio_cs, g_pp.ilist.as<int>());
             C2_HIP(hipGetLastError());
         }
-        C2_TRY(launch_pp_tiled(A, g_pp.chunks.as<int>(), flags + P.cap, P.grid_pairs, P.mode, s));
+        const int* chunks = g_pp.chunks.as<int>();
+        if (pp_order_chunks()) {
+            Stage st("pp_order", (double)P.cap * 9);
+            int* hist = g_pp.order_hist.as<int>();
+            k_chunk_cost<<<kOrderBlocks, kOrderBins, 0, s>>>(A, chunks, flags + P.cap, pp_dense_lg(),
+                                                             g_pp.order_bin.as<unsigned char>(), hist);
+            C2_HIP(hipGetLastError());
+            C2_TRY(exclusive_scan(hist, (int64_t)kOrderBins * kOrderBlocks, g_pp.scan_tmp, s));
+            k_chunk_order<<<kOrderBlocks, kOrderBins, 0, s>>>(chunks, g_pp.order_bin.as<unsigned char>(),
+                                                              flags + P.cap, hist, g_pp.order.as<int>());
+            C2_HIP(hipGetLastError());
+            chunks = g_pp.order.as<int>();
+        }
+        C2_TRY(launch_pp_tiled(A, chunks, flags + P.cap, P.grid_pairs, P.mode, s));
         if (P.learned) C2_TRY(learned_enqueue(P.lp, A, g_pp.tile_first.as<int>(), s));
     }
     if (P.split) return CUBE2_OK;  // pp_enqueue_apply updates vp and copies the results

@@ -81,4 +81,8 @@ inline int pp_dense_lg() {
     return v ? v : 4;
 }
 
+// Whether the pair kernel takes its chunks heaviest first (built in) or in
+// cell order.
+inline bool pp_order_chunks() { return ctx().tune[CUBE2_TUNE_PP_ORDER] != 2; }
+
 }  // namespace cube2

@@ -131,7 +131,9 @@ int cube2_set_pp_mode(int mode);
  *                     after both (built-in); 2: PM3, then PP;
  *   PP_OVERLAP_BLOCKS 1..64, blocks per CU of the pair kernel while PM3
  *                     runs beside it, beyond the resident maximum if asked
- *                     (built-in: the resident maximum).
+ *                     (built-in: the resident maximum);
+ *   PP_ORDER          1: the pair kernel takes its chunks in decreasing order
+ *                     of their candidate pairs (built-in); 2: in cell order.
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
@@ -141,7 +143,8 @@ enum {
     CUBE2_TUNE_PP_DENSE_LG = 4,
     CUBE2_TUNE_KICK_OVERLAP = 5,
     CUBE2_TUNE_PP_OVERLAP_BLOCKS = 6,
-    CUBE2_TUNE_COUNT = 7
+    CUBE2_TUNE_PP_ORDER = 7,
+    CUBE2_TUNE_COUNT = 8
 };
 int cube2_set_tuning(int key, int value);
 int cube2_get_tuning(int key, int* value); /* the value in use (built-in choice resolved) */

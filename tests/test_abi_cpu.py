@@ -164,4 +164,6 @@ def test_tuning_api_validates_and_round_trips():
     assert _lib.get_tuning(_lib.TUNE_INTERP_SEG) == 4          # built-in choice
     assert _lib.get_tuning(_lib.TUNE_PP_DIRECT_LG) == 2
     assert _lib.get_tuning(_lib.TUNE_PP_DENSE_LG) == 4
+    assert _lib.get_tuning(_lib.TUNE_PP_ORDER) == 1
+    assert L.cube2_set_tuning(_lib.TUNE_PP_ORDER, 3) == _lib.ERR_ARG
     assert L.cube2_set_tuning(_lib.TUNE_PP_DENSE_LG, 2) == _lib.ERR_ARG

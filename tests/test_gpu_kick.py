@@ -205,7 +205,8 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
     r0 = k.kick_graph(1.0, 0.02, 1e-3)
     base = k.vp_host()
     settings = [(lib.TUNE_PP_BLOCKS_PER_CU, 3), (lib.TUNE_PP_DIRECT_LG, 6),
-                (lib.TUNE_FFT_MIN_LGC, 4), (lib.TUNE_INTERP_SEG, 2), (lib.TUNE_PP_DENSE_LG, 6)]
+                (lib.TUNE_FFT_MIN_LGC, 4), (lib.TUNE_INTERP_SEG, 2), (lib.TUNE_PP_DENSE_LG, 6),
+                (lib.TUNE_PP_ORDER, 2)]
     try:
         for key, v in settings:
             lib.set_tuning(key, v)
@@ -221,6 +222,38 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
     finally:
         for key, _ in settings:
             lib.set_tuning(key, 0)
+
+
+@pytest.mark.parametrize("mode", ["ref", "fast"])
+def test_chunk_order_changes_no_result(lib, mixed_layout, mode):
+    """The pair kernel's chunks heaviest first or in cell order: every particle
+    is summed within its own chunk, so velocities, maxima and pair counts are
+    the same bit for bit (a chunk lost or taken twice would change the
+    velocities or the counts)."""
+    import torch
+    from cube2_amd.kick import ShortRangeKick
+    lay, ires = mixed_layout
+    lib.set_pp_mode({"ref": lib.PP_REF, "fast": lib.PP_FAST}[mode])
+    k = ShortRangeKick(lay)
+    k.set_levels(ires)
+    vp0 = k.d_vp.clone()
+    runs = {}
+    try:
+        for order in (2, 1):
+            lib.set_tuning(lib.TUNE_PP_ORDER, order)
+            k.d_vp.copy_(vp0)
+            torch.cuda.synchronize()
+            r = k.kick(1.3, 0.05, 2e-3)
+            runs[order] = (r, k.vp_host())
+    finally:
+        lib.set_tuning(lib.TUNE_PP_ORDER, 0)
+        lib.set_pp_mode(lib.PP_REF)
+    (rc, vc), (rs, vs) = runs[2], runs[1]
+    print(f"{mode}: vp differs in {int((vs != vc).sum())} entries", flush=True)
+    assert np.array_equal(vs.view(np.uint32), vc.view(np.uint32))
+    assert rs.f2max_pp == rc.f2max_pp and np.array_equal(rs.vmax, rc.vmax)
+    assert (rs.p_cand, rs.p_acc) == (rc.p_cand, rc.p_acc)
+    assert np.array_equal(rs.ptotal_pp, rc.ptotal_pp)
 
 
 @pytest.mark.parametrize("mode", ["ref", "fast", "precise"])
