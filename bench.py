@@ -248,6 +248,7 @@ def apply_tune_args(args):
     """Sets the tunables given by --tune NAME=VALUE; returns them, or None."""
     if not args.tune:
         return None
+    from cube2_amd import _lib
     chosen = {}
     for item in args.tune:
         name, _, value = item.partition("=")

@@ -19,6 +19,7 @@ CANDIDATES = {
     "pp_direct_lg": [0, 7, 5, 4],
     "fft_min_lgc": [0, 3, 4],
     "interp_seg": [0, 2, 8],
+    "pp_order": [0, 20],
 }
 
 

@@ -411,14 +411,23 @@ __global__ void k_chunk_emit(const int* __restrict__ scanned, int n, int* __rest
     if (scanned[a + 1] != scanned[a]) chunks[scanned[a]] = a;
 }
 
-// Chunk order of the pair kernel: heaviest first.  Its wavefronts take chunks
-// by a static stride, so on a list sorted by decreasing cost two wavefronts'
-// totals differ by at most the cost of one chunk, where in cell order the
-// clumps' heavy chunks gather on some wavefronts (at 256^3 REF they ended
-// between 32 and 80 ms).  The cost is the chunk's candidate pairs, (its i
-// slots + 1) x (the j in its neighbour rows), as 8 steps per octave; the
-// sort is a stable counting sort over these 256 bins (k_chunk_cost, a scan,
-// k_chunk_order), so chunks of equal bin keep their cell order.
+// Chunk order of the pair kernel.  Its wavefronts take chunks by a static
+// stride over the list; in cell order the 6144 chunks in flight at a time
+// are neighbours, whose j the L2 caches share, but a chunk in a clump's core
+// can cost as much as a wavefront's average total, and where such chunks
+// fall late a wavefront ends late (at 256^3 REF wavefronts ended between 32
+// and 80 ms).  On request (a tunable; off by default) the heavy chunks, whose
+// estimated candidate pairs are at least 2^heavy_lg, go first, heaviest
+// first, and the rest follow in cell order.  The estimate is (i slots + 1) x
+// (particles of the chunk's PP cell) x (2R+1)^3, as 8 steps per octave; the
+// sort is a stable counting sort over 256 bins (k_chunk_cost, a scan,
+// k_chunk_order), all light chunks in the last bin, 0.3 ms at 256^3.
+// Measured at 256^3 REF (pair kernel alone): cell order 80.9 ms; heavy
+// first from 2^14, 2^17, 2^20, 2^22: 86.6, 86.3, 86.3, 88.9 ms; from 2^24
+// (few chunks) 80.5 ms; the whole list by cost (the in-flight chunks
+// scattered over the volume) 87.2 ms.  The wavefronts that end early leave
+// their SIMD's issue slots to the others, so the end-time spread costs less
+// than the heavy chunks lose by running together.
 constexpr int kOrderBins = 256;
 constexpr int kOrderBlocks = 1024;
 
@@ -430,7 +439,8 @@ __device__ inline void order_segment(int n, int& beg, int& end) {
 
 __global__ __launch_bounds__(kOrderBins) void k_chunk_cost(PPArgs A, const int* __restrict__ chunks,
                                                         const int* __restrict__ nchunks_dev,
-                                                        int dense_lg, unsigned char* __restrict__ bin,
+                                                        int dense_lg, int heavy_lg,
+                                                        unsigned char* __restrict__ bin,
                                                         int* __restrict__ hist) {
     __shared__ int h[kOrderBins];
     int beg, end;
@@ -450,23 +460,17 @@ __global__ __launch_bounds__(kOrderBins) void k_chunk_cost(PPArgs A, const int* 
             if (A.tiles[mid].cell_off <= (long long)cid) lo = mid;
             else hi = mid - 1;
         }
-        const PPTile T = A.tiles[lo];
-        const long long key = (long long)cid - T.cell_off;
-        const int S = T.S;
-        const int x = (int)(key % S), y = (int)((key / S) % S), z = (int)(key / ((long long)S * S));
-        const int R = (int)ceilf(A.pos4[a0].w * (float)T.rcp / (float)A.ratio_cs);
-        const int xlo = max(x - R, 0), xhi = min(x + R, S - 1);
-        long long M = 0;
-        for (int zz = max(z - R, 0); zz <= min(z + R, S - 1); zz++)
-            for (int yy = max(y - R, 0); yy <= min(y + R, S - 1); yy++) {
-                const long long row = T.cell_off + ((long long)zz * S + yy) * S;
-                M += A.cell_start[row + xhi + 1] - A.cell_start[row + xlo];
-            }
-        const float cost = (float)M * (float)(I + 1);
+        const int rcp = A.tiles[lo].rcp;
+        const int W = 2 * (int)ceilf(A.pos4[a0].w * (float)rcp / (float)A.ratio_cs) + 1;
+        const float cost = (float)(I + 1) * (float)(cend - cbeg) * (float)(W * W * W);
         const int k = min(kOrderBins - 1, (int)(8.0f * log2f(cost + 1.0f)));
-        const int b = kOrderBins - 1 - k;
+        const int b = k >= 8 * heavy_lg ? kOrderBins - 1 - k : kOrderBins - 1;
         bin[ck] = (unsigned char)b;
-        atomicAdd(&h[b], 1);
+        // the light chunks share the last bin: one LDS atomic per wavefront
+        const uint64_t lm = __ballot(b == kOrderBins - 1);
+        if (b != kOrderBins - 1) atomicAdd(&h[b], 1);
+        else if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)lm) - 1)
+            atomicAdd(&h[b], __popcll(lm));
     }
     __syncthreads();
     hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];  // bin-major for the scan
@@ -744,7 +748,7 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.tile_first.reserve((P.ntile + 1) * sizeof(int)));
     C2_TRY(g_pp.chunk_flags.reserve((P.cap + 1) * sizeof(int)));
     C2_TRY(g_pp.chunks.reserve((P.cap + 1) * sizeof(int)));
-    if (pp_order_chunks()) {
+    if (pp_heavy_lg()) {
         C2_TRY(g_pp.order.reserve((P.cap + 1) * sizeof(int)));
         C2_TRY(g_pp.order_bin.reserve(P.cap));
         C2_TRY(g_pp.order_hist.reserve(((size_t)kOrderBins * kOrderBlocks + 1) * sizeof(int)));
@@ -870,10 +874,11 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             C2_HIP(hipGetLastError());
         }
         const int* chunks = g_pp.chunks.as<int>();
-        if (pp_order_chunks()) {
+        if (pp_heavy_lg()) {
             Stage st("pp_order", (double)P.cap * 9);
             int* hist = g_pp.order_hist.as<int>();
             k_chunk_cost<<<kOrderBlocks, kOrderBins, 0, s>>>(A, chunks, flags + P.cap, pp_dense_lg(),
+                                                             pp_heavy_lg(),
                                                              g_pp.order_bin.as<unsigned char>(), hist);
             C2_HIP(hipGetLastError());
             C2_TRY(exclusive_scan(hist, (int64_t)kOrderBins * kOrderBlocks, g_pp.scan_tmp, s));

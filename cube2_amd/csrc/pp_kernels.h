@@ -81,8 +81,12 @@ inline int pp_dense_lg() {
     return v ? v : 4;
 }
 
-// Whether the pair kernel takes its chunks heaviest first (built in) or in
-// cell order.
-inline bool pp_order_chunks() { return ctx().tune[CUBE2_TUNE_PP_ORDER] != 2; }
+// log2 of the estimated candidate pairs from which the pair kernel takes a
+// chunk ahead of the others (heaviest first), or 0: all in cell order
+// (built in).
+inline int pp_heavy_lg() {
+    const int v = ctx().tune[CUBE2_TUNE_PP_ORDER];
+    return v > 1 ? v : 0;
+}
 
 }  // namespace cube2

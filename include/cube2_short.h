@@ -132,8 +132,10 @@ int cube2_set_pp_mode(int mode);
  *   PP_OVERLAP_BLOCKS 1..64, blocks per CU of the pair kernel while PM3
  *                     runs beside it, beyond the resident maximum if asked
  *                     (built-in: the resident maximum);
- *   PP_ORDER          1: the pair kernel takes its chunks in decreasing order
- *                     of their candidate pairs (built-in); 2: in cell order.
+ *   PP_ORDER          1: the pair kernel takes its chunks in cell order
+ *                     (built-in); 2..30: chunks of an estimated 2^v
+ *                     candidate pairs or more first, heaviest first, then
+ *                     the rest in cell order.
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,

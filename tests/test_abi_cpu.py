@@ -165,5 +165,5 @@ def test_tuning_api_validates_and_round_trips():
     assert _lib.get_tuning(_lib.TUNE_PP_DIRECT_LG) == 2
     assert _lib.get_tuning(_lib.TUNE_PP_DENSE_LG) == 4
     assert _lib.get_tuning(_lib.TUNE_PP_ORDER) == 1
-    assert L.cube2_set_tuning(_lib.TUNE_PP_ORDER, 3) == _lib.ERR_ARG
+    assert L.cube2_set_tuning(_lib.TUNE_PP_ORDER, 31) == _lib.ERR_ARG
     assert L.cube2_set_tuning(_lib.TUNE_PP_DENSE_LG, 2) == _lib.ERR_ARG

@@ -206,7 +206,7 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
     base = k.vp_host()
     settings = [(lib.TUNE_PP_BLOCKS_PER_CU, 3), (lib.TUNE_PP_DIRECT_LG, 6),
                 (lib.TUNE_FFT_MIN_LGC, 4), (lib.TUNE_INTERP_SEG, 2), (lib.TUNE_PP_DENSE_LG, 6),
-                (lib.TUNE_PP_ORDER, 2)]
+                (lib.TUNE_PP_ORDER, 18)]
     try:
         for key, v in settings:
             lib.set_tuning(key, v)
@@ -226,10 +226,10 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
 
 @pytest.mark.parametrize("mode", ["ref", "fast"])
 def test_chunk_order_changes_no_result(lib, mixed_layout, mode):
-    """The pair kernel's chunks heaviest first or in cell order: every particle
-    is summed within its own chunk, so velocities, maxima and pair counts are
-    the same bit for bit (a chunk lost or taken twice would change the
-    velocities or the counts)."""
+    """The pair kernel's chunks in cell order, all sorted heaviest first, or
+    the heavy ones first: every particle is summed within its own chunk, so
+    velocities, maxima and pair counts are the same bit for bit (a chunk lost
+    or taken twice would change the velocities or the counts)."""
     import torch
     from cube2_amd.kick import ShortRangeKick
     lay, ires = mixed_layout
@@ -239,7 +239,7 @@ def test_chunk_order_changes_no_result(lib, mixed_layout, mode):
     vp0 = k.d_vp.clone()
     runs = {}
     try:
-        for order in (2, 1):
+        for order in (1, 2, 9):
             lib.set_tuning(lib.TUNE_PP_ORDER, order)
             k.d_vp.copy_(vp0)
             torch.cuda.synchronize()
@@ -248,12 +248,14 @@ def test_chunk_order_changes_no_result(lib, mixed_layout, mode):
     finally:
         lib.set_tuning(lib.TUNE_PP_ORDER, 0)
         lib.set_pp_mode(lib.PP_REF)
-    (rc, vc), (rs, vs) = runs[2], runs[1]
-    print(f"{mode}: vp differs in {int((vs != vc).sum())} entries", flush=True)
-    assert np.array_equal(vs.view(np.uint32), vc.view(np.uint32))
-    assert rs.f2max_pp == rc.f2max_pp and np.array_equal(rs.vmax, rc.vmax)
-    assert (rs.p_cand, rs.p_acc) == (rc.p_cand, rc.p_acc)
-    assert np.array_equal(rs.ptotal_pp, rc.ptotal_pp)
+    rc, vc = runs[1]
+    for order in (2, 9):
+        rs, vs = runs[order]
+        print(f"{mode} order {order}: vp differs in {int((vs != vc).sum())} entries", flush=True)
+        assert np.array_equal(vs.view(np.uint32), vc.view(np.uint32)), order
+        assert rs.f2max_pp == rc.f2max_pp and np.array_equal(rs.vmax, rc.vmax)
+        assert (rs.p_cand, rs.p_acc) == (rc.p_cand, rc.p_acc)
+        assert np.array_equal(rs.ptotal_pp, rc.ptotal_pp)
 
 
 @pytest.mark.parametrize("mode", ["ref", "fast", "precise"])
