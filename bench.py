@@ -627,6 +627,23 @@ def main(argv=None) -> int:
                     avg_launch_ms=round(per_launch_ms, 3))
     stages = {n: round(v["ms"] / args.steps, 3) for n, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
     log(f"per-step stage ms: {stages}")
+    if name.startswith("pp_pairs") and _lib.get_tuning(_lib.TUNE_KICK_OVERLAP) == 1:
+        # the pair kernel shares the GPU with PM3 in the timed kicks; one more
+        # kick with PM3 and PP serial times it alone (reported beside the
+        # roofline of the timed region, not in place of it)
+        _lib.set_tuning(_lib.TUNE_KICK_OVERLAP, 2)
+        _lib.profile(True)
+        _lib.profile_reset()
+        k.kick(mass, amid, dt)
+        torch.cuda.synchronize()
+        alone = _lib.profile_stats()[name]
+        _lib.profile(False)
+        _lib.set_tuning(_lib.TUNE_KICK_OVERLAP, 0)
+        alone_ms = alone["ms"] / alone["launches"]
+        roof["alone_avg_launch_ms"] = round(alone_ms, 3)
+        roof["alone_frac"] = round(flops / (alone_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)
+        log(f"pair kernel alone (PM3 and PP serial): {alone_ms:.2f} ms, "
+            f"frac {roof['alone_frac']}")
 
     cpu = None
     if not args.no_cpu:
