@@ -7,7 +7,7 @@
 //            (x_resolution*ratio_cs)) (wrapping).  A slot whose target is a
 //            physical cell of the same tile gets the key (tile, cell, layer
 //            of its source z), the others a key past every valid one;
-//   sort     stable radix sort of (key, slot): within a target cell the
+//   sort     stable radix sort (sort.hip) of (key, slot): within a target cell the
 //            particles come by source layer, then in slot order (source z,
 //            y, x, slot), the arrival order of the layered loop
 //            drift.f90:36-62, 90-122 (one source z per layer can reach a
@@ -17,12 +17,11 @@
 //   gather   xp, vp of the kept slots in sorted order.
 #include <cstring>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include "common.h"
 #include "engine.h"
 #include "profile.h"
 #include "scan.h"
+#include "sort.h"
 
 namespace cube2 {
 
@@ -301,18 +300,9 @@ int drift(const int* d_rhoc, const int64_t* d_idx_b_r, const int16_t* d_xp, cons
     }
     {
         Stage st("drift_sort", (double)n_used * 16);
-        size_t tmp = 0;
-        C2_HIP(rocprim::radix_sort_pairs(nullptr, tmp, g_dr.key.as<uint32_t>(),
-                                         g_dr.key_sorted.as<uint32_t>(), g_dr.val.as<uint32_t>(),
-                                         g_dr.val_sorted.as<uint32_t>(), (size_t)n_used, 0, end_bit,
-                                         s));
-        C2_TRY(g_dr.sort_tmp.reserve(std::max<size_t>(tmp, 1)));
-        if (n_used)
-            C2_HIP(rocprim::radix_sort_pairs(g_dr.sort_tmp.ptr, tmp, g_dr.key.as<uint32_t>(),
-                                             g_dr.key_sorted.as<uint32_t>(),
-                                             g_dr.val.as<uint32_t>(),
-                                             g_dr.val_sorted.as<uint32_t>(), (size_t)n_used, 0,
-                                             end_bit, s));
+        C2_TRY(radix_sort_pairs_u32(g_dr.key.as<uint32_t>(), g_dr.key_sorted.as<uint32_t>(),
+                                    g_dr.val.as<uint32_t>(), g_dr.val_sorted.as<uint32_t>(),
+                                    n_used, (int)end_bit, g_dr.sort_tmp, g_dr.scan_tmp, s));
     }
     {
         Stage st("drift_cells", (double)n_kept * 4 + (double)ncells * 8);

@@ -5,7 +5,7 @@
 // buffer_particle.f90:23-345 for a single image):
 //   encode   global coarse cell and 16-bit in-cell position of every particle
 //            (FP64, parameters.f90:87-88);
-//   sort     stable radix sort of (coarse cell, particle index): the order of
+//   sort     stable radix sort (sort.hip) of (coarse cell, particle index): the order of
 //            the particles of one cell is their input order;
 //   starts   first sorted position of every coarse cell (binary search);
 //   rhoc     particles per coarse cell of every tile's extended region (the
@@ -15,12 +15,11 @@
 //            velocities into their slots.
 #include <cstring>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include "common.h"
 #include "engine.h"
 #include "profile.h"
 #include "scan.h"
+#include "sort.h"
 
 namespace cube2 {
 
@@ -159,18 +158,9 @@ int layout_build(const double* d_pos, const float* d_vel, int64_t n, int* d_rhoc
     }
     {
         Stage st("layout_sort", (double)n * 16);
-        size_t tmp = 0;
-        C2_HIP(rocprim::radix_sort_pairs(nullptr, tmp, g_lay.gid.as<uint32_t>(),
-                                         g_lay.gid_sorted.as<uint32_t>(), g_lay.idx.as<uint32_t>(),
-                                         g_lay.idx_sorted.as<uint32_t>(), (size_t)n, 0,
-                                         (unsigned)end_bit, s));
-        C2_TRY(g_lay.sort_tmp.reserve(std::max<size_t>(tmp, 1)));
-        if (n)
-            C2_HIP(rocprim::radix_sort_pairs(g_lay.sort_tmp.ptr, tmp, g_lay.gid.as<uint32_t>(),
-                                             g_lay.gid_sorted.as<uint32_t>(),
-                                             g_lay.idx.as<uint32_t>(),
-                                             g_lay.idx_sorted.as<uint32_t>(), (size_t)n, 0,
-                                             (unsigned)end_bit, s));
+        C2_TRY(radix_sort_pairs_u32(g_lay.gid.as<uint32_t>(), g_lay.gid_sorted.as<uint32_t>(),
+                                    g_lay.idx.as<uint32_t>(), g_lay.idx_sorted.as<uint32_t>(), n,
+                                    end_bit, g_lay.sort_tmp, g_lay.scan_tmp, s));
     }
     int* cnt = g_lay.cnt.as<int>();
     int* slot = g_lay.slot.as<int>();
