@@ -203,6 +203,12 @@ def bench_c2(args) -> int:
         from cube2_amd.autotune import autotune
         tuning = autotune(k, mass, amid, dt, log=log)
         log(f"autotune chose {tuning['chosen']}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        k.kick(mass, amid, dt)
+    torch.cuda.synchronize()
+    eager_ms = 1e3 * (time.perf_counter() - t1) / args.steps
     _lib.profile(True)
     _lib.profile_reset()
     for _ in range(args.steps):
@@ -224,7 +230,8 @@ def bench_c2(args) -> int:
                "precise": "pp_pairs_precise", "plummer": "pp_pairs_plummer"}[args.pp_mode]
     pp_ms = stats[pp_name]["ms"] / args.steps
     stages = {nm: round(v["ms"] / args.steps, 3) for nm, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
-    log(f"C2 kick {ms:.1f} ms (graph), first call {cold_ms:.1f} ms; stages {stages}")
+    log(f"C2 kick {ms:.1f} ms (graph), {eager_ms:.1f} ms eager, first call {cold_ms:.1f} ms; "
+        f"stages {stages}")
     out = {
         "metric": "C2 batching kick ms/step", "value": round(ms, 3), "unit": "ms/step",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -235,6 +242,7 @@ def bench_c2(args) -> int:
                                "lognormal counts median 4000 sigma 1.5 cap 200k",
                    "particles": int(n), "n_used_slots": int(n_used), "pp_mode": args.pp_mode,
                    "levels": {int(a): int(b) for a, b in zip(*np.unique(k.ires, return_counts=True))}},
+        "kick_ms_eager": round(eager_ms, 3),
         "kick_ms_first_call": round(cold_ms, 3), "pp_ms_per_step": round(pp_ms, 3),
         "pp_pair_interactions_per_s": float(f"{res.p_acc / (pp_ms * 1e-3):.4g}"),
         "p_cand": int(res.p_cand), "p_acc": int(res.p_acc), "stage_ms_per_step": stages,

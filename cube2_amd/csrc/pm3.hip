@@ -305,17 +305,26 @@ __global__ __launch_bounds__(kDepThreads) void k_mesh_deposit(MeshArgs A, float*
 // Brick deposit: block (brick, job) owns a kBrick^3 region of the job's
 // mesh (storage indices) and sums in LDS every TSC weight that lands in it
 // from the particles of the coarse cells that can reach it (64-bit fixed
-// point, each weight rounded to 2^-32: integer LDS atomics), then stores
-// the brick.  Every mesh point belongs to one brick: the mesh needs no
-// clearing and HBM sees no atomics, at the price of visiting a particle
-// once per brick its cell overlaps.
+// point, each weight rounded to 2^-32: integer LDS atomics, so the sums do not
+// depend on the order of the adds), then stores the brick.  Every mesh point
+// belongs to one brick: the mesh needs no clearing and HBM sees no atomics,
+// at the price of visiting a particle once per brick its cell overlaps.
+// The block first reads the counts and first slots of all its cells at once
+// (at most kBrickCells) and takes their particles as one list, 256 at a
+// time, so that neither a chain of per-cell loads nor a cell of a few
+// particles keeps the block's other lanes waiting (at 256^3 most bricks of
+// the dense levels see a few cells of a few particles each).
 constexpr int kBrick = 16;
+constexpr int kBrickCells = 256;
 
 __device__ inline int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
 __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __restrict__ rho,
                                                        size_t rs, int nbx, int* __restrict__ err) {
     __shared__ unsigned long long box[kBrick * kBrick * kBrick];
+    __shared__ int cstart[kBrickCells + 1];  // exclusive prefix of the cells' counts
+    __shared__ int64_t cbase[kBrickCells];   // first xp slot of each cell
+    __shared__ int wsum[4];
     const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
     // neighbouring bricks visit the same particles: they run on one XCD
     const unsigned lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
@@ -329,7 +338,7 @@ __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __rest
     // A particle with c = floor(xpos) writes storage c+off .. c+off+2, so it
     // reaches [lo, lo+kBrick) iff c in [lo-off-2, lo+kBrick-off-1]; its
     // coarse cell (nte index) is floor((c + u*nfp)/cf) + ncb.
-    int clo[3], chi[3];
+    int clo[3], ext[3];
     for (int d = 0; d < 3; d++) {
         const int cmin = lo[d] - off - 2 + J.u[d] * A.nfp;
         const int cmax = lo[d] + kBrick - off - 1 + J.u[d] * A.nfp;
@@ -338,57 +347,83 @@ __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __rest
         const int dlo = J.u[d] * A.ntt + A.ncb - A.nhalo,
                   dhi = (J.u[d] + 1) * A.ntt + A.ncb - 1 + A.nhalo;
         clo[d] = max(a, dlo);
-        chi[d] = min(e, dhi);
+        ext[d] = max(0, min(e, dhi) - clo[d] + 1);
+    }
+    const int ncell = ext[0] * ext[1] * ext[2];  // <= kBrickCells (checked on the host)
+    {
+        const int q = threadIdx.x, wave = q >> 6, lane = q & 63;
+        int np = 0;
+        int64_t base = 0;
+        if (q < ncell) {
+            const int ci = clo[0] + q % ext[0], cj = clo[1] + (q / ext[0]) % ext[1],
+                      ck = clo[2] + q / (ext[0] * ext[1]);
+            const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
+            np = A.rhoc[cell];
+            base = A.nzero[cell];
+            if (np <= 0) {
+                np = 0;
+            } else if (base < 0 || base + np > A.n_used) {
+                atomicOr(err, 2);
+                np = 0;
+            }
+        }
+        int inc = np;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        for (int w = 0; w < wave; w++) inc += wsum[w];
+        if (q < kBrickCells) {
+            cstart[q + 1] = inc;
+            cbase[q] = base;
+        }
+        if (q == 0) cstart[0] = 0;
     }
     __syncthreads();
-    for (int ck = clo[2]; ck <= chi[2]; ck++)
-        for (int cj = clo[1]; cj <= chi[1]; cj++)
-            for (int ci = clo[0]; ci <= chi[0]; ci++) {
-                const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
-                const int np = A.rhoc[cell];
-                const int64_t base = A.nzero[cell];
-                if (np <= 0) continue;
-                if (base < 0 || base + np > A.n_used) {
-                    if (threadIdx.x == 0) atomicOr(err, 2);
-                    continue;
-                }
-                for (int l = threadIdx.x; l < np; l += blockDim.x) {
-                    const int16_t* p = A.xp + 3 * (base + l);
-                    // floor of the mesh coordinate in integer arithmetic (the
-                    // same value tsc_axis computes): particles that miss the
-                    // brick are rejected before the FP64 work
-                    const int X0 = fine_floor(A, ci, p[0], J.u[0], cf) + off - lo[0];
-                    const int Y0 = fine_floor(A, cj, p[1], J.u[1], cf) + off - lo[1];
-                    const int Z0 = fine_floor(A, ck, p[2], J.u[2], cf) + off - lo[2];
-                    if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 ||
-                        Z0 > kBrick - 1 || Z0 < -2)
-                        continue;
-                    int cx, cy, cz;
-                    float wx[3], wy[3], wz[3];
-                    tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
-                    tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
-                    tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
-                    (void)cx, (void)cy, (void)cz;  // equal to the integer floors above
+    const int M = cstart[ncell];
+    int c = 0;
+    for (int v = threadIdx.x; v < M; v += blockDim.x) {
+        while (cstart[c + 1] <= v) c++;  // v < M: stops at a cell that holds v
+        const int ci = clo[0] + c % ext[0], cj = clo[1] + (c / ext[0]) % ext[1],
+                  ck = clo[2] + c / (ext[0] * ext[1]);
+        const int16_t* p = A.xp + 3 * (cbase[c] + (v - cstart[c]));
+        // floor of the mesh coordinate in integer arithmetic (the same value
+        // tsc_axis computes): particles that miss the brick are rejected
+        // before the FP64 work
+        const int X0 = fine_floor(A, ci, p[0], J.u[0], cf) + off - lo[0];
+        const int Y0 = fine_floor(A, cj, p[1], J.u[1], cf) + off - lo[1];
+        const int Z0 = fine_floor(A, ck, p[2], J.u[2], cf) + off - lo[2];
+        if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 || Z0 > kBrick - 1 ||
+            Z0 < -2)
+            continue;
+        int cx, cy, cz;
+        float wx[3], wy[3], wz[3];
+        tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
+        tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
+        tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+        (void)cx, (void)cy, (void)cz;  // equal to the integer floors above
 #pragma unroll
-                    for (int a3 = 0; a3 < 3; a3++) {
-                        const int Z = Z0 + a3;
-                        if ((unsigned)Z >= (unsigned)kBrick) continue;
+        for (int a3 = 0; a3 < 3; a3++) {
+            const int Z = Z0 + a3;
+            if ((unsigned)Z >= (unsigned)kBrick) continue;
 #pragma unroll
-                        for (int a2 = 0; a2 < 3; a2++) {
-                            const int Y = Y0 + a2;
-                            if ((unsigned)Y >= (unsigned)kBrick) continue;
+            for (int a2 = 0; a2 < 3; a2++) {
+                const int Y = Y0 + a2;
+                if ((unsigned)Y >= (unsigned)kBrick) continue;
 #pragma unroll
-                            for (int a1 = 0; a1 < 3; a1++) {
-                                const int X = X0 + a1;
-                                if ((unsigned)X >= (unsigned)kBrick) continue;
-                                const float w = wx[a1] * wy[a2] * wz[a3];  // <= 0.75^3
-                                atomicAdd(&box[(Z * kBrick + Y) * kBrick + X],
-                                          (unsigned long long)(unsigned)rintf(w * 4294967296.0f));
-                            }
-                        }
-                    }
+                for (int a1 = 0; a1 < 3; a1++) {
+                    const int X = X0 + a1;
+                    if ((unsigned)X >= (unsigned)kBrick) continue;
+                    const float w = wx[a1] * wy[a2] * wz[a3];  // <= 0.75^3
+                    atomicAdd(&box[(Z * kBrick + Y) * kBrick + X],
+                              (unsigned long long)(unsigned)rintf(w * 4294967296.0f));
                 }
             }
+        }
+    }
     __syncthreads();
     float* R = rho + (size_t)jb * rs;
     for (int q = threadIdx.x; q < kBrick * kBrick * kBrick; q += blockDim.x) {
@@ -397,6 +432,13 @@ __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __rest
         if (X < n && Y < n && Z < n)
             R[((size_t)Z * n + Y) * (n + 2) + X] = (float)((double)box[q] * (1.0 / 4294967296.0));
     }
+}
+
+// Coarse cells per axis a brick can visit: the c range of 2 + kBrick fine
+// cells spans at most (kBrick + 1) / cf + 2 cells.
+static bool brick_cells_fit(int cf) {
+    const int per = (kBrick + 1) / std::max(1, cf) + 2;
+    return per * per * per <= kBrickCells;
 }
 
 // Which deposit a mesh gets: bricks on the fine PM3 meshes, cell-group work
@@ -799,6 +841,11 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
     const FFTPlan* P = nullptr;
     C2_TRY(fft_plan(n, &P));
     if (use_bricks(A)) {
+        if (!brick_cells_fit(A.ratio_cs * A.rsf)) {
+            set_error("PM3 deposit: %d fine cells per coarse cell is below the brick minimum",
+                      A.ratio_cs * A.rsf);
+            return CUBE2_ERR_GEOMETRY;
+        }
         const int nbx = (n + kBrick - 1) / kBrick;
         Stage st(l2 ? "pm2_deposit" : "pm3_deposit", (double)nb * rs * 4);
         dim3 grid((unsigned)(nbx * nbx * nbx), (unsigned)nb);
