@@ -20,6 +20,7 @@ CANDIDATES = {
     "fft_min_lgc": [0, 3, 4],
     "interp_seg": [0, 2, 8],
     "pp_order": [0, 20],
+    "pp_jbatch": [0, 2],
 }
 
 

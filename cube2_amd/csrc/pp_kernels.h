@@ -81,6 +81,9 @@ inline int pp_dense_lg() {
     return v ? v : 4;
 }
 
+// j candidates per batch of the pair kernel: 64 (built in) or 128.
+inline int pp_jbatch() { return ctx().tune[CUBE2_TUNE_PP_JBATCH] == 2 ? 128 : 64; }
+
 // log2 of the estimated candidate pairs from which the pair kernel takes a
 // chunk ahead of the others (heaviest first), or 0: all in cell order
 // (built in).

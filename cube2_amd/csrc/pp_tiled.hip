@@ -224,19 +224,19 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
 // block barrier: the wavefronts of a block run independent trip counts).
 __device__ inline void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
 
-template <int kMode>
+template <int kMode, int kJB = 64>
 struct WaveLDS {
     using Acc = typename std::conditional<kMode == kModePrecise || kMode == kModePlummer, double,
                                           float>::type;
     float4 q[kQ];           // queued chunks: accepted pairs (dx, dy, dz, r^2)
     unsigned char qi[kQ];   // and their i slot
     Acc acc[64][3];         // per-i force sums
-    float4 jb[64];          // current batch of j positions
+    float4 jb[kJB];         // current batch of j positions
     int rs[64], rp[64];     // row group: j = v + rs[k] for rp[k-1] <= v < rp[k]
 };
 
-template <int kMode>
-__device__ inline void flush(WaveLDS<kMode>& L, int lane, int begin, int count, const FraK& Ka,
+template <int kMode, int kJB>
+__device__ inline void flush(WaveLDS<kMode, kJB>& L, int lane, int begin, int count, const FraK& Ka,
                              const FraK& Kr) {
     if (lane < count) {
         const int s = (begin + lane) & (kQ - 1);
@@ -244,7 +244,7 @@ __device__ inline void flush(WaveLDS<kMode>& L, int lane, int begin, int count, 
         const int ii = L.qi[s];
         float fx, fy, fz;
         pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
-        using Acc = typename WaveLDS<kMode>::Acc;
+        using Acc = typename WaveLDS<kMode, kJB>::Acc;
         atomicAdd(&L.acc[ii][0], (Acc)fx);
         atomicAdd(&L.acc[ii][1], (Acc)fy);
         atomicAdd(&L.acc[ii][2], (Acc)fz);
@@ -275,13 +275,15 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
 // ms; budgets for 5 and 7 wavefronts 89.2 and 86.0 ms; 4 blocks per CU
 // instead of 6: 105 ms).  The REF variant spills a few registers outside its
 // inner loops.
-template <int kMode>
+template <int kMode, int kJB>
 __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
                                                   const int* __restrict__ nchunks_dev,
                                                   int direct_lg, int dense_lg) {
-    __shared__ WaveLDS<kMode> lds[4];
+    static_assert(kJB == 64 || kJB == 128, "j batch of 64 or 128");
+    constexpr int kH = kJB / 64;  // fetched float4 per lane per batch
+    __shared__ WaveLDS<kMode, kJB> lds[4];
     const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-    WaveLDS<kMode>& L = lds[wave];
+    WaveLDS<kMode, kJB>& L = lds[wave];
     const int nchunks = uni(*nchunks_dev);
     float f2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f;
     unsigned long long pc = 0, pa = 0;
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int qhead = 0, qtail = 0;
         const bool direct = lgI >= direct_lg;
-        using Acc = typename WaveLDS<kMode>::Acc;
+        using Acc = typename WaveLDS<kMode, kJB>::Acc;
         Acc dax = 0, day = 0, daz = 0;
         unsigned npa = 0;
         // direct chunks: bounding box of the chunk's i; a j farther than the
@@ -387,23 +389,35 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                 }
                 return q4;
             };
-            float4 qcur = fetch(0);
-            for (int vb = 0; vb < M; vb += 64) {
-                int nb;
+            // a batch is kJB candidates, kH fetches of 64; the next batch's
+            // fetches are issued before this batch's pairs run
+            float4 qcur[kH];
+#pragma unroll
+            for (int h = 0; h < kH; h++) qcur[h] = fetch(64 * h);
+            for (int vb = 0; vb < M; vb += kJB) {
+                int nb = 0;
                 if (direct) {  // keep the j near the chunk's box, compacted
-                    const float ex = fmaxf(fmaxf(bx0 - qcur.x, qcur.x - bx1), 0.0f);
-                    const float ey = fmaxf(fmaxf(by0 - qcur.y, qcur.y - by1), 0.0f);
-                    const float ez = fmaxf(fmaxf(bz0 - qcur.z, qcur.z - bz1), 0.0f);
-                    const bool keep = vb + lane < M && ex * ex + ey * ey + ez * ez < pp2m;
-                    const uint64_t km = __ballot(keep);
-                    if (keep) L.jb[__popcll(km & lt)] = qcur;
-                    nb = __popcll(km);
+#pragma unroll
+                    for (int h = 0; h < kH; h++) {
+                        const float4 qh = qcur[h];
+                        const float ex = fmaxf(fmaxf(bx0 - qh.x, qh.x - bx1), 0.0f);
+                        const float ey = fmaxf(fmaxf(by0 - qh.y, qh.y - by1), 0.0f);
+                        const float ez = fmaxf(fmaxf(bz0 - qh.z, qh.z - bz1), 0.0f);
+                        const bool keep =
+                            vb + 64 * h + lane < M && ex * ex + ey * ey + ez * ez < pp2m;
+                        const uint64_t km = __ballot(keep);
+                        if (keep) L.jb[nb + __popcll(km & lt)] = qh;
+                        nb += __popcll(km);
+                    }
                 } else {
-                    L.jb[lane] = qcur;
-                    nb = min(64, M - vb);
+#pragma unroll
+                    for (int h = 0; h < kH; h++) L.jb[64 * h + lane] = qcur[h];
+                    nb = min(kJB, M - vb);
                 }
                 wave_lds_sync();
-                if (vb + 64 < M) qcur = fetch(vb + 64);
+#pragma unroll
+                for (int h = 0; h < kH; h++)
+                    if (vb + kJB + 64 * h < M) qcur[h] = fetch(vb + kJB + 64 * h);
                 // (i, j) of slot t: the difference vector and whether it pairs
                 // (branch-free: the slot is read clamped and masked after)
                 auto test = [&](int t, float4& e) {
@@ -513,13 +527,26 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
     block_sum_to<2>(&A.red->p_cand, sm);
 }
 
+template <int kJB>
+static hipError_t pp_occupancy(int mode, int* per) {
+    if (mode == CUBE2_PP_REF)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeRef, kJB>, 256, 0);
+    if (mode == CUBE2_PP_FAST)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeFast, kJB>, 256, 0);
+    if (mode == CUBE2_PP_PLUMMER)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModePlummer, kJB>, 256,
+                                                            0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModePrecise, kJB>, 256, 0);
+}
+
 // Blocks of the persistent pair kernel: the resident maximum over all CUs
 // (occupancy query, cached per mode).
 // overlap: the kernel runs beside PM3 (CUBE2_TUNE_PP_OVERLAP_BLOCKS per CU,
 // at most the occupancy maximum).
 int pp_tiled_grid(int mode, bool overlap) {
-    static int cache[8] = {};
-    const int m = mode & 7;
+    static int cache[16] = {};
+    const int jb = pp_jbatch();
+    const int m = (mode & 7) + (jb == 128 ? 8 : 0);
     const int forced = ctx().tune[CUBE2_TUNE_PP_BLOCKS_PER_CU];
     int dev = 0, ncu = 0, per = 0;
     if (overlap) {
@@ -543,18 +570,26 @@ int pp_tiled_grid(int mode, bool overlap) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         ncu = 256;
-    hipError_t e = hipErrorInvalidValue;
-    if (mode == CUBE2_PP_REF)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModeRef>, 256, 0);
-    else if (mode == CUBE2_PP_FAST)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModeFast>, 256, 0);
-    else if (mode == CUBE2_PP_PLUMMER)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModePlummer>, 256, 0);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pp_tiled<kModePrecise>, 256, 0);
+    const hipError_t e = jb == 128 ? pp_occupancy<128>(mode, &per) : pp_occupancy<64>(mode, &per);
     if (e != hipSuccess || per < 1) per = 2;
     cache[m] = std::max(1, ncu) * per;
     return cache[m];
+}
+
+template <int kJB>
+static void launch_pp_tiled_jb(const PPArgs& a, const int* chunks, const int* nchunks,
+                               unsigned blocks, int mode, int direct_lg, int dense_lg,
+                               hipStream_t s) {
+    if (mode == CUBE2_PP_REF)
+        k_pp_tiled<kModeRef, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
+    else if (mode == CUBE2_PP_FAST)
+        k_pp_tiled<kModeFast, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
+    else if (mode == CUBE2_PP_PLUMMER)
+        k_pp_tiled<kModePlummer, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg,
+                                                             dense_lg);
+    else
+        k_pp_tiled<kModePrecise, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg,
+                                                             dense_lg);
 }
 
 int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int grid, int mode,
@@ -568,14 +603,10 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
     // chunks of more than 2 particles sum directly unless tuned otherwise
     const int direct_lg = ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] ? ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] : 2;
     const int dense_lg = pp_dense_lg();
-    if (mode == CUBE2_PP_REF)
-        k_pp_tiled<kModeRef><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
-    else if (mode == CUBE2_PP_FAST)
-        k_pp_tiled<kModeFast><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
-    else if (mode == CUBE2_PP_PLUMMER)
-        k_pp_tiled<kModePlummer><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
+    if (pp_jbatch() == 128)
+        launch_pp_tiled_jb<128>(a, chunks, nchunks, blocks, mode, direct_lg, dense_lg, s);
     else
-        k_pp_tiled<kModePrecise><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
+        launch_pp_tiled_jb<64>(a, chunks, nchunks, blocks, mode, direct_lg, dense_lg, s);
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }

@@ -135,7 +135,9 @@ int cube2_set_pp_mode(int mode);
  *   PP_ORDER          1: the pair kernel takes its chunks in cell order
  *                     (built-in); 2..30: chunks of an estimated 2^v
  *                     candidate pairs or more first, heaviest first, then
- *                     the rest in cell order.
+ *                     the rest in cell order;
+ *   PP_JBATCH         1: the pair kernel fetches and tests j candidates 64
+ *                     at a time (built-in); 2: 128 at a time.
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
@@ -146,7 +148,8 @@ enum {
     CUBE2_TUNE_KICK_OVERLAP = 5,
     CUBE2_TUNE_PP_OVERLAP_BLOCKS = 6,
     CUBE2_TUNE_PP_ORDER = 7,
-    CUBE2_TUNE_COUNT = 8
+    CUBE2_TUNE_PP_JBATCH = 8,
+    CUBE2_TUNE_COUNT = 9
 };
 int cube2_set_tuning(int key, int value);
 int cube2_get_tuning(int key, int* value); /* the value in use (built-in choice resolved) */
