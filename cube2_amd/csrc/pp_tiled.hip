@@ -47,6 +47,9 @@
 namespace cube2 {
 
 constexpr int kQ = 128;  // queue entries per wavefront
+// A position farther from every particle than any range: its squared distance
+// overflows to +inf, so a pair test with it fails without a bounds check.
+constexpr float kFar = 1e30f;
 
 // 1/x from an approximation y0: with e = 1 - x*y0 (one FMA, rounded once),
 // 1/x = y0/(1-e) = y0 (1 + e + e^2 + e^3/(1-e)), so y0 + y0 (e + e^2) is
@@ -231,7 +234,7 @@ struct WaveLDS {
     float4 q[kQ];           // queued chunks: accepted pairs (dx, dy, dz, r^2)
     unsigned char qi[kQ];   // and their i slot
     Acc acc[64][3];         // per-i force sums
-    float4 jb[kJB];         // current batch of j positions
+    float4 jb[kJB + 64];    // current batch of j positions, then J far-away ones
     int rs[64], rp[64];     // row group: j = v + rs[k] for rp[k-1] <= v < rp[k]
 };
 
@@ -350,6 +353,8 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
             bz1 = uni(bz1);
         }
         const float pp2m = pp2 * 1.0001f;
+        // the pair tests' i position: a far-away one for lanes past the chunk
+        const float pix = vi ? p.x : -kFar, piy = vi ? p.y : -kFar, piz = vi ? p.z : -kFar;
         const int xlo = max(x - R, 0), xhi = min(x + R, S - 1);
         const int W = 2 * R + 1, nrows = W * W;
         // Neighbour rows (each a contiguous run of sorted j) in groups of 64:
@@ -414,25 +419,27 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                     for (int h = 0; h < kH; h++) L.jb[64 * h + lane] = qcur[h];
                     nb = min(kJB, M - vb);
                 }
+                // the J slots past the batch hold far-away j: a lane's last
+                // iteration reads them instead of a clamped slot
+                if (lane < J) L.jb[nb + lane] = make_float4(kFar, kFar, kFar, 0.0f);
                 wave_lds_sync();
 #pragma unroll
                 for (int h = 0; h < kH; h++)
                     if (vb + kJB + 64 * h < M) qcur[h] = fetch(vb + kJB + 64 * h);
                 // (i, j) of slot t: the difference vector and whether it pairs
-                // (branch-free: the slot is read clamped and masked after)
+                // (branch-free: slots past the batch hold far-away j, and an
+                // i slot past the chunk tests from a far-away i)
                 auto test = [&](int t, float4& e) {
 #pragma clang fp contract(off)
-                    const int jj = t + jslot;
-                    const bool in = vi & (jj < nb);
-                    const float4 qj = L.jb[in ? jj : 0];
-                    e.x = qj.x - p.x;
-                    e.y = qj.y - p.y;
-                    e.z = qj.z - p.z;
+                    const float4 qj = L.jb[t + jslot];
+                    e.x = qj.x - pix;
+                    e.y = qj.y - piy;
+                    e.z = qj.z - piz;
                     if (kMode == kModePrecise || kMode == kModePlummer)
                         e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
                     else
                         e.w = e.x * e.x + e.y * e.y + e.z * e.z;
-                    return in & (e.w > 0.0f) & (e.w < pp2);
+                    return (e.w > 0.0f) & (e.w < pp2);
                 };
                 // two loops on the chunk-uniform branch, so that the direct
                 // loop carries only its accumulators
