@@ -528,6 +528,30 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
             ring[slot][ly][lx] = P[zoff + (size_t)(y0 + ly + nfb - 3) * sr + (x0 + lx + nfb - 3)];
         }
     };
+    // the same in two halves: the loads into registers are issued before the
+    // previous plane's outputs are computed, so their latency hides behind
+    // that work; the registers go to the ring after it
+    constexpr int kPre = ((kGradW + 4) * (kGradH + 4) + 255) / 256;
+    float pre[kPre];
+    auto fetch = [&](int k) {
+        const size_t zoff = (size_t)(k + nfb - 1) * sp;
+#pragma unroll
+        for (int i = 0; i < kPre; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < lw * lh) {
+                const int lx = e % lw, ly = e / lw;
+                pre[i] = P[zoff + (size_t)(y0 + ly + nfb - 3) * sr + (x0 + lx + nfb - 3)];
+            }
+        }
+    };
+    auto store = [&](int k) {
+        const int slot = (k + 5) % 5;
+#pragma unroll
+        for (int i = 0; i < kPre; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < lw * lh) ring[slot][e / lw][e % lw] = pre[i];
+        }
+    };
     // real(4) weights of alpha = 4/3 (parameters.f90:49-50)
     const float al = 4.0f / 3.0f;
     const float w0 = (al - 1.0f) / 4.0f, w1 = -al / 2.0f, w2 = al / 2.0f, w3 = (1.0f - al) / 4.0f;
@@ -535,9 +559,11 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
     float* F = force + (size_t)jb * 3 * fs;
     float f2 = 0.0f;
     for (int k = k_begin - 2; k < k_begin + 2; k++) load(k);
+    if (k_begin < k_end) fetch(k_begin + 2);
     for (int k0 = k_begin; k0 < k_end; k0++) {
-        load(k0 + 2);
+        store(k0 + 2);  // its slot held plane k0 - 3, last read before the previous barrier
         __syncthreads();
+        if (k0 + 1 < k_end) fetch(k0 + 3);
         const int sm2 = (k0 + 3) % 5, sm1 = (k0 + 4) % 5, s0 = k0 % 5, sp1 = (k0 + 1) % 5,
                   sp2 = (k0 + 2) % 5;
         for (int e = threadIdx.x; e < ex * ey; e += blockDim.x) {
