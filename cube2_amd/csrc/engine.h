@@ -32,6 +32,7 @@ struct PPPlan {
     bool count_tiles = false;  // count pairs per parent tile
     bool split = false;        // pair sums apart from the velocity update (pp_enqueue_apply)
     bool learned = false;      // the network stands in for PP in sparse level-2 cells
+    bool all_active = false;   // every subtile has a PP range (apm3 > 0 at its level)
     LearnedPlan lp;
     std::array<int, 7> ratio_sf{};
     std::array<float, 7> apm3{};

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_learned_apply(PPArgs A, LearnedArgs L, 
         for (int d = 0; d < 3; d++) am[d] = acc[d] * L.y_scale * mass * mass;
         f2 = fmaxf(f2, am[0] * am[0] + am[1] * am[1] + am[2] * am[2]);
 #pragma unroll
-        for (int d = 0; d < 3; d++) A.term[3 * (size_t)a + d] = am[d] * a_mid * dt / 6.0f;
+        for (int d = 0; d < 3; d++) A.term[3 * (size_t)A.slot[a] + d] = am[d] * a_mid * dt / 6.0f;
     }
     const float mx[1] = {f2};
     block_max_to<1>(&A.red->f2max_bits, mx);

@@ -42,7 +42,8 @@ struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
     DevBuf chunk_flags, chunks, ilist, big, dense;
     DevBuf order, order_bin, order_hist;  // chunks heaviest first (k_chunk_cost/order)
-    DevBuf term;  // split pass: per sorted particle velocity term [cap][3]
+    DevBuf term;  // split pass: per slot velocity term [n_used][3] (active slots)
+    DevBuf rows_len, rows_base;  // split pass: physical rows' slot counts (scanned) and first slots
     HostBuf h_res;  // PPRed, per-location counts, per-tile pairs, tile_first (pinned)
     std::vector<PPTile> h_tiles;
     std::vector<uint64_t> tile_pairs;  // [ntile][2] of the last call (when counted)
@@ -756,7 +757,14 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_TRY(g_pp.ilist.reserve(P.cap * sizeof(int)));
     C2_TRY(g_pp.big.reserve(std::max<int64_t>(P.ncells_ext, 1) * sizeof(int)));
     C2_TRY(g_pp.dense.reserve((P.cap / 64 + 1) * sizeof(int)));
-    if (split) C2_TRY(g_pp.term.reserve(P.cap * 3 * sizeof(float)));
+    if (split) {
+        C2_TRY(g_pp.term.reserve(P.cap * 3 * sizeof(float)));
+        const int64_t rows = (int64_t)g.ntile * g.nt * g.nt;
+        C2_TRY(g_pp.rows_len.reserve((rows + 1) * sizeof(int)));
+        C2_TRY(g_pp.rows_base.reserve(std::max<int64_t>(rows, 1) * sizeof(int64_t)));
+    }
+    P.all_active = true;
+    for (int lv : P.iloc) P.all_active = P.all_active && lv >= 1 && lv <= 7 && apm3[lv - 1] > 0.0f;
     if (learned) {
         C2_TRY(learned_prepare(g, P.iloc, ratio_sf, apm3, P.lp, learned_model_occupancy()));
         C2_TRY(learned_reserve(P.lp));
@@ -777,6 +785,72 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
 
 static int pp_results_to_host(const PPPlan& P, hipStream_t s);
 
+// Physical rows (t, k, j) of the image: each is one contiguous run of slots,
+// from the first slot of its first physical coarse cell to that of the
+// first halo cell after its last (nzero there).  len = its slots, base =
+// its first slot; a run outside [0, n_used) counts as empty (the cell-list
+// kernels flag such a layout as an error).
+__global__ void k_phys_rows(const int64_t* __restrict__ nzero, int nt, int ncb, int nte,
+                            int64_t nrows, int64_t n_used, int* __restrict__ len,
+                            int64_t* __restrict__ base) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int64_t per = (int64_t)nt * nt;
+    const int64_t t = r / per, rem = r - t * per;
+    const int j = ncb + (int)(rem % nt), k = ncb + (int)(rem / nt);
+    const int64_t row0 = ((t * nte + k) * nte + j) * nte;
+    const int64_t b = nzero[row0 + ncb], e = nzero[row0 + ncb + nt];
+    const bool ok = b >= 0 && e >= b && e <= n_used;
+    base[r] = ok ? b : 0;
+    len[r] = ok ? (int)(e - b) : 0;
+}
+
+// Velocity update of a split pass in slot order, when every particle of the
+// physical rows is active (every subtile has a PP range): the physical slots
+// as one list over the scanned rows, kApplySeg at a time per block, each
+// thread walking the rows forward from the segment's first; vp = RN_f(vp +
+// dv64 + term / pi) in FP64 (the unsplit kernel's operation order), vmax
+// over them.  The slots, Delta-v and terms are read in order.
+constexpr int kApplySeg = 2048;
+
+__global__ __launch_bounds__(256) void k_pp_apply_rows(const int* __restrict__ prefix,
+                                                      const int64_t* __restrict__ base, int nrows,
+                                                      const float* __restrict__ term,
+                                                      const double* __restrict__ dv64,
+                                                      float* __restrict__ vp, PPRed* red) {
+    __shared__ int srow;
+    float vm[3] = {0.f, 0.f, 0.f};
+    const int total = red->err == 0 ? prefix[nrows] : 0;  // a failed call leaves no terms
+    for (int seg0 = blockIdx.x * kApplySeg; seg0 < total; seg0 += gridDim.x * kApplySeg) {
+        if (threadIdx.x == 0) {  // last row starting at or before seg0
+            int lo = 0, hi = nrows - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (prefix[mid] <= seg0) lo = mid;
+                else hi = mid - 1;
+            }
+            srow = lo;
+        }
+        __syncthreads();
+        int row = srow;
+        for (int e = seg0 + threadIdx.x; e < min(total, seg0 + kApplySeg); e += blockDim.x) {
+            while (prefix[row + 1] <= e) row++;  // e < total = prefix[nrows]
+            const size_t ip = (size_t)(base[row] + (e - prefix[row])) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double t = (double)term[ip + c] / 3.14159265358979323846;
+                double v = (double)vp[ip + c];
+                if (dv64) v += dv64[ip + c];
+                const float nv = (float)(v + t);
+                vp[ip + c] = nv;
+                vm[c] = fmaxf(vm[c], fabsf(nv));
+            }
+        }
+        __syncthreads();  // srow is rewritten for the next segment
+    }
+    block_max_to<3>(red->vmax_bits, vm);
+}
+
 int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp,
                float* d_vp, const float* d_scal, const double* dv64, hipStream_t s) {
     const Geom& g = ctx().g;
@@ -787,6 +861,15 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     C2_TRY(fill_zero(red, red_bytes(g, P.ntile), s));
     const int64_t* nzp = nullptr;
     C2_TRY(compute_nzero(d_rhoc, d_idx, &nzp, s));
+    const int64_t prow = (int64_t)g.ntile * g.nt * g.nt;
+    if (P.split && P.all_active && prow > 0) {
+        Stage st("pp_rows", (double)prow * 28);
+        k_phys_rows<<<ceil_div(prow, 256), 256, 0, s>>>(nzp, g.nt, g.ncb, g.nte, prow, P.n_used,
+                                                        g_pp.rows_len.as<int>(),
+                                                        g_pp.rows_base.as<int64_t>());
+        C2_HIP(hipGetLastError());
+        C2_TRY(exclusive_scan(g_pp.rows_len.as<int>(), prow, g_pp.scan_tmp, s));
+    }
     if (P.ncells_ext > 0) {
         Stage st("pp_count", (double)P.n_used * 6 + P.c_tot * 4.0);
         k_pp_count<<<ceil_div(P.ncells_ext, kWaves), 256, 0, s>>>(
@@ -897,6 +980,8 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
 // Velocity update of a split pass, once the pair kernel and PM3 are done:
 // for every active sorted particle (range > 0), vp = RN_f(vp + dv64 +
 // term / pi) in FP64, the unsplit kernel's operation order; vmax over them.
+// (Used when some subtile has no PP range; its particles keep their
+// velocities, as in the unsplit pass.)
 __global__ __launch_bounds__(256) void k_pp_apply(const float4* __restrict__ pos4,
                                                  const uint32_t* __restrict__ slot,
                                                  const float* __restrict__ term,
@@ -911,7 +996,7 @@ __global__ __launch_bounds__(256) void k_pp_apply(const float4* __restrict__ pos
         const size_t ip = (size_t)slot[a] * 3;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double t = (double)term[3 * (size_t)a + c] / 3.14159265358979323846;
+            const double t = (double)term[ip + c] / 3.14159265358979323846;
             double v = (double)vp[ip + c];
             if (dv64) v += dv64[ip + c];
             const float nv = (float)(v + t);
@@ -928,12 +1013,27 @@ int pp_enqueue_apply(const PPPlan& P, float* d_vp, const double* dv64, hipStream
         return CUBE2_ERR_STATE;
     }
     {
-        Stage st("pp_apply", (double)P.n_used * (16 + 4 + 12 + 12 + 24 + 12));
-        const unsigned blocks = std::min<unsigned>(ceil_div(P.cap, 256), 8192);
-        k_pp_apply<<<blocks, 256, 0, s>>>(g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>(),
-                                          g_pp.term.as<float>(), g_pp.counts.as<int>() + P.c_tot,
-                                          (int)P.cap, dv64, d_vp, g_pp.red.as<PPRed>());
-        C2_HIP(hipGetLastError());
+        const Geom& g = ctx().g;
+        const int64_t prow = (int64_t)g.ntile * g.nt * g.nt;
+        if (P.all_active) {
+            Stage st("pp_apply", (double)P.n_used * (12 + 24 + 12 + 12));
+            if (prow > 0) {
+                const unsigned blocks = std::min<unsigned>(ceil_div(P.cap, kApplySeg), 4096);
+                k_pp_apply_rows<<<blocks, 256, 0, s>>>(g_pp.rows_len.as<int>(),
+                                                       g_pp.rows_base.as<int64_t>(), (int)prow,
+                                                       g_pp.term.as<float>(), dv64, d_vp,
+                                                       g_pp.red.as<PPRed>());
+                C2_HIP(hipGetLastError());
+            }
+        } else {
+            Stage st("pp_apply", (double)P.n_used * (16 + 4 + 12 + 12 + 24 + 12));
+            const unsigned blocks = std::min<unsigned>(ceil_div(P.cap, 256), 8192);
+            k_pp_apply<<<blocks, 256, 0, s>>>(g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>(),
+                                              g_pp.term.as<float>(),
+                                              g_pp.counts.as<int>() + P.c_tot, (int)P.cap, dv64,
+                                              d_vp, g_pp.red.as<PPRed>());
+            C2_HIP(hipGetLastError());
+        }
     }
     return pp_results_to_host(P, s);
 }
@@ -1097,7 +1197,8 @@ void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
                       &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks,
-                      &p.ilist, &p.big, &p.dense, &p.term})
+                      &p.ilist, &p.big, &p.dense, &p.term, &p.rows_len, &p.rows_base,
+                      &p.order, &p.order_bin, &p.order_hist})
         b->release();
     p.valid = false;
 }

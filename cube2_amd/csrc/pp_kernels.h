@@ -57,8 +57,9 @@ struct PPArgs {
     float* vp;
     // split pass (non-null): the pair kernel stores each active particle's
     // velocity term a * m^2 * a_mid * dt / 6 (FP32, before the division by
-    // pi) at term[3 * sorted index], and k_pp_apply updates vp and vmax
-    // after PM3 has joined; vp and dv64 are then unused by the pair kernel
+    // pi) at term[3 * slot], and the velocity update (k_pp_apply_rows or
+    // k_pp_apply) applies it after PM3 has joined; vp and dv64 are then
+    // unused by the pair kernel
     float* term;
     PPRed* red;
     // per parent tile {P_cand, P_acc} [ntile][2], or nullptr (counted only

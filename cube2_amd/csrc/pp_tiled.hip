@@ -502,7 +502,8 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
             const float acc[3] = {a0m, a1m, a2m};
             if (A.term) {  // split pass: k_pp_apply finishes the update
 #pragma unroll
-                for (int c = 0; c < 3; c++) A.term[3 * (size_t)a + c] = acc[c] * a_mid * dt / 6.0f;
+                for (int c = 0; c < 3; c++)
+                    A.term[3 * (size_t)A.slot[a] + c] = acc[c] * a_mid * dt / 6.0f;
             } else {
                 const size_t ip = (size_t)A.slot[a] * 3;
                 float nv[3];
