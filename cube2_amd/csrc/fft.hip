@@ -163,7 +163,9 @@ __device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ 
         const int task = threadIdx.x + k * kFFTThreads;
         if (task < tasks) {
             const int l = task & (L - 1), j = task >> lgL;
-            const int kk = mod_p(j, p, invp);
+            // p is the product of the earlier radices: a power of two for
+            // every stage before the first radix 3, 5 or 7
+            const int kk = (p & (p - 1)) == 0 ? (j & (p - 1)) : mod_p(j, p, invp);
             if (kk) {
 #pragma unroll
                 for (int q = 1; q < R; q++) {
