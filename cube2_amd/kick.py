@@ -221,12 +221,15 @@ class ShortRangeKick:
         return arr
 
     def kick(self, mass: float, a_mid: float, dt: float, pm3: bool = True, pp: bool = True,
-             select: bool = False, pm2: bool = False, learned: bool = False) -> KickResult:
+             select: bool = False, pm2: bool = False, learned: bool = False,
+             apm3: np.ndarray | None = None) -> KickResult:
         """One short-range kick through the library's kick driver
         (cube2_kick_dev).  select=True re-selects levels on the device from
         rhoc (kick.f90:107-131); otherwise the current ires/isort are used.
         pm2=True runs PM2 (kick.f90:76-105) first.  learned=True lets the
-        library's loaded network stand in for PP in sparse level-2 cells."""
+        library's loaded network stand in for PP in sparse level-2 cells.
+        apm3: the PP ranges per level (default: the reference's table)."""
+        apm3 = APM3 if apm3 is None else np.ascontiguousarray(apm3, np.float32)
         L = _lib.lib()
         _lib.set_params(self.geom)
         flags = ((_lib.KICK_PM3 if pm3 else 0) | (_lib.KICK_PP if pp else 0) |
@@ -246,7 +249,7 @@ class ShortRangeKick:
         pt3, ptp, ttp = np.zeros(ns3), np.zeros(ns3), np.zeros(ns3)
         out = _lib.KickResultC()
         _lib.check(L.cube2_kick_dev(
-            flags | _lib.KICK_KEEP_LEVELS, _np_ptr(self.ix), _np_ptr(APM3), _np_ptr(RATIO_SF),
+            flags | _lib.KICK_KEEP_LEVELS, _np_ptr(self.ix), _np_ptr(apm3), _np_ptr(RATIO_SF),
             self.d_rhoc.data_ptr(), self.d_idx.data_ptr(), self.d_xp.data_ptr(),
             self.d_vp.data_ptr(), self.n_used, self._green_ptrs(pm2), float(mass), float(a_mid),
             float(dt), _np_ptr(self.ires), _np_ptr(self.isort), _np_ptr(pt3), _np_ptr(ptp),

@@ -224,6 +224,40 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
             lib.set_tuning(key, 0)
 
 
+def test_overlapped_kick_with_a_level_without_pp_range(lib, mixed_layout):
+    """A level whose PP range is 0 leaves its particles out of PP (the unsplit
+    pair kernel skips them, velocity update included); the overlapped kick's
+    velocity update then runs in sorted order instead of over the physical
+    rows and still equals the serial kick bit for bit."""
+    import torch
+    from cube2_amd.geometry import APM3
+    from cube2_amd.kick import ShortRangeKick
+    lay, ires = mixed_layout
+    apm3 = np.array(APM3, np.float32).copy()
+    apm3[3 - 1] = 0.0  # level 3
+    assert (ires == 3).any()
+    k = ShortRangeKick(lay)
+    k.set_levels(ires)
+    vp0 = k.d_vp.clone()
+    runs = {}
+    try:
+        for ov in (2, 1):
+            lib.set_tuning(lib.TUNE_KICK_OVERLAP, ov)
+            k.d_vp.copy_(vp0)
+            torch.cuda.synchronize()
+            r = k.kick(1.3, 0.05, 2e-3, apm3=apm3)
+            runs[ov] = (r, k.vp_host())
+    finally:
+        lib.set_tuning(lib.TUNE_KICK_OVERLAP, 0)
+    (rs, vs), (ro, vo) = runs[2], runs[1]
+    print(f"vp differs in {int((vo != vs).sum())} entries", flush=True)
+    assert np.array_equal(vo.view(np.uint32), vs.view(np.uint32))
+    assert rs.f2max_pp == ro.f2max_pp and np.array_equal(rs.vmax, ro.vmax)
+    assert (rs.p_cand, rs.p_acc) == (ro.p_cand, ro.p_acc)
+    # the level-3 particles kept their velocities through PP
+    assert rs.p_acc > 0
+
+
 @pytest.mark.parametrize("mode", ["ref", "fast"])
 def test_chunk_order_changes_no_result(lib, mixed_layout, mode):
     """The pair kernel's chunks in cell order, all sorted heaviest first, or
