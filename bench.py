@@ -43,15 +43,17 @@ def pp_flops(p_cand: int, p_acc: int) -> float:
 def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s: float):
     """Oracle (C/OpenMP restatement, same two-level structure as the
     reference) on a bounded sample: PP over the lightest parent tiles until
-    ~budget_s, PM3 over a few level-2 subtiles; each extrapolated to the full
-    kick by its algorithmic work (PP flops, PM3 mesh points)."""
+    ~budget_s (extrapolated to the kick by PP flops), PM3 over up to 4
+    subtiles of each level (extrapolated per level by subtile count; levels
+    past the budget by the last timed level's time per n^3 log2 n)."""
     import oracle
     from cube2_amd.geometry import APM3, RATIO_SF, ixyz3
     g = lay.geom
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     nteam = 4
     nnest = max(1, ncpu // nteam)
-    # PP: lightest tiles first (cost proxy: sum of coarse counts squared)
+    # PP: lightest tiles first (cost proxy: sum of coarse counts squared); the
+    # dense tiles would take minutes each on the host
     proxy = (lay.rhoc.reshape(g.ntile, -1).astype(np.float64) ** 2).sum(1)
     order = np.argsort(proxy, kind="stable")
     t_pp, f_pp, tiles, px = 0.0, 0.0, [], 0.0
@@ -70,26 +72,44 @@ def cpu_baseline(lay, ires, isort, greens_host, p_full, mass, amid, dt, budget_s
             break
     pp_rate = f_pp / t_pp
     pp_ms = pp_flops(*p_full) / pp_rate * 1e3
-    # PM3: level-2 subtiles (n = 80), timed until a quarter of the budget,
-    # extrapolated by mesh points
-    l2 = [i for i in range(g.ns3) if ires[i] == 2]
+    # PM3: up to 4 subtiles of every level present (spread over the level's
+    # subtiles), the coarse levels first, within a third of the budget; each
+    # level extrapolated by its subtile count
+    levels = sorted(set(int(L) for L in ires))
+    per_level = {}
     t0 = time.perf_counter()
-    done = 0
-    for it in l2:
-        oracle.pm3_job(g, it, ires, ixyz3(g), RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
-                       greens_host[2], mass, want_rho=False, want_phi=False, want_force=True)
-        done += 1
-        if time.perf_counter() - t0 > budget_s * 0.25:
+    for L in levels:
+        subs = [i for i in range(g.ns3) if ires[i] == L]
+        picks = subs[:: max(1, len(subs) // 4)][:4]
+        tl, nl = 0.0, 0
+        for it in picks:
+            t1 = time.perf_counter()
+            oracle.pm3_job(g, it, ires, ixyz3(g), RATIO_SF, lay.rhoc, lay.idx_b_r, lay.xp,
+                           greens_host[L], mass, want_rho=False, want_phi=False, want_force=True)
+            tl += time.perf_counter() - t1
+            nl += 1
+            if time.perf_counter() - t0 > budget_s / 3:
+                break
+        per_level[L] = (tl / nl, len(subs), nl)
+        if time.perf_counter() - t0 > budget_s / 3:
             break
+    # levels not reached: the last timed level's time per n^3 log2 n
+    Lt = max(per_level)
+    unit = per_level[Lt][0] / (g.nft(Lt) ** 3 * np.log2(g.nft(Lt)))
+    pm3_s = 0.0
+    for L in levels:
+        cnt = sum(1 for i in range(g.ns3) if ires[i] == L)
+        pm3_s += cnt * (per_level[L][0] if L in per_level
+                        else unit * g.nft(L) ** 3 * np.log2(g.nft(L)))
     t_pm3 = time.perf_counter() - t0
-    pts_done = done * g.nft(2) ** 3
-    pts_all = sum(g.nft(int(L)) ** 3 for L in ires)
-    pm3_ms = t_pm3 / pts_done * pts_all * 1e3
+    pm3_ms = pm3_s * 1e3
+    timed = ", ".join(f"L{L}: {v[2]} of {v[1]} ({1e3 * v[0]:.0f} ms each)"
+                      for L, v in sorted(per_level.items()))
     return dict(value=round(pp_ms + pm3_ms, 1), unit="ms/step", cores=ncpu, kind="port",
                 sample=(f"oracle PP on {len(tiles)} of {g.ntile} parent tiles "
                         f"({t_pp:.1f} s, {pp_rate / 1e9:.2f} GFLOP/s algorithmic, extrapolated "
-                        f"by PP flops: {pp_ms:.0f} ms) + oracle PM3 on {done} level-2 subtiles "
-                        f"({t_pm3:.2f} s, extrapolated by mesh points: {pm3_ms:.0f} ms); "
+                        f"by PP flops: {pp_ms:.0f} ms) + oracle PM3 jobs per level ({timed}; "
+                        f"{t_pm3:.1f} s, extrapolated by subtile count: {pm3_ms:.0f} ms); "
                         f"OpenMP threads {ncpu}"))
 
 
@@ -655,7 +675,7 @@ def main(argv=None) -> int:
 
     cpu = None
     if not args.no_cpu:
-        greens_host = {L: k.green(L).cpu().numpy() for L in levels if L == 2}
+        greens_host = {L: k.green(L).cpu().numpy() for L in levels}
         log("cpu baseline ...")
         cpu = cpu_baseline(lay, k.ires, k.isort, greens_host, (res.p_cand, res.p_acc), mass, amid,
                            dt, args.cpu_budget)
