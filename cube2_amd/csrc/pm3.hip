@@ -519,38 +519,52 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
     const size_t sp = (size_t)n * sr;
     const int ex = min(W, nf - x0), ey = min(kGradH, nf - y0);  // outputs of the tile
     const int lw = ex + 4, lh = ey + 4;
-    // phi plane of output plane k (storage z = k + nfb - 1) into a ring slot
-    auto load = [&](int k) {
-        const int slot = (k + 5) % 5;
-        const size_t zoff = (size_t)(k + nfb - 1) * sp;
-        for (int e = threadIdx.x; e < lw * lh; e += blockDim.x) {
-            const int lx = e % lw, ly = e / lw;
-            ring[slot][ly][lx] = P[zoff + (size_t)(y0 + ly + nfb - 3) * sr + (x0 + lx + nfb - 3)];
-        }
-    };
-    // the same in two halves: the loads into registers are issued before the
-    // previous plane's outputs are computed, so their latency hides behind
-    // that work; the registers go to the ring after it
-    constexpr int kPre = ((kGradW + 4) * (kGradH + 4) + 255) / 256;
+    constexpr int kRW = kGradW + 4;  // ring row pitch
+    // Each thread's share of a plane, fixed for the whole march: the ring
+    // cells it loads (offset in the phi plane, place in the ring) and the
+    // outputs it computes (offset in a force plane, place of the stencil's
+    // centre in the ring), so that a plane costs no index arithmetic beyond
+    // one base address per array.
+    constexpr int kPre = ((kGradH + 4) * kRW + 255) / 256;
+    int loff[kPre], lpos[kPre];
+#pragma unroll
+    for (int i = 0; i < kPre; i++) {
+        const int e = threadIdx.x + 256 * i;
+        const int lx = e % kRW, ly = e / kRW;  // constant divisor
+        const bool v = lx < lw && ly < lh;
+        loff[i] = v ? (y0 + ly + nfb - 3) * sr + (x0 + lx + nfb - 3) : -1;
+        lpos[i] = ly * kRW + lx;
+    }
+    constexpr int kOut = kGradW * kGradH / 256;  // outputs per thread
+    int ooff[kOut], opos[kOut];
+    bool inner[kOut];
+#pragma unroll
+    for (int q = 0; q < kOut; q++) {
+        const int lx = (int)threadIdx.x % kGradW, ly = (int)threadIdx.x / kGradW + q * (256 / kGradW);
+        const bool v = lx < ex && ly < ey;
+        const int i0 = x0 + lx, j0 = y0 + ly;
+        ooff[q] = v ? j0 * nf + i0 : -1;
+        opos[q] = (ly + 2) * kRW + lx + 2;
+        inner[q] = v && i0 >= 1 && i0 <= nfp && j0 >= 1 && j0 <= nfp;
+    }
+    float* ring0 = &ring[0][0][0];
+    constexpr int kSlot = (kGradH + 4) * kRW;
+    // phi plane of output plane k (storage z = k + nfb - 1) into a ring slot,
+    // in two halves: the loads into registers are issued before the previous
+    // plane's outputs are computed, so their latency hides behind that work;
+    // the registers go to the ring after it
     float pre[kPre];
     auto fetch = [&](int k) {
-        const size_t zoff = (size_t)(k + nfb - 1) * sp;
+        const float* Pz = P + (size_t)(k + nfb - 1) * sp;
 #pragma unroll
-        for (int i = 0; i < kPre; i++) {
-            const int e = threadIdx.x + 256 * i;
-            if (e < lw * lh) {
-                const int lx = e % lw, ly = e / lw;
-                pre[i] = P[zoff + (size_t)(y0 + ly + nfb - 3) * sr + (x0 + lx + nfb - 3)];
-            }
-        }
+        for (int i = 0; i < kPre; i++)
+            if (loff[i] >= 0) pre[i] = Pz[loff[i]];
     };
     auto store = [&](int k) {
-        const int slot = (k + 5) % 5;
+        float* R = ring0 + ((k + 5) % 5) * kSlot;
 #pragma unroll
-        for (int i = 0; i < kPre; i++) {
-            const int e = threadIdx.x + 256 * i;
-            if (e < lw * lh) ring[slot][e / lw][e % lw] = pre[i];
-        }
+        for (int i = 0; i < kPre; i++)
+            if (loff[i] >= 0) R[lpos[i]] = pre[i];
     };
     // real(4) weights of alpha = 4/3 (parameters.f90:49-50)
     const float al = 4.0f / 3.0f;
@@ -558,35 +572,39 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
     const size_t fs = (size_t)nf * nf * nf;
     float* F = force + (size_t)jb * 3 * fs;
     float f2 = 0.0f;
-    for (int k = k_begin - 2; k < k_begin + 2; k++) load(k);
+    for (int k = k_begin - 2; k < k_begin + 2; k++) {
+        fetch(k);
+        store(k);
+    }
     if (k_begin < k_end) fetch(k_begin + 2);
     for (int k0 = k_begin; k0 < k_end; k0++) {
         store(k0 + 2);  // its slot held plane k0 - 3, last read before the previous barrier
         __syncthreads();
         if (k0 + 1 < k_end) fetch(k0 + 3);
-        const int sm2 = (k0 + 3) % 5, sm1 = (k0 + 4) % 5, s0 = k0 % 5, sp1 = (k0 + 1) % 5,
-                  sp2 = (k0 + 2) % 5;
-        for (int e = threadIdx.x; e < ex * ey; e += blockDim.x) {
-            const int lx = e % ex, ly = e / ex;
-            const int cx = lx + 2, cy = ly + 2;
+        const float* Rm2 = ring0 + ((k0 + 3) % 5) * kSlot;
+        const float* Rm1 = ring0 + ((k0 + 4) % 5) * kSlot;
+        const float* R0 = ring0 + (k0 % 5) * kSlot;
+        const float* Rp1 = ring0 + ((k0 + 1) % 5) * kSlot;
+        const float* Rp2 = ring0 + ((k0 + 2) % 5) * kSlot;
+        float* Fx = F + (size_t)k0 * nf * nf;
+        const bool kin = k0 >= 1 && k0 <= nfp;
+#pragma unroll
+        for (int q = 0; q < kOut; q++) {
+            if (ooff[q] < 0) continue;
+            const int c = opos[q];
             float fx, fy, fz;
             {
 #pragma clang fp contract(off)
                 // phi arrives divided by n^3 (folded into the Green's table)
-                fx = ring[s0][cy][cx - 2] * w0 + ring[s0][cy][cx - 1] * w1 +
-                     ring[s0][cy][cx + 1] * w2 + ring[s0][cy][cx + 2] * w3;
-                fy = ring[s0][cy - 2][cx] * w0 + ring[s0][cy - 1][cx] * w1 +
-                     ring[s0][cy + 1][cx] * w2 + ring[s0][cy + 2][cx] * w3;
-                fz = ring[sm2][cy][cx] * w0 + ring[sm1][cy][cx] * w1 + ring[sp1][cy][cx] * w2 +
-                     ring[sp2][cy][cx] * w3;
+                fx = R0[c - 2] * w0 + R0[c - 1] * w1 + R0[c + 1] * w2 + R0[c + 2] * w3;
+                fy = R0[c - 2 * kRW] * w0 + R0[c - kRW] * w1 + R0[c + kRW] * w2 +
+                     R0[c + 2 * kRW] * w3;
+                fz = Rm2[c] * w0 + Rm1[c] * w1 + Rp1[c] * w2 + Rp2[c] * w3;
             }
-            const int i0 = x0 + lx, j0 = y0 + ly;
-            const size_t u = ((size_t)k0 * nf + j0) * nf + i0;
-            F[u] = fx;
-            F[fs + u] = fy;
-            F[2 * fs + u] = fz;
-            if (i0 >= 1 && i0 <= nfp && j0 >= 1 && j0 <= nfp && k0 >= 1 && k0 <= nfp)
-                f2 = fmaxf(f2, fx * fx + fy * fy + fz * fz);
+            Fx[ooff[q]] = fx;
+            Fx[fs + ooff[q]] = fy;
+            Fx[2 * fs + ooff[q]] = fz;
+            if (inner[q] && kin) f2 = fmaxf(f2, fx * fx + fy * fy + fz * fz);
         }
         __syncthreads();  // the next load overwrites plane k0 - 2
     }
