@@ -16,6 +16,7 @@
 // Unnormalised in both directions (FFTW/cuFFT convention, kick.f90:552).
 #include <cmath>
 #include <map>
+#include <type_traits>
 
 #include "fft.h"
 #include "profile.h"
@@ -140,10 +141,14 @@ __device__ inline int mod_p(int j, int p, float invp) {
 // of line l at l*ls + e*es), in place: every thread first reads the inputs
 // of all its butterflies into registers, the block synchronises, then the
 // outputs are written.  Consecutive threads take consecutive lines.
-template <int R, int S, int CAP>
-__device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ tw, int n, int p,
-                             float invp, int lgL, int ls, int es) {
+// N, PST > 0: the size and the stage's stride p are compile-time constants
+// (the kernels specialised for the meshes of the reference geometries), so
+// that every index of the stage is shifts and constant multiplies.
+template <int R, int S, int CAP, int N = 0, int PST = 0>
+__device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ tw, int n_rt,
+                             int p_rt, float invp, int lgL, int ls, int es) {
     constexpr int KMAX = (CAP / R + kFFTThreads - 1) / kFFTThreads;
+    const int n = N ? N : n_rt, p = PST ? PST : p_rt;
     const int m = n / R, ts = n / (p * R);
     const int L = 1 << lgL;
     const int tasks = L * m;
@@ -165,7 +170,9 @@ __device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ 
             const int l = task & (L - 1), j = task >> lgL;
             // p is the product of the earlier radices: a power of two for
             // every stage before the first radix 3, 5 or 7
-            const int kk = (p & (p - 1)) == 0 ? (j & (p - 1)) : mod_p(j, p, invp);
+            const int kk = PST             ? (int)((unsigned)j % (unsigned)PST)
+                           : (p & (p - 1)) == 0 ? (j & (p - 1))
+                                                : mod_p(j, p, invp);
             if (kk) {
 #pragma unroll
                 for (int q = 1; q < R; q++) {
@@ -200,6 +207,52 @@ __device__ void fft_lds(float2* a, const float2* tw, const FFTPlan& P, int lgL, 
     }
 }
 
+// Compile-time plan of an FFT size: the radices in fft_plan's order (4, 2,
+// 3, 5, 7) and each stage's stride.
+struct CPlan {
+    int nst = 0;
+    int rad[16] = {};
+    int pst[16] = {};
+};
+constexpr CPlan make_cplan(int n) {
+    CPlan P;
+    const int radices[5] = {4, 2, 3, 5, 7};
+    int m = n;
+    for (int k = 0; k < 5; k++)
+        while (m % radices[k] == 0 && P.nst < 16) {
+            P.rad[P.nst++] = radices[k];
+            m /= radices[k];
+        }
+    int p = 1;
+    for (int st = 0; st < P.nst; st++) {
+        P.pst[st] = p;
+        p *= P.rad[st];
+    }
+    return P;
+}
+
+// fft_lds for a size known at compile time (every stage's constants folded).
+template <int S, int N, int CAP, int ST = 0>
+__device__ inline void fft_lds_c(float2* a, const float2* tw, int lgL, int ls, int es) {
+    constexpr CPlan P = make_cplan(N);
+    if constexpr (ST < P.nst) {
+        constexpr int r = P.rad[ST], p = P.pst[ST];
+        static_assert(r == 2 || r == 3 || r == 4 || r == 5, "no radix 7 in the specialised sizes");
+        stage<r, S, CAP, N, p>(a, tw, N, p, 1.0f / (float)p, lgL, ls, es);
+        fft_lds_c<S, N, CAP, ST + 1>(a, tw, lgL, ls, es);
+    }
+}
+
+// The generic transform (N = 0) or the specialised one.
+template <int S, int CAP, bool R7, int N>
+__device__ inline void fft_run(float2* a, const float2* tw, const FFTPlan& P, int lgL, int ls,
+                               int es) {
+    if constexpr (N == 0)
+        fft_lds<S, CAP, R7>(a, tw, P, lgL, ls, es);
+    else
+        fft_lds_c<S, N, CAP>(a, tw, lgL, ls, es);
+}
+
 __device__ inline void load_twiddles(float2* tw, const FFTPlan& P) {
     for (int t = threadIdx.x; t < P.n; t += blockDim.x) tw[t] = P.tw[t];
 }
@@ -217,13 +270,13 @@ __device__ inline float* row_pair(float* d, int g, int n, size_t rs) {
 
 // Lines are row pairs; 2^lgL of them per block, one wavefront per line at a
 // time along the row (coalesced).
-template <bool R7>
+template <bool R7, int N = 0>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d, FFTPlan P,
                                                            int npairs, size_t rs, int lgL,
                                                            const float* __restrict__ scale_dev) {
     const float scale = scale_dev ? *scale_dev : 1.0f;
     extern __shared__ float2 sm[];
-    const int n = P.n, nh = n / 2 + 1;
+    const int n = N ? N : P.n, nh = n / 2 + 1;
     const int L = 1 << lgL, es = L + 1;  // padded: the lanes of a row hit distinct banks
     float2* tw = sm;
     float2* A = sm + n;
@@ -239,7 +292,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d
         }
     }
     __syncthreads();
-    fft_lds<-1, kFFTElems, R7>(A, tw, P, lgL, 1, es);
+    fft_run<-1, kFFTElems, R7, N>(A, tw, P, lgL, 1, es);
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) break;
         float2* ra = reinterpret_cast<float2*>(row_pair(d, g0 + l, n, rs));
@@ -261,12 +314,12 @@ __device__ inline float* row_pair_w(float* d, int g, int n, size_t rs, FFTWin w)
     return d + (size_t)job * rs + ((size_t)z * n + 2 * yp) * (n + 2);
 }
 
-template <bool R7>
+template <bool R7, int N = 0>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d, FFTPlan P,
                                                            int npairs, size_t rs, int lgL,
                                                            FFTWin w) {
     extern __shared__ float2 sm[];
-    const int n = P.n, nh = n / 2 + 1, half = n / 2;
+    const int n = N ? N : P.n, nh = n / 2 + 1, half = n / 2;
     const int L = 1 << lgL, es = L + 1;  // padded: the lanes of a row hit distinct banks
     float2* tw = sm;
     float2* A = sm + n;
@@ -297,7 +350,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
         }
     }
     __syncthreads();
-    fft_lds<1, kFFTElems, R7>(A, tw, P, lgL, 1, es);
+    fft_run<1, kFFTElems, R7, N>(A, tw, P, lgL, 1, es);
     for (int l = wave; l < L; l += kFFTThreads / 64) {
         if (g0 + l >= npairs) break;
         float* ra = row_pair_w(d, g0 + l, n, rs, w);
@@ -311,13 +364,13 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
 
 // Tiles of C = 2^lgC consecutive kx columns x n rows along y (axis 1) or z
 // (axis 2); LDS element (row i, column c) at i*C + c.
-template <int MODE, int CAP, bool R7>
+template <int MODE, int CAP, bool R7, int N = 0>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dreal, FFTPlan P,
                                                           size_t rs, int axis, int lgC,
                                                           const float* __restrict__ green, int o0,
                                                           int ilo, int ihi) {
     extern __shared__ float2 sm[];
-    const int n = P.n, nh = n / 2 + 1;
+    const int n = N ? N : P.n, nh = n / 2 + 1;
     const int C = 1 << lgC;
     const int ntiles = (nh + C - 1) >> lgC;
     float2* tw = sm;
@@ -340,9 +393,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
     }
     __syncthreads();
     if (MODE == 1) {
-        fft_lds<1, CAP, R7>(A, tw, P, lgC, 1, C);
+        fft_run<1, CAP, R7, N>(A, tw, P, lgC, 1, C);
     } else {
-        fft_lds<-1, CAP, R7>(A, tw, P, lgC, 1, C);
+        fft_run<-1, CAP, R7, N>(A, tw, P, lgC, 1, C);
         if (MODE == 2) {
             // Green's table in tile order: [y][tile][z][c], contiguous per tile
             const float* gt = green + ((size_t)o * ntiles + tile) * n * C;
@@ -354,7 +407,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dr
                 }
             }
             __syncthreads();
-            fft_lds<1, CAP, R7>(A, tw, P, lgC, 1, C);
+            fft_run<1, CAP, R7, N>(A, tw, P, lgC, 1, C);
         }
     }
     // rows ilo .. ihi-1 go back (an inverse z pass stores only the planes
@@ -389,6 +442,40 @@ __global__ void k_green_tile(const float* __restrict__ G, float* __restrict__ T,
 // ------------------------------------------------------------- host side
 
 static std::map<int, FFTPlan> g_plans;
+
+// Sizes whose kernels are specialised at compile time (fft_lds_c): the PM3
+// meshes of the 256^3 geometry (levels 2..6) and of the 64^3 one; the others
+// run the generic kernels.  f is called with std::integral_constant<int, N>,
+// N = 0 for the generic kernels.
+template <class F>
+static void with_size(int n, bool has7, F&& f) {
+    if (!has7) switch (n) {
+            case 80: return f(std::integral_constant<int, 80>{});
+            case 144: return f(std::integral_constant<int, 144>{});
+            case 160: return f(std::integral_constant<int, 160>{});
+            case 240: return f(std::integral_constant<int, 240>{});
+            case 320: return f(std::integral_constant<int, 320>{});
+            case 480: return f(std::integral_constant<int, 480>{});
+            default: break;
+        }
+    f(std::integral_constant<int, 0>{});
+}
+
+template <int N>
+static int set_lds_attributes_n() {
+    const int bytes = (kFFTElems + 2048) * (int)sizeof(float2);
+    const int bytes_cols = (kColsMax + 1024) * (int)sizeof(float2);
+    const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_r2c<false, N>, a, bytes));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_x_c2r<false, N>, a, bytes));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kFFTElems, false, N>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kFFTElems, false, N>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kFFTElems, false, N>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kColsMax, false, N>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kColsMax, false, N>, a, bytes_cols));
+    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kColsMax, false, N>, a, bytes_cols));
+    return CUBE2_OK;
+}
 static bool g_attr_set = false;
 
 template <bool R7>
@@ -413,6 +500,12 @@ static int set_lds_attributes() {
     // x passes: n twiddles + n (L+1) <= n + kFFTElems + n elements
     C2_TRY(set_lds_attributes_r<false>());
     C2_TRY(set_lds_attributes_r<true>());
+    C2_TRY(set_lds_attributes_n<80>());
+    C2_TRY(set_lds_attributes_n<144>());
+    C2_TRY(set_lds_attributes_n<160>());
+    C2_TRY(set_lds_attributes_n<240>());
+    C2_TRY(set_lds_attributes_n<320>());
+    C2_TRY(set_lds_attributes_n<480>());
     g_attr_set = true;
     return CUBE2_OK;
 }
@@ -482,8 +575,13 @@ int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale,
     }
     const size_t lds = (size_t)(n + (L + 1) * n) * sizeof(float2);
     const unsigned grid = (unsigned)((npairs + L - 1) / L);
-    if (P.has7) k_fft_x_r2c<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
-    else k_fft_x_r2c<false><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
+    with_size(n, P.has7, [&](auto c) {
+        constexpr int N = decltype(c)::value;
+        if (N == 0 && P.has7)
+            k_fft_x_r2c<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
+        else
+            k_fft_x_r2c<false, N><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, scale);
+    });
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
@@ -504,8 +602,13 @@ int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s, cons
     }
     const size_t lds = (size_t)(n + (L + 1) * n) * sizeof(float2);
     const unsigned grid = (unsigned)((npairs + L - 1) / L);
-    if (P.has7) k_fft_x_c2r<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, w);
-    else k_fft_x_c2r<false><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, w);
+    with_size(n, P.has7, [&](auto c) {
+        constexpr int N = decltype(c)::value;
+        if (N == 0 && P.has7)
+            k_fft_x_c2r<true><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, w);
+        else
+            k_fft_x_c2r<false, N><<<grid, kFFTThreads, lds, s>>>(d, P, (int)npairs, rs, lgL, w);
+    });
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
@@ -522,18 +625,18 @@ int fft_cols_lgC(int n) {
     return lgC;
 }
 
-template <bool R7>
+template <bool R7, int N = 0>
 static void launch_cols(bool wide, int mode, dim3 grid, size_t lds, hipStream_t s, float* d,
                         const FFTPlan& P, size_t rs, int axis, int lgC, const float* green,
                         int o0, int ilo, int ihi) {
     if (wide) {
-        if (mode == 0) k_fft_cols<0, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
-        else if (mode == 1) k_fft_cols<1, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
-        else k_fft_cols<2, kColsMax, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        if (mode == 0) k_fft_cols<0, kColsMax, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else if (mode == 1) k_fft_cols<1, kColsMax, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else k_fft_cols<2, kColsMax, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
     } else {
-        if (mode == 0) k_fft_cols<0, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
-        else if (mode == 1) k_fft_cols<1, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
-        else k_fft_cols<2, kFFTElems, R7><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        if (mode == 0) k_fft_cols<0, kFFTElems, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else if (mode == 1) k_fft_cols<1, kFFTElems, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
+        else k_fft_cols<2, kFFTElems, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
     }
 }
 
@@ -561,8 +664,15 @@ int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
     }
     dim3 grid((unsigned)(no * ntiles), (unsigned)nb);
     const size_t lds = (size_t)(n + C * n) * sizeof(float2);
-    if (P.has7) launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0, ilo, ihi);
-    else launch_cols<false>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0, ilo, ihi);
+    with_size(n, P.has7, [&](auto c) {
+        constexpr int N = decltype(c)::value;
+        if (N == 0 && P.has7)
+            launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0,
+                              ilo, ihi);
+        else
+            launch_cols<false, N>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC,
+                                  green, o0, ilo, ihi);
+    });
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
 }
