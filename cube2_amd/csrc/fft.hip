@@ -233,7 +233,7 @@ constexpr CPlan make_cplan(int n) {
 
 // fft_lds for a size known at compile time (every stage's constants folded).
 template <int S, int N, int CAP, int ST = 0>
-__device__ inline void fft_lds_c(float2* a, const float2* tw, int lgL, int ls, int es) {
+__device__ __forceinline__ void fft_lds_c(float2* a, const float2* tw, int lgL, int ls, int es) {
     constexpr CPlan P = make_cplan(N);
     if constexpr (ST < P.nst) {
         constexpr int r = P.rad[ST], p = P.pst[ST];
@@ -245,7 +245,7 @@ __device__ inline void fft_lds_c(float2* a, const float2* tw, int lgL, int ls, i
 
 // The generic transform (N = 0) or the specialised one.
 template <int S, int CAP, bool R7, int N>
-__device__ inline void fft_run(float2* a, const float2* tw, const FFTPlan& P, int lgL, int ls,
+__device__ __forceinline__ void fft_run(float2* a, const float2* tw, const FFTPlan& P, int lgL, int ls,
                                int es) {
     if constexpr (N == 0)
         fft_lds<S, CAP, R7>(a, tw, P, lgL, ls, es);
