@@ -382,15 +382,17 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
             L.rp[lane] = inc;
             wave_lds_sync();
             if (vi && jslot == 0) pc += (unsigned long long)M;
+            // a lane's candidates v = vb + lane only grow, so its row (the
+            // first with rp > v) is found by walking forward from the last
+            // one: rp of the rows past the group is M > v, so the walk stops
+            // inside the group
+            int frow = 0;
             auto fetch = [&](int vb) {
                 const int v = vb + lane;
                 float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (v < M) {
-                    int lo = 0;  // first row with rp > v
-#pragma unroll
-                    for (int step = 32; step > 0; step >>= 1)
-                        if (L.rp[lo + step - 1] <= v) lo += step;
-                    q4 = A.pos4[v + L.rs[lo]];
+                    while (L.rp[frow] <= v) frow++;
+                    q4 = A.pos4[v + L.rs[frow]];
                 }
                 return q4;
             };
