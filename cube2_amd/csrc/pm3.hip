@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __rest
     __shared__ unsigned long long box[kBrick * kBrick * kBrick];
     __shared__ int cstart[kBrickCells + 1];  // exclusive prefix of the cells' counts
     __shared__ int64_t cbase[kBrickCells];   // first xp slot of each cell
-    __shared__ int ccell[3][kBrickCells];    // its nte coordinates
+    __shared__ int cmesh[3][kBrickCells];    // its mesh coordinate origin (ci-ncb)*cf - u*nfp
     __shared__ int wsum[4];
     const int n = A.n, off = A.nfb - 1;  // storage x = Fortran index + nfb - 1
     // neighbouring bricks visit the same particles: they run on one XCD
@@ -395,9 +395,9 @@ __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __rest
         if (q < ncell) {
             const int ci = clo[0] + q % ext[0], cj = clo[1] + (q / ext[0]) % ext[1],
                       ck = clo[2] + q / (ext[0] * ext[1]);
-            ccell[0][q] = ci;
-            ccell[1][q] = cj;
-            ccell[2][q] = ck;
+            cmesh[0][q] = (ci - A.ncb) * cf - J.u[0] * A.nfp;
+            cmesh[1][q] = (cj - A.ncb) * cf - J.u[1] * A.nfp;
+            cmesh[2][q] = (ck - A.ncb) * cf - J.u[2] * A.nfp;
             const int64_t cell = (((int64_t)J.t * A.nte + ck) * A.nte + cj) * A.nte + ci;
             np = A.rhoc[cell];
             base = A.nzero[cell];
@@ -428,22 +428,28 @@ __global__ __launch_bounds__(256) void k_brick_deposit(MeshArgs A, float* __rest
     int c = 0;
     for (int v = threadIdx.x; v < M; v += blockDim.x) {
         while (cstart[c + 1] <= v) c++;  // v < M: stops at a cell that holds v
-        const int ci = ccell[0][c], cj = ccell[1][c], ck = ccell[2][c];
         const int16_t* p = A.xp + 3 * (cbase[c] + (v - cstart[c]));
-        // floor of the mesh coordinate in integer arithmetic (the same value
-        // tsc_axis computes): particles that miss the brick are rejected
-        // before the FP64 work
-        const int X0 = fine_floor(A, ci, p[0], J.u[0], cf) + off - lo[0];
-        const int Y0 = fine_floor(A, cj, p[1], J.u[1], cf) + off - lo[1];
-        const int Z0 = fine_floor(A, ck, p[2], J.u[2], cf) + off - lo[2];
+        // The mesh coordinate is the cell's origin plus frac * cf, frac =
+        // (U + 1/2) / 2^16 with U the 16-bit code read unsigned; its floor is
+        // origin + ((2U + 1) cf >> 17) in integer arithmetic (the value
+        // tsc_axis computes), so particles that miss the brick are rejected
+        // before the FP64 work.  (2U + 1) cf < 2^24: a 24-bit multiply.
+        const int fx = (2 * (int)(uint16_t)p[0] + 1) * cf, fy = (2 * (int)(uint16_t)p[1] + 1) * cf,
+                  fz = (2 * (int)(uint16_t)p[2] + 1) * cf;
+        const int mx = cmesh[0][c], my = cmesh[1][c], mz = cmesh[2][c];
+        const int X0 = mx + (fx >> 17) + off - lo[0];
+        const int Y0 = my + (fy >> 17) + off - lo[1];
+        const int Z0 = mz + (fz >> 17) + off - lo[2];
         if (X0 > kBrick - 1 || X0 < -2 || Y0 > kBrick - 1 || Y0 < -2 || Z0 > kBrick - 1 ||
             Z0 < -2)
             continue;
         int cx, cy, cz;
         float wx[3], wy[3], wz[3];
-        tsc_axis(mesh_coord(A, ci, p[0], J.u[0]), cx, wx);
-        tsc_axis(mesh_coord(A, cj, p[1], J.u[1]), cy, wy);
-        tsc_axis(mesh_coord(A, ck, p[2], J.u[2]), cz, wz);
+        // mesh_coord's value: every term is exact in FP64, so origin +
+        // (2U + 1) cf 2^-17 equals ((c - ncb) + frac) cf - u nfp
+        tsc_axis((double)mx + (double)fx * 0x1p-17, cx, wx);
+        tsc_axis((double)my + (double)fy * 0x1p-17, cy, wy);
+        tsc_axis((double)mz + (double)fz * 0x1p-17, cz, wz);
         (void)cx, (void)cy, (void)cz;  // equal to the integer floors above
         if (rot)
             brick_add<true>(box, X0, Y0, Z0, wx, wy, wz);
