@@ -27,6 +27,23 @@ constexpr int kFFTThreads = 256;
 constexpr int kFFTElems = 2048;  // complex elements per LDS tile (measured best: 2048 > 4096 > 1024)
 constexpr int kColsMax = 4096;   // largest LDS tile of the y/z passes (wider minimum tiles)
 
+constexpr int lg_floor_c(int v) {
+    int l = 0;
+    while ((2 << l) <= v) l++;
+    return l;
+}
+// The built-in shapes of a size (the host code applies the same rules):
+// log2 of the row pairs per block of the x passes, and of the column-tile
+// width of the y/z passes (about kFFTElems LDS elements per tile, no wider
+// than the half spectrum, within kColsMax).
+constexpr int x_lgl(int n) { return lg_floor_c(kFFTElems / n > 1 ? kFFTElems / n : 1); }
+constexpr int cols_lgc_builtin(int n) {
+    int lg = x_lgl(n);
+    while (lg > 0 && (1 << lg) > 2 * (n / 2 + 1)) lg--;
+    while (lg > 0 && (1 << lg) * n > kColsMax) lg--;
+    return lg;
+}
+
 __device__ inline float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -272,11 +289,12 @@ __device__ inline float* row_pair(float* d, int g, int n, size_t rs) {
 // time along the row (coalesced).
 template <bool R7, int N = 0>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_r2c(float* __restrict__ d, FFTPlan P,
-                                                           int npairs, size_t rs, int lgL,
+                                                           int npairs, size_t rs, int lgL_arg,
                                                            const float* __restrict__ scale_dev) {
     const float scale = scale_dev ? *scale_dev : 1.0f;
     extern __shared__ float2 sm[];
     const int n = N ? N : P.n, nh = n / 2 + 1;
+    const int lgL = N ? x_lgl(N) : lgL_arg;
     const int L = 1 << lgL, es = L + 1;  // padded: the lanes of a row hit distinct banks
     float2* tw = sm;
     float2* A = sm + n;
@@ -316,10 +334,11 @@ __device__ inline float* row_pair_w(float* d, int g, int n, size_t rs, FFTWin w)
 
 template <bool R7, int N = 0>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d, FFTPlan P,
-                                                           int npairs, size_t rs, int lgL,
+                                                           int npairs, size_t rs, int lgL_arg,
                                                            FFTWin w) {
     extern __shared__ float2 sm[];
     const int n = N ? N : P.n, nh = n / 2 + 1, half = n / 2;
+    const int lgL = N ? x_lgl(N) : lgL_arg;
     const int L = 1 << lgL, es = L + 1;  // padded: the lanes of a row hit distinct banks
     float2* tw = sm;
     float2* A = sm + n;
@@ -366,11 +385,13 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_x_c2r(float* __restrict__ d
 // (axis 2); LDS element (row i, column c) at i*C + c.
 template <int MODE, int CAP, bool R7, int N = 0>
 __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(float* __restrict__ dreal, FFTPlan P,
-                                                          size_t rs, int axis, int lgC,
+                                                          size_t rs, int axis, int lgC_arg,
                                                           const float* __restrict__ green, int o0,
                                                           int ilo, int ihi) {
     extern __shared__ float2 sm[];
     const int n = N ? N : P.n, nh = n / 2 + 1;
+    // a specialised size runs with its built-in tile width (the host checks)
+    const int lgC = N ? cols_lgc_builtin(N) : lgC_arg;
     const int C = 1 << lgC;
     const int ntiles = (nh + C - 1) >> lgC;
     float2* tw = sm;
@@ -471,9 +492,6 @@ static int set_lds_attributes_n() {
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kFFTElems, false, N>, a, bytes_cols));
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kFFTElems, false, N>, a, bytes_cols));
     C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kFFTElems, false, N>, a, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<0, kColsMax, false, N>, a, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<1, kColsMax, false, N>, a, bytes_cols));
-    C2_HIP(hipFuncSetAttribute((const void*)k_fft_cols<2, kColsMax, false, N>, a, bytes_cols));
     return CUBE2_OK;
 }
 static bool g_attr_set = false;
@@ -566,7 +584,7 @@ static int lg_floor(int v) {
 
 int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale, hipStream_t s) {
     const int n = P.n;
-    const int lgL = lg_floor(std::max(1, kFFTElems / n));
+    const int lgL = x_lgl(n);  // the specialised kernels assume it
     const int L = 1 << lgL;
     const int64_t npairs = (int64_t)nb * n * (n / 2);
     if (npairs >= (int64_t)1 << 31) {
@@ -588,7 +606,7 @@ int fft_x_r2c(float* d, const FFTPlan& P, int nb, size_t rs, const float* scale,
 
 int fft_x_c2r(float* d, const FFTPlan& P, int nb, size_t rs, hipStream_t s, const FFTWin* win) {
     const int n = P.n;
-    const int lgL = lg_floor(std::max(1, kFFTElems / n));
+    const int lgL = x_lgl(n);  // the specialised kernels assume it
     const int L = 1 << lgL;
     const FFTWin w = win ? *win : FFTWin{0, n, 0, n / 2};
     if (w.z0 < 0 || w.nz < 1 || w.z0 + w.nz > n || w.yp0 < 0 || w.nyp < 1 || w.yp0 + w.nyp > n / 2) {
@@ -629,6 +647,7 @@ template <bool R7, int N = 0>
 static void launch_cols(bool wide, int mode, dim3 grid, size_t lds, hipStream_t s, float* d,
                         const FFTPlan& P, size_t rs, int axis, int lgC, const float* green,
                         int o0, int ilo, int ihi) {
+    if constexpr (N != 0) wide = false;  // never wide (fft_cols dispatches so)
     if (wide) {
         if (mode == 0) k_fft_cols<0, kColsMax, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
         else if (mode == 1) k_fft_cols<1, kColsMax, R7, N><<<grid, kFFTThreads, lds, s>>>(d, P, rs, axis, lgC, green, o0, ilo, ihi);
@@ -664,14 +683,16 @@ int fft_cols(float* d, const FFTPlan& P, int nb, size_t rs, int axis, int mode,
     }
     dim3 grid((unsigned)(no * ntiles), (unsigned)nb);
     const size_t lds = (size_t)(n + C * n) * sizeof(float2);
-    with_size(n, P.has7, [&](auto c) {
+    const bool wide = C * n > kFFTElems;
+    // the specialised kernels exist for the built-in tile width only
+    const bool builtin = !wide && lgC == cols_lgc_builtin(n);
+    with_size(builtin ? n : 0, P.has7, [&](auto c) {
         constexpr int N = decltype(c)::value;
         if (N == 0 && P.has7)
-            launch_cols<true>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0,
-                              ilo, ihi);
+            launch_cols<true>(wide, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0, ilo, ihi);
         else
-            launch_cols<false, N>(C * n > kFFTElems, mode, grid, lds, s, d, P, rs, axis, lgC,
-                                  green, o0, ilo, ihi);
+            launch_cols<false, N>(wide, mode, grid, lds, s, d, P, rs, axis, lgC, green, o0, ilo,
+                                  ihi);
     });
     C2_HIP(hipGetLastError());
     return CUBE2_OK;
