@@ -190,7 +190,9 @@ __device__ inline void stage(float2* __restrict__ a, const float2* __restrict__ 
             const int kk = PST             ? (int)((unsigned)j % (unsigned)PST)
                            : (p & (p - 1)) == 0 ? (j & (p - 1))
                                                 : mod_p(j, p, invp);
-            if (kk) {
+            // a specialised stage after the first multiplies every task by
+            // its twiddles, kk = 0 (w = 1) included: no branch to merge
+            if (PST ? PST > 1 : kk != 0) {
 #pragma unroll
                 for (int q = 1; q < R; q++) {
                     float2 w = tw[q * kk * ts];
