@@ -592,7 +592,7 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
         const float* R0 = ring0 + (k0 % 5) * kSlot;
         const float* Rp1 = ring0 + ((k0 + 1) % 5) * kSlot;
         const float* Rp2 = ring0 + ((k0 + 2) % 5) * kSlot;
-        float* Fx = F + (size_t)k0 * nf * nf;
+        float* Fx = F + (size_t)3 * k0 * nf * nf;  // force points interleaved (x, y, z)
         const bool kin = k0 >= 1 && k0 <= nfp;
 #pragma unroll
         for (int q = 0; q < kOut; q++) {
@@ -607,9 +607,10 @@ __global__ __launch_bounds__(256) void k_pm3_grad(const float* __restrict__ phi,
                      R0[c + 2 * kRW] * w3;
                 fz = Rm2[c] * w0 + Rm1[c] * w1 + Rp1[c] * w2 + Rp2[c] * w3;
             }
-            Fx[ooff[q]] = fx;
-            Fx[fs + ooff[q]] = fy;
-            Fx[2 * fs + ooff[q]] = fz;
+            float* Fp = Fx + 3 * (size_t)ooff[q];
+            Fp[0] = fx;
+            Fp[1] = fy;
+            Fp[2] = fz;
             if (inner[q] && kin) f2 = fmaxf(f2, fx * fx + fy * fy + fz * fz);
         }
         __syncthreads();  // the next load overwrites plane k0 - 2
@@ -689,9 +690,10 @@ __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __r
                     for (int a1 = 0; a1 < 3; a1++) {
 #pragma clang fp contract(off)
                         const size_t u = ((size_t)(cz + a3) * nf + (cy + a2)) * nf + (cx + a1);
-                        X = X + F[u] * wx[a1] * wy[a2] * wz[a3];
-                        Y = Y + F[fs + u] * wx[a1] * wy[a2] * wz[a3];
-                        Z = Z + F[2 * fs + u] * wx[a1] * wy[a2] * wz[a3];
+                        const float* Fp = F + 3 * u;  // one 12-byte point
+                        X = X + Fp[0] * wx[a1] * wy[a2] * wz[a3];
+                        Y = Y + Fp[1] * wx[a1] * wy[a2] * wz[a3];
+                        Z = Z + Fp[2] * wx[a1] * wy[a2] * wz[a3];
                     }
             const float acc[3] = {X, Y, Z};
 #pragma unroll
@@ -1007,9 +1009,12 @@ static int run_batch(const MeshArgs& base, const PM3Job* d_jobs, size_t nb,
         k_pm3_grad<<<grid, 256, 0, s>>>(rho, rs, force, n, nfp, base.nfb, W, ntx, Zc, f2bits);
         C2_HIP(hipGetLastError());
     }
-    if (dbg && dbg->force) {
-        C2_HIP(hipMemcpyAsync(dbg->force, force, 3 * fs * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (dbg && dbg->force) {  // the planes [3][nf^3] from the interleaved points
+        std::vector<float> h(3 * fs);
+        C2_HIP(hipMemcpyAsync(h.data(), force, 3 * fs * sizeof(float), hipMemcpyDeviceToHost, s));
         C2_HIP(hipStreamSynchronize(s));
+        for (size_t u = 0; u < fs; u++)
+            for (int c = 0; c < 3; c++) dbg->force[c * fs + u] = h[3 * u + c];
     }
     if (dv64) {
         const int seg64 = ctx().tune[CUBE2_TUNE_INTERP_SEG] ? ctx().tune[CUBE2_TUNE_INTERP_SEG] : 4;
