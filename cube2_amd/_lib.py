@@ -18,10 +18,11 @@ _LIB = None
 OK, ERR_ARG, ERR_GEOMETRY, ERR_CAPACITY, ERR_HIP, ERR_STATE = range(6)
 PP_EXACT, PP_FAST, PP_REF, PP_PRECISE, PP_PLUMMER = 0, 1, 2, 3, 4
 (TUNE_PP_BLOCKS_PER_CU, TUNE_PP_DIRECT_LG, TUNE_FFT_MIN_LGC, TUNE_INTERP_SEG, TUNE_PP_DENSE_LG,
- TUNE_KICK_OVERLAP, TUNE_PP_OVERLAP_BLOCKS, TUNE_PP_ORDER, TUNE_PP_JBATCH) = range(9)
+ TUNE_KICK_OVERLAP, TUNE_PP_OVERLAP_BLOCKS, TUNE_PP_ORDER, TUNE_PP_JBATCH,
+ TUNE_PP_NEWTON) = range(10)
 TUNE_NAMES = {"pp_blocks_per_cu": 0, "pp_direct_lg": 1, "fft_min_lgc": 2, "interp_seg": 3,
               "pp_dense_lg": 4, "kick_overlap": 5, "pp_overlap_blocks": 6, "pp_order": 7,
-              "pp_jbatch": 8}
+              "pp_jbatch": 8, "pp_newton": 9}
 
 
 class Cube2Error(RuntimeError):

@@ -21,6 +21,7 @@ CANDIDATES = {
     "interp_seg": [0, 2, 8],
     "pp_order": [0, 20],
     "pp_jbatch": [0, 2],
+    "pp_newton": [0, 2],
 }
 
 

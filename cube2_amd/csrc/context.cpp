@@ -304,8 +304,8 @@ int cube2_set_params(const cube2_params_t* p) {
 }
 
 int cube2_set_tuning(int key, int value) {
-    static const int lo[CUBE2_TUNE_COUNT] = {1, 1, 1, 1, 3, 1, 1, 1, 1},
-                     hi[CUBE2_TUNE_COUNT] = {16, 7, 6, 32, 6, 2, 64, 30, 2};
+    static const int lo[CUBE2_TUNE_COUNT] = {1, 1, 1, 1, 3, 1, 1, 1, 1, 1},
+                     hi[CUBE2_TUNE_COUNT] = {16, 7, 6, 32, 6, 2, 64, 30, 2, 2};
     if (key < 0 || key >= CUBE2_TUNE_COUNT || (value != 0 && (value < lo[key] || value > hi[key]))) {
         set_error("tuning key %d value %d out of range", key, value);
         return CUBE2_ERR_ARG;
@@ -324,7 +324,7 @@ int cube2_get_tuning(int key, int* value) {
         set_error("bad tuning key %d", key);
         return CUBE2_ERR_ARG;
     }
-    static const int builtin[CUBE2_TUNE_COUNT] = {0, 2, 0, 4, 4, 1, 0, 1, 1};
+    static const int builtin[CUBE2_TUNE_COUNT] = {0, 2, 0, 4, 4, 1, 0, 1, 1, 1};
     const int v = ctx().tune[key];
     *value = v ? v : builtin[key];
     if (key == CUBE2_TUNE_PP_BLOCKS_PER_CU && !v) *value = 0;  // occupancy-derived at launch

@@ -33,6 +33,8 @@ struct PPPlan {
     bool split = false;        // pair sums apart from the velocity update (pp_enqueue_apply)
     bool learned = false;      // the network stands in for PP in sparse level-2 cells
     bool all_active = false;   // every subtile has a PP range (apm3 > 0 at its level)
+    bool newton = false;       // pairs of one range evaluated once (split pass)
+    float react_scale = 0.0f;  // its fixed-point scale, a power of two
     LearnedPlan lp;
     std::array<int, 7> ratio_sf{};
     std::array<float, 7> apm3{};
@@ -52,7 +54,7 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
              int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
              double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned = false);
 // Blocks of the persistent pair kernel (overlap: beside PM3).
-int pp_tiled_grid(int mode, bool overlap);
+int pp_tiled_grid(int mode, bool overlap, bool newton = false);
 // The last PP call's sorted particles and cell lists (for the learned
 // stand-in's training inputs); CUBE2_ERR_STATE if there is none.
 int pp_last(const PPPlan** P, PPArgs* A, const int** tile_first);

@@ -359,7 +359,8 @@ static int graph_record(const GraphKey& K, const int* ixyz3, const float* apm3,
                            g_graph.pm3));
     C2_TRY(pp_prepare(K.ires.data(), ixyz3, apm3, ratio_sf, K.n_used, g_graph.pp,
                       K.overlap || K.learned, K.learned));
-    if (!K.overlap) g_graph.pp.grid_pairs = pp_tiled_grid(g_graph.pp.mode, false);
+    if (!K.overlap)
+        g_graph.pp.grid_pairs = pp_tiled_grid(g_graph.pp.mode, false, g_graph.pp.newton);
     C2_TRY(C.dv64.reserve((size_t)std::max<int64_t>(K.n_used, 1) * 3 * sizeof(double)));
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, a_mid, dt, s, &d_scal));  // reserves the scalar blocks

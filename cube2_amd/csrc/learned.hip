@@ -108,7 +108,7 @@ __global__ void k_learned_finish(float* __restrict__ X, int64_t ncell) {
 // in a PP cell of fewer than occ particles.
 __device__ inline int learned_job_of(const PPArgs& A, const LearnedArgs& L, int a, float4& p) {
     p = A.pos4[a];
-    if (p.w != L.range) return -1;
+    if (fabsf(p.w) != L.range) return -1;  // (w < 0: pairs-once small cell)
     const uint32_t cid = A.cellid[a];
     if (A.cell_start[cid + 1] - A.cell_start[cid] >= L.occupancy) return -1;
     const int t = find_tile_l(A.tiles, A.ntile, cid);

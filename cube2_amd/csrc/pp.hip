@@ -36,14 +36,16 @@ namespace cube2 {
 
 int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int grid, int mode,
                     hipStream_t s);
-int pp_tiled_grid(int mode, bool overlap);
+int pp_tiled_grid(int mode, bool overlap, bool newton);
 
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
-    DevBuf chunk_flags, chunks, ilist, big, dense;
+    DevBuf chunk_flags, chunks, big, dense;
+    DevBuf sub_pos4, sub_slot;  // staging of the dense cells' sub-cell reorder
     DevBuf order, order_bin, order_hist;  // chunks heaviest first (k_chunk_cost/order)
     DevBuf term;  // split pass: per slot velocity term [n_used][3] (active slots)
     DevBuf rows_len, rows_base;  // split pass: physical rows' slot counts (scanned) and first slots
+    DevBuf react;  // pairs-once pass: per sorted particle 64-bit fixed-point sums [cap][3]
     HostBuf h_res;  // PPRed, per-location counts, per-tile pairs, tile_first (pinned)
     std::vector<PPTile> h_tiles;
     std::vector<uint64_t> tile_pairs;  // [ntile][2] of the last call (when counted)
@@ -386,9 +388,13 @@ __global__ void k_pp_ptotal(const int* __restrict__ rhoc, int nt, int ncb, int n
 // where a chunk starts.
 // The sorted count n comes from the scan on the device; flags[a] = 0 for
 // n <= a < cap, and for every a if the call has failed.
-__global__ void k_chunk_flags(const float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
+// Pairs-once pass (small > 0): the particles of active cells of at most
+// `small` particles, whose chunks evaluate their pairs one-sidedly, are
+// marked by a negative range (w = -range); every reader of w then takes |w|
+// as the range and w != 0 as active.
+__global__ void k_chunk_flags(float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
                               const int* __restrict__ cell_start, const int* __restrict__ n_sorted,
-                              const int* __restrict__ err, int cap, int dense_lg,
+                              const int* __restrict__ err, int cap, int dense_lg, int small,
                               int* __restrict__ flags, int* __restrict__ dense_list,
                               int* __restrict__ ndense) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -397,11 +403,12 @@ __global__ void k_chunk_flags(const float4* __restrict__ pos4, const uint32_t* _
     int f = 0;
     if (a < n && pos4[a].w > 0.0f) {
         const uint32_t c = cellid[a];
-        const int c0 = cell_start[c];
-        const bool dense = cell_start[c + 1] - c0 >= 64;
+        const int c0 = cell_start[c], cn = cell_start[c + 1] - c0;
+        const bool dense = cn >= 64;
         const int mask = dense ? (1 << dense_lg) - 1 : 63;
         f = ((a - c0) & mask) == 0;
         if (dense && a == c0) dense_list[atomicAdd(ndense, 1)] = (int)c;
+        if (cn <= small) pos4[a].w = -pos4[a].w;
     }
     flags[a] = f;
 }
@@ -462,7 +469,7 @@ __global__ __launch_bounds__(kOrderBins) void k_chunk_cost(PPArgs A, const int* 
             else hi = mid - 1;
         }
         const int rcp = A.tiles[lo].rcp;
-        const int W = 2 * (int)ceilf(A.pos4[a0].w * (float)rcp / (float)A.ratio_cs) + 1;
+        const int W = 2 * (int)ceilf(fabsf(A.pos4[a0].w) * (float)rcp / (float)A.ratio_cs) + 1;
         const float cost = (float)(I + 1) * (float)(cend - cbeg) * (float)(W * W * W);
         const int k = min(kOrderBins - 1, (int)(8.0f * log2f(cost + 1.0f)));
         const int b = k >= 8 * heavy_lg ? kOrderBins - 1 - k : kOrderBins - 1;
@@ -524,14 +531,15 @@ __global__ __launch_bounds__(kOrderBins) void k_chunk_order(const int* __restric
 __device__ inline int spread3(int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4); }
 
 // One wavefront per PP cell of at least 64 particles (listed by
-// k_chunk_flags): ilist[cell range] = the cell's sorted positions ordered by
-// an 8x8x8 Morton sub-cell key (counting sort in LDS), so that consecutive
-// chunks of a dense cell are spatially compact.  The order of the cell list
-// itself (the j stream) is not changed.
+// k_chunk_flags): the cell's particles (pos4, slot) are reordered in place by
+// an 8x8x8 Morton sub-cell key (counting sort in LDS, staged through tpos /
+// tslot), so that consecutive chunks of a dense cell are spatially compact
+// and a chunk is a range of sorted indices.  Cells keep their ranges.
 __global__ __launch_bounds__(256) void k_pp_subsort(
-    const float4* __restrict__ pos4, const int* __restrict__ cell_start,
+    float4* __restrict__ pos4, uint32_t* __restrict__ slot, const int* __restrict__ cell_start,
     const PPTile* __restrict__ tiles, int ntile, const int* __restrict__ dense_list,
-    const int* __restrict__ ndense_dev, int ratio_cs, int* __restrict__ ilist) {
+    const int* __restrict__ ndense_dev, int ratio_cs, float4* __restrict__ tpos,
+    uint32_t* __restrict__ tslot) {
     __shared__ int bins[kWaves][512];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ndense = *ndense_dev;
@@ -587,7 +595,13 @@ __global__ __launch_bounds__(256) void k_pp_subsort(
     __builtin_amdgcn_wave_barrier();
     for (int a = c0 + lane; a < c1; a += 64) {
         const int pos = atomicAdd(&b[subkey(a)], 1);
-        ilist[c0 + pos] = a;
+        tpos[c0 + pos] = pos4[a];
+        tslot[c0 + pos] = slot[a];
+    }
+    __threadfence();  // the staged cell is complete and visible to every lane
+    for (int a = c0 + lane; a < c1; a += 64) {
+        pos4[a] = tpos[a];
+        slot[a] = tslot[a];
     }
     __builtin_amdgcn_wave_barrier();  // bins are cleared for the next cell
     }
@@ -754,15 +768,33 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
         C2_TRY(g_pp.order_bin.reserve(P.cap));
         C2_TRY(g_pp.order_hist.reserve(((size_t)kOrderBins * kOrderBlocks + 1) * sizeof(int)));
     }
-    C2_TRY(g_pp.ilist.reserve(P.cap * sizeof(int)));
+    if (P.mode != CUBE2_PP_EXACT) {
+        C2_TRY(g_pp.sub_pos4.reserve(P.cap * sizeof(float4)));
+        C2_TRY(g_pp.sub_slot.reserve(P.cap * sizeof(uint32_t)));
+    }
     C2_TRY(g_pp.big.reserve(std::max<int64_t>(P.ncells_ext, 1) * sizeof(int)));
     C2_TRY(g_pp.dense.reserve((P.cap / 64 + 1) * sizeof(int)));
-    if (split) {
+    // Pairs once: each pair force component is below 10 / app^2 in magnitude
+    // (F_ra(r, a) <= 342 / (35 a^2), and 1/r^2 < 1/app^2 beyond app), and a
+    // particle has fewer than cap partners, so sums of (value * 2^k) stay
+    // below 2^62 for 2^k <= 2^62 / (10 / app^2 * cap); at 256^3 (app 0.06)
+    // k = 24, a resolution of 6e-8 against forces of order 1 to 1e3.
+    P.newton = false;
+    if (pp_newton_on() && pp_newton_mode(P.mode) && !learned && g.p.app > 0.0) {
+        const double bound = 10.0 / (g.p.app * g.p.app) * (double)P.cap;
+        const int k = std::min(40, (int)std::floor(62.0 - std::log2(bound)));
+        if (k >= 12) {
+            P.newton = true;
+            P.react_scale = std::ldexp(1.0f, k);
+        }
+    }
+    if (split || P.newton) {
         C2_TRY(g_pp.term.reserve(P.cap * 3 * sizeof(float)));
         const int64_t rows = (int64_t)g.ntile * g.nt * g.nt;
         C2_TRY(g_pp.rows_len.reserve((rows + 1) * sizeof(int)));
         C2_TRY(g_pp.rows_base.reserve(std::max<int64_t>(rows, 1) * sizeof(int64_t)));
     }
+    if (P.newton) C2_TRY(g_pp.react.reserve(P.cap * 3 * sizeof(unsigned long long)));
     P.all_active = true;
     for (int lv : P.iloc) P.all_active = P.all_active && lv >= 1 && lv <= 7 && apm3[lv - 1] > 0.0f;
     if (learned) {
@@ -778,12 +810,13 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
     C2_HIP(hipMemcpyAsync(g_pp.apm3.ptr, apm3, 7 * sizeof(float), hipMemcpyHostToDevice, s));
     C2_HIP(hipStreamSynchronize(s));  // host tables may go out of scope
     buffers_changed();                 // the device tables now describe this call
-    P.grid_pairs = pp_tiled_grid(P.mode, split);
+    P.grid_pairs = pp_tiled_grid(P.mode, split, P.newton);
     g_pp.valid = false;
     return CUBE2_OK;
 }
 
 static int pp_results_to_host(const PPPlan& P, hipStream_t s);
+static int pp_apply_kernels(const PPPlan& P, float* d_vp, const double* dv64, hipStream_t s);
 
 // Physical rows (t, k, j) of the image: each is one contiguous run of slots,
 // from the first slot of its first physical coarse cell to that of the
@@ -803,6 +836,44 @@ __global__ void k_phys_rows(const int64_t* __restrict__ nzero, int nt, int ncb, 
     const bool ok = b >= 0 && e >= b && e <= n_used;
     base[r] = ok ? b : 0;
     len[r] = ok ? (int)(e - b) : 0;
+}
+
+// Pairs-once pass: clears the fixed-point sums of the sorted particles.
+__global__ void k_react_zero(unsigned long long* __restrict__ react, const int* __restrict__ n_sorted,
+                             int cap, const int* __restrict__ err) {
+    const int64_t n = (*err & 4) ? 0 : 3 * (int64_t)min(*n_sorted, cap);
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x)
+        react[q] = 0;
+}
+
+// Pairs-once pass, after the pair kernel: each active sorted particle's
+// acceleration sum (fixed point / scale, rounded once to FP32) becomes its
+// velocity term a * m^2 * a_mid * dt / 6 at term[3 * slot], in the unsplit
+// kernel's FP32 operation order, and f2max = max |a m^2|^2.
+__global__ __launch_bounds__(256) void k_pp_newton_terms(
+    const float4* __restrict__ pos4, const uint32_t* __restrict__ slot,
+    const unsigned long long* __restrict__ react, const int* __restrict__ n_sorted, int cap,
+    const float* __restrict__ scal, double inv_scale, float* __restrict__ term, PPRed* red) {
+#pragma clang fp contract(off)
+    float f2 = 0.0f;
+    const int n = red->err == 0 ? min(*n_sorted, cap) : 0;
+    const float mass = scal[0], a_mid = scal[1], dt = scal[2];
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
+        if (pos4[a].w == 0.0f) continue;  // halo sources
+        float acc[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float v = (float)((double)(long long)react[3 * (size_t)a + c] * inv_scale);
+            acc[c] = v * mass * mass;
+        }
+        f2 = fmaxf(f2, acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2]);
+        const size_t ip = (size_t)slot[a] * 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) term[ip + c] = acc[c] * a_mid * dt / 6.0f;
+    }
+    const float mx[1] = {f2};
+    block_max_to<1>(&red->f2max_bits, mx);
 }
 
 // Velocity update of a split pass in slot order, when every particle of the
@@ -862,7 +933,7 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     const int64_t* nzp = nullptr;
     C2_TRY(compute_nzero(d_rhoc, d_idx, &nzp, s));
     const int64_t prow = (int64_t)g.ntile * g.nt * g.nt;
-    if (P.split && P.all_active && prow > 0) {
+    if ((P.split || P.newton) && P.all_active && prow > 0) {
         Stage st("pp_rows", (double)prow * 28);
         k_phys_rows<<<ceil_div(prow, 256), 256, 0, s>>>(nzp, g.nt, g.ncb, g.nte, prow, P.n_used,
                                                         g_pp.rows_len.as<int>(),
@@ -918,7 +989,6 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     A.slot = g_pp.slot.as<uint32_t>();
     A.cellid = g_pp.cellid.as<uint32_t>();
     A.cell_start = counts;
-    A.ilist = g_pp.ilist.as<int>();
     A.tiles = g_pp.tiles.as<PPTile>();
     A.ntile = P.ntile;
     A.n_sorted = n_sorted;
@@ -933,16 +1003,22 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
     A.tile_pairs = P.count_tiles ? ploc + g.ns3 : nullptr;
     A.learned_occ = P.learned ? P.lp.args.occupancy : 0;
     A.learned_range = P.learned ? P.lp.args.range : 0.0f;
+    A.ntt = g.ntt;
+    if (P.newton) {
+        A.term = g_pp.term.as<float>();
+        A.react = g_pp.react.as<unsigned long long>();
+        A.react_scale = P.react_scale;
+    }
     if (P.mode == CUBE2_PP_EXACT) {
         C2_TRY(launch_pp_pairs(A, P.mode, s));
     } else {
         int* flags = g_pp.chunk_flags.as<int>();
         {
             Stage st("pp_chunks", (double)P.n_used * 24);
-            k_chunk_flags<<<ceil_div(P.cap, 256), 256, 0, s>>>(A.pos4, A.cellid, A.cell_start,
-                                                                n_sorted, &red->err, (int)P.cap,
-                                                                pp_dense_lg(), flags,
-                                                                g_pp.dense.as<int>(), &red->ndense);
+            k_chunk_flags<<<ceil_div(P.cap, 256), 256, 0, s>>>(
+                g_pp.pos4.as<float4>(), A.cellid, A.cell_start, n_sorted, &red->err, (int)P.cap,
+                pp_dense_lg(), P.newton ? (1 << pp_direct_lg()) / 2 : 0, flags,
+                g_pp.dense.as<int>(), &red->ndense);
             C2_HIP(hipGetLastError());
             C2_TRY(exclusive_scan(flags, (int64_t)P.cap, g_pp.scan_tmp, s));
             k_chunk_emit<<<ceil_div(P.cap, 256), 256, 0, s>>>(flags, (int)P.cap, g_pp.chunks.as<int>());
@@ -952,8 +1028,9 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             Stage st("pp_subsort", (double)P.n_used * 24);
             // one wavefront per chunk (only first chunks of dense cells work)
             k_pp_subsort<<<(unsigned)std::max(1, P.grid_pairs), 256, 0, s>>>(
-                A.pos4, A.cell_start, A.tiles, P.ntile, g_pp.dense.as<int>(), &red->ndense,
-                g.p.ratio_cs, g_pp.ilist.as<int>());
+                g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>(), A.cell_start, A.tiles, P.ntile,
+                g_pp.dense.as<int>(), &red->ndense, g.p.ratio_cs, g_pp.sub_pos4.as<float4>(),
+                g_pp.sub_slot.as<uint32_t>());
             C2_HIP(hipGetLastError());
         }
         const int* chunks = g_pp.chunks.as<int>();
@@ -970,10 +1047,23 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             C2_HIP(hipGetLastError());
             chunks = g_pp.order.as<int>();
         }
+        if (P.newton) {
+            Stage st("pp_react_zero", (double)P.n_used * 24);
+            k_react_zero<<<2048, 256, 0, s>>>(A.react, n_sorted, (int)P.cap, &red->err);
+            C2_HIP(hipGetLastError());
+        }
         C2_TRY(launch_pp_tiled(A, chunks, flags + P.cap, P.grid_pairs, P.mode, s));
         if (P.learned) C2_TRY(learned_enqueue(P.lp, A, g_pp.tile_first.as<int>(), s));
+        if (P.newton) {
+            Stage st("pp_terms", (double)P.n_used * (16 + 24 + 4 + 12));
+            k_pp_newton_terms<<<4096, 256, 0, s>>>(A.pos4, A.slot, A.react, n_sorted, (int)P.cap,
+                                                    d_scal, 1.0 / (double)P.react_scale, A.term,
+                                                    red);
+            C2_HIP(hipGetLastError());
+        }
     }
     if (P.split) return CUBE2_OK;  // pp_enqueue_apply updates vp and copies the results
+    if (P.newton) C2_TRY(pp_apply_kernels(P, d_vp, dv64, s));
     return pp_results_to_host(P, s);
 }
 
@@ -992,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_pp_apply(const float4* __restrict__ pos
     const bool ok = red->err == 0;  // a failed call leaves the sorted arrays unread
     const int n = ok ? min(*n_sorted, cap) : 0;
     for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
-        if (!(pos4[a].w > 0.0f)) continue;  // halo sources are not updated
+        if (pos4[a].w == 0.0f) continue;  // halo sources are not updated
         const size_t ip = (size_t)slot[a] * 3;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -1007,11 +1097,7 @@ __global__ __launch_bounds__(256) void k_pp_apply(const float4* __restrict__ pos
     block_max_to<3>(red->vmax_bits, vm);
 }
 
-int pp_enqueue_apply(const PPPlan& P, float* d_vp, const double* dv64, hipStream_t s) {
-    if (!P.split) {
-        set_error("velocity apply of an unsplit PP pass");
-        return CUBE2_ERR_STATE;
-    }
+static int pp_apply_kernels(const PPPlan& P, float* d_vp, const double* dv64, hipStream_t s) {
     {
         const Geom& g = ctx().g;
         const int64_t prow = (int64_t)g.ntile * g.nt * g.nt;
@@ -1035,6 +1121,15 @@ int pp_enqueue_apply(const PPPlan& P, float* d_vp, const double* dv64, hipStream
             C2_HIP(hipGetLastError());
         }
     }
+    return CUBE2_OK;
+}
+
+int pp_enqueue_apply(const PPPlan& P, float* d_vp, const double* dv64, hipStream_t s) {
+    if (!P.split) {
+        set_error("velocity apply of an unsplit PP pass");
+        return CUBE2_ERR_STATE;
+    }
+    C2_TRY(pp_apply_kernels(P, d_vp, dv64, s));
     return pp_results_to_host(P, s);
 }
 
@@ -1107,7 +1202,6 @@ int pp_last(const PPPlan** P, PPArgs* A, const int** tile_first) {
     a.slot = g_pp.slot.as<uint32_t>();
     a.cellid = g_pp.cellid.as<uint32_t>();
     a.cell_start = g_pp.counts.as<int>();
-    a.ilist = g_pp.ilist.as<int>();
     a.tiles = g_pp.tiles.as<PPTile>();
     a.n_sorted = g_pp.counts.as<int>() + L.c_tot;
     a.ntile = L.ntile;
@@ -1149,7 +1243,7 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
     }
     PPPlan P;
     C2_TRY(pp_prepare(ires, ixyz3, apm3, ratio_sf, n_used, P, learned, learned));
-    P.grid_pairs = pp_tiled_grid(P.mode, false);  // nothing runs beside it here
+    P.grid_pairs = pp_tiled_grid(P.mode, false, P.newton);  // nothing runs beside it here
     const double* dv = (C.dv_state == 1) ? C.dv64.as<double>() : nullptr;
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, a_mid, dt, C.stream, &d_scal));
@@ -1183,8 +1277,20 @@ int pp_debug_celllist(int it, int* rcp, int* S, int* N, uint32_t* keys, uint32_t
         C2_HIP(hipMemcpy(cid.data(), g_pp.cellid.as<uint32_t>() + a0, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         for (int q = 0; q < n; q++) keys[q] = (uint32_t)(cid[q] - T.cell_off);
     }
-    if (slot && n > 0)
+    if (slot && n > 0) {
         C2_HIP(hipMemcpy(slot, g_pp.slot.as<uint32_t>() + a0, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        // dense cells are stored in sub-cell order; the reference lists a
+        // cell's particles in slot order
+        std::vector<uint32_t> cid(n);
+        C2_HIP(hipMemcpy(cid.data(), g_pp.cellid.as<uint32_t>() + a0, n * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+        for (int q = 0; q < n;) {
+            int e = q + 1;
+            while (e < n && cid[e] == cid[q]) e++;
+            std::sort(slot + q, slot + e);
+            q = e;
+        }
+    }
     if (cell_start) {
         C2_HIP(hipMemcpy(cell_start, g_pp.counts.as<int>() + T.cell_off, (ncell + 1) * sizeof(int),
                          hipMemcpyDeviceToHost));
@@ -1197,8 +1303,8 @@ void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
                       &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks,
-                      &p.ilist, &p.big, &p.dense, &p.term, &p.rows_len, &p.rows_base,
-                      &p.order, &p.order_bin, &p.order_hist})
+                      &p.sub_pos4, &p.sub_slot, &p.big, &p.dense, &p.term, &p.rows_len,
+                      &p.rows_base, &p.order, &p.order_bin, &p.order_hist, &p.react})
         b->release();
     p.valid = false;
 }

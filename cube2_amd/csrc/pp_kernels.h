@@ -47,7 +47,6 @@ struct PPArgs {
     const uint32_t* slot;      // sorted -> xp/vp slot
     const uint32_t* cellid;    // sorted -> global cell (tile offset + key)
     const int* cell_start;     // c_tot + 1 exclusive starts
-    const int* ilist;          // cells of >= 64 particles: their positions in sub-cell order
     const PPTile* tiles;
     const int* n_sorted;       // particles in the cell lists (device, from the scan)
     int ntile, cap, ratio_cs;  // cap: capacity of the sorted arrays
@@ -69,9 +68,19 @@ struct PPArgs {
     // with range learned_range are left to the network (0: none)
     int learned_occ;
     float learned_range;
+    // pairs-once pass (non-null): each pair of two active particles of one
+    // range is evaluated once and its force added to both, the per-particle
+    // sums kept as signed 64-bit fixed point (value * react_scale) at
+    // react[3 * sorted index]; k_pp_newton_terms turns them into term
+    unsigned long long* react;
+    float react_scale;
+    int ntt;  // coarse cells per subtile side
 };
 
 int launch_pp_pairs(const PPArgs& a, int mode, hipStream_t s);
+
+// Whether the pair kernel has a pairs-once variant for this PP mode.
+bool pp_newton_mode(int mode);
 
 // Whether PP calls count pairs per parent tile (off by default).
 bool& pp_count_tiles();
@@ -84,6 +93,15 @@ inline int pp_dense_lg() {
 
 // j candidates per batch of the pair kernel: 64 (built in) or 128.
 inline int pp_jbatch() { return ctx().tune[CUBE2_TUNE_PP_JBATCH] == 2 ? 128 : 64; }
+
+// Chunks of > 2^(v-1) particles sum directly (v = 2 built in).
+inline int pp_direct_lg() {
+    const int v = ctx().tune[CUBE2_TUNE_PP_DIRECT_LG];
+    return v ? v : 2;
+}
+
+// Whether split PP passes evaluate each pair of one range once (built in).
+inline bool pp_newton_on() { return ctx().tune[CUBE2_TUNE_PP_NEWTON] != 2; }
 
 // log2 of the estimated candidate pairs from which the pair kernel takes a
 // chunk ahead of the others (heaviest first), or 0: all in cell order

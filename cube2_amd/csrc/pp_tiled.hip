@@ -227,7 +227,7 @@ __device__ inline void pair_force(float4 e, const FraK& Ka, const FraK& Kr, floa
 // block barrier: the wavefronts of a block run independent trip counts).
 __device__ inline void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
 
-template <int kMode, int kJB = 64>
+template <int kMode, int kJB = 64, bool kNewton = false>
 struct WaveLDS {
     using Acc = typename std::conditional<kMode == kModePrecise || kMode == kModePlummer, double,
                                           float>::type;
@@ -236,18 +236,51 @@ struct WaveLDS {
     Acc acc[64][3];         // per-i force sums
     float4 jb[kJB + 64];    // current batch of j positions, then J far-away ones
     int rs[64], rp[64];     // row group: j = v + rs[k] for rp[k-1] <= v < rp[k]
+    // pairs-once pass: per batch slot, the sum over the chunk's i of the
+    // pair forces on i from that slot's j (j's reaction, negated; one
+    // 16-byte LDS store per slot)
+    float4 jr[kNewton ? kJB : 1];
 };
 
-template <int kMode, int kJB>
-__device__ inline void flush(WaveLDS<kMode, kJB>& L, int lane, int begin, int count, const FraK& Ka,
-                             const FraK& Kr) {
+// Sums over groups of 2^LG consecutive lanes (LG <= 6), left in each
+// group's last lane: DPP row shifts within the 16-lane rows (shift 1, 2, 4,
+// 8), then row broadcasts across them (lane 15 to the next row, lane 31 to
+// the upper half).  Every lane must be active.
+template <int K>
+__device__ inline float dpp_add(float x) {
+    constexpr int ctrl = K < 4 ? 0x110 + (1 << K) : (K == 4 ? 0x142 : 0x143);
+    constexpr int rows = K < 4 ? 0xf : (K == 4 ? 0xa : 0xc);
+    return x + __int_as_float(
+                   __builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, rows, 0xf, K < 4));
+}
+
+template <int LG>
+__device__ inline void group_sum3(float& x, float& y, float& z) {
+    if constexpr (LG > 0) {
+        group_sum3<LG - 1>(x, y, z);
+        x = dpp_add<LG - 1>(x);
+        y = dpp_add<LG - 1>(y);
+        z = dpp_add<LG - 1>(z);
+    }
+}
+
+// Signed fixed point (value * scale, rounded to nearest) of a pairs-once
+// partial sum, added to a particle's 64-bit accumulator: integer adds
+// commute, so the totals do not depend on the order of the atomics.
+__device__ inline void fixed_add(unsigned long long* dst, float v, float scale) {
+    if (v != 0.0f) atomicAdd(dst, (unsigned long long)__double2ll_rn((double)v * (double)scale));
+}
+
+template <int kMode, int kJB, bool kN>
+__device__ inline void flush(WaveLDS<kMode, kJB, kN>& L, int lane, int begin, int count,
+                             const FraK& Ka, const FraK& Kr) {
     if (lane < count) {
         const int s = (begin + lane) & (kQ - 1);
         const float4 e = L.q[s];
         const int ii = L.qi[s];
         float fx, fy, fz;
         pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
-        using Acc = typename WaveLDS<kMode, kJB>::Acc;
+        using Acc = typename WaveLDS<kMode, kJB, kN>::Acc;
         atomicAdd(&L.acc[ii][0], (Acc)fx);
         atomicAdd(&L.acc[ii][1], (Acc)fy);
         atomicAdd(&L.acc[ii][2], (Acc)fz);
@@ -278,16 +311,17 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
 // ms; budgets for 5 and 7 wavefronts 89.2 and 86.0 ms; 4 blocks per CU
 // instead of 6: 105 ms).  The REF variant spills a few registers outside its
 // inner loops.
-template <int kMode, int kJB>
+template <int kMode, int kJB, bool kNewton>
 __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
                                                   const int* __restrict__ nchunks_dev,
                                                   int direct_lg, int dense_lg) {
     static_assert(kJB == 64 || kJB == 128, "j batch of 64 or 128");
     constexpr int kH = kJB / 64;  // fetched float4 per lane per batch
-    __shared__ WaveLDS<kMode, kJB> lds[4];
+    __shared__ WaveLDS<kMode, kJB, kNewton> lds[4];
     const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-    WaveLDS<kMode, kJB>& L = lds[wave];
+    WaveLDS<kMode, kJB, kNewton>& L = lds[wave];
     const int nchunks = uni(*nchunks_dev);
+
     float f2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f;
     unsigned long long pc = 0, pa = 0;
     for (int ck = blockIdx.x * 4 + wave; ck < nchunks; ck += gridDim.x * 4) {
@@ -295,8 +329,9 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const uint32_t cid = (uint32_t)uni((int)A.cellid[a0]);
         const int cend = uni(A.cell_start[cid + 1]);
         const int cbeg = uni(A.cell_start[cid]);
-        // cells of >= 64 particles are visited in sub-cell order (ilist), so
-        // that a chunk is spatially compact
+        // cells of >= 64 particles are stored in sub-cell order
+        // (k_pp_subsort) and taken 2^dense_lg at a time, so that a chunk is
+        // spatially compact
         const bool grouped = cend - cbeg >= 64;
         const int ni = min(grouped ? 1 << dense_lg : 64, cend - a0);
         int lgI = 0;
@@ -304,7 +339,7 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const int I = 1 << lgI, J = 64 >> lgI;
         const int islot = lane & (I - 1), jslot = lane >> lgI;
         const bool vi = islot < ni;
-        const int ia = grouped ? A.ilist[a0 + (vi ? islot : 0)] : a0 + (vi ? islot : 0);
+        const int ia = a0 + (vi ? islot : 0);
         const float4 p = A.pos4[ia];
         const int ti = uni(find_tile2(A.tiles, A.ntile, cid));
         const PPTile T = A.tiles[ti];
@@ -312,7 +347,9 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         const long long key = (long long)cid - T.cell_off;
         const int S = T.S;
         const int x = (int)(key % S), y = (int)((key / S) % S), z = (int)(key / ((long long)S * S));
-        const float range = uni(p.w);  // one subtile per PP cell: same range for the chunk
+        // one subtile per PP cell: same range for the chunk (w < 0 marks the
+        // pairs-once pass's small cells)
+        const float range = fabsf(uni(p.w));
         // learned kick: the network stands in for the sparse level-2 cells
         if (A.learned_occ && range == A.learned_range && cend - cbeg < A.learned_occ) continue;
         int R;
@@ -327,8 +364,30 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         wave_lds_sync();
         const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int qhead = 0, qtail = 0;
-        const bool direct = lgI >= direct_lg;
-        using Acc = typename WaveLDS<kMode, kJB>::Acc;
+        // pairs once: chunks of cells of more than 2^(direct_lg-1) particles
+        // (each such cell's chunks alike, k_chunk_flags marks the others)
+        const bool direct = kNewton ? cend - cbeg > (1 << direct_lg) / 2 : lgI >= direct_lg;
+        const bool nwc = kNewton && direct;
+        using Acc = typename WaveLDS<kMode, kJB, kNewton>::Acc;
+        // Pairs once (kNewton): a j of the chunk's own range in a cell of
+        // more than 2^(direct_lg-1) particles is an active particle of the
+        // same tile and level, an i of a chunk of this kind whose stencil
+        // holds this chunk's cell; of the two, the one with the lower sorted
+        // index evaluates the pair and adds the force to both (the pair force
+        // is exactly antisymmetric: RN(a - b) = -RN(b - a)).  Other j (halo
+        // copies, other levels, small cells) act on i alone, and the chunks
+        // of small cells evaluate all their pairs themselves.  A chunk whose
+        // stencil lies inside its own subtile meets only j of its own range,
+        // so it skips the rows and cells before its own.
+        bool interior = false;
+        if (nwc) {
+            const int cps = A.ntt * T.rcp, span = S - 2 * T.rcp;
+            auto inside = [&](int c) {
+                const int lo = c - R - T.rcp, hi = c + R - T.rcp;
+                return lo >= 0 && hi < span && lo / cps == hi / cps;
+            };
+            interior = inside(x) && inside(y) && inside(z);
+        }
         Acc dax = 0, day = 0, daz = 0;
         unsigned npa = 0;
         // direct chunks: bounding box of the chunk's i; a j farther than the
@@ -361,7 +420,7 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         // the group's runs form one virtual list of M candidates, fetched 64
         // at a time with one coalesced load, prefetched a batch ahead.
         for (int rg = 0; rg < nrows; rg += 64) {
-            int js = 0, cnt = 0;
+            int js = 0, cnt = 0, cfull = 0;
             const int r = rg + lane;
             if (r < nrows) {
                 const int zz = z + r / W - R, yy = y + r % W - R;
@@ -369,6 +428,26 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                     const long long row = T.cell_off + ((long long)zz * S + yy) * S;
                     js = A.cell_start[row + xlo];
                     cnt = A.cell_start[row + xhi + 1] - js;
+                    cfull = cnt;
+                    if (kNewton && interior && zz * S + yy <= z * S + y) {
+                        // rows before the chunk's: none; its own row: from its
+                        // first i -- unless a small cell lies there, whose j
+                        // act on this chunk's i only from here
+                        const bool own = zz * S + yy == z * S + y;
+                        const int xe = own ? x : xhi + 1, small = (1 << direct_lg) / 2;
+                        bool has_small = false;
+                        int c0 = js;
+                        for (int xc = xlo; xc < xe; xc++) {
+                            const int c1 = A.cell_start[row + xc + 1];
+                            has_small |= c1 - c0 > 0 && c1 - c0 <= small;
+                            c0 = c1;
+                        }
+                        if (!has_small) {
+                            const int j0 = own ? a0 : js + cnt;
+                            cnt -= j0 - js;
+                            js = j0;
+                        }
+                    }
                 }
             }
             int inc = cnt;
@@ -381,7 +460,14 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
             L.rs[lane] = js - (inc - cnt);
             L.rp[lane] = inc;
             wave_lds_sync();
-            if (vi && jslot == 0) pc += (unsigned long long)M;
+            if (kNewton && interior) {  // the candidates counted: the whole stencil
+                int sf = cfull;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) sf += __shfl_xor(sf, o, 64);
+                if (vi && jslot == 0) pc += (unsigned long long)sf;
+            } else if (vi && jslot == 0) {
+                pc += (unsigned long long)M;
+            }
             // a lane's candidates v = vb + lane only grow, so its row (the
             // first with rp > v) is found by walking forward from the last
             // one: rp of the rows past the group is M > v, so the walk stops
@@ -392,7 +478,11 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                 float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (v < M) {
                     while (L.rp[frow] <= v) frow++;
-                    q4 = A.pos4[v + L.rs[frow]];
+                    const int jx = v + L.rs[frow];
+                    q4 = A.pos4[jx];
+                    // pairs once: w = the sorted index of a j of the chunk's
+                    // range, INT_MAX for a j that acts on i alone
+                    if (kNewton) q4.w = __int_as_float(q4.w == range ? jx : INT_MAX);
                 }
                 return q4;
             };
@@ -410,8 +500,11 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                         const float ex = fmaxf(fmaxf(bx0 - qh.x, qh.x - bx1), 0.0f);
                         const float ey = fmaxf(fmaxf(by0 - qh.y, qh.y - by1), 0.0f);
                         const float ez = fmaxf(fmaxf(bz0 - qh.z, qh.z - bz1), 0.0f);
-                        const bool keep =
-                            vb + 64 * h + lane < M && ex * ex + ey * ey + ez * ez < pp2m;
+                        // (pairs once: a j of the chunk's range before its first
+                        // i evaluates the pair itself)
+                        const bool keep = vb + 64 * h + lane < M &&
+                                          ex * ex + ey * ey + ez * ez < pp2m &&
+                                          (!nwc || __float_as_int(qh.w) >= a0);
                         const uint64_t km = __ballot(keep);
                         if (keep) L.jb[nb + __popcll(km & lt)] = qh;
                         nb += __popcll(km);
@@ -433,7 +526,7 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                 // i slot past the chunk tests from a far-away i)
                 auto test = [&](int t, float4& e) {
 #pragma clang fp contract(off)
-                    const float4 qj = L.jb[t + jslot];
+                    const float4 qj = L.jb[t];
                     e.x = qj.x - pix;
                     e.y = qj.y - piy;
                     e.z = qj.z - piz;
@@ -443,12 +536,67 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                         e.w = e.x * e.x + e.y * e.y + e.z * e.z;
                     return (e.w > 0.0f) & (e.w < pp2);
                 };
+                if (nwc) {
+                    // Lane (i, s) takes slot t + s at step t, as below; the I
+                    // lanes of slot t + s then hold the forces on all the
+                    // chunk's i from that j, and their sum (a DPP reduction,
+                    // no LDS traffic) is j's reaction for the batch, which
+                    // the group's last lane stores.  (LDS float atomics cost
+                    // ~190 cycles per wavefront instruction on gfx950.)
+                    // (one loop per chunk size: the reduction's DPP steps
+                    // and the group shape are then constants)
+                    auto pairs = [&](auto lg_c) {
+                        constexpr int LG = decltype(lg_c)::value;
+                        constexpr int IC = 1 << LG, JC = 64 >> LG;
+                        for (int t = 0; t < nb; t += JC) {
+                            float4 e;
+                            const int jw = __float_as_int(L.jb[t + jslot].w);
+                            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                            if (test(t + jslot, e) & (jw > ia)) {
+#pragma clang fp contract(off)
+                                pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
+                                dax += (Acc)fx;
+                                day += (Acc)fy;
+                                daz += (Acc)fz;
+                                npa += jw != INT_MAX ? 2u : 1u;
+                            }
+                            group_sum3<LG>(fx, fy, fz);
+                            if ((lane & (IC - 1)) == IC - 1 && t + jslot < nb)
+                                L.jr[t + jslot] = make_float4(fx, fy, fz, 0.0f);
+                        }
+                    };
+                    switch (lgI) {
+                        case 0: pairs(std::integral_constant<int, 0>{}); break;
+                        case 1: pairs(std::integral_constant<int, 1>{}); break;
+                        case 2: pairs(std::integral_constant<int, 2>{}); break;
+                        case 3: pairs(std::integral_constant<int, 3>{}); break;
+                        case 4: pairs(std::integral_constant<int, 4>{}); break;
+                        case 5: pairs(std::integral_constant<int, 5>{}); break;
+                        default: pairs(std::integral_constant<int, 6>{}); break;
+                    }
+                    wave_lds_sync();
+                    // the reactions of the batch's j of the chunk's range
+#pragma unroll
+                    for (int h = 0; h < kH; h++) {
+                        const int l = 64 * h + lane;
+                        if (l < nb) {
+                            const int jw = __float_as_int(L.jb[l].w);
+                            if (jw != INT_MAX) {
+                                const float4 r = L.jr[l];
+                                unsigned long long* d = A.react + 3 * (size_t)jw;
+                                fixed_add(d, -r.x, A.react_scale);
+                                fixed_add(d + 1, -r.y, A.react_scale);
+                                fixed_add(d + 2, -r.z, A.react_scale);
+                            }
+                        }
+                    }
+                } else
                 // two loops on the chunk-uniform branch, so that the direct
                 // loop carries only its accumulators
                 if (direct) {
                     for (int t = 0; t < nb; t += J) {
                         float4 e;
-                        if (test(t, e)) {
+                        if (test(t + jslot, e)) {
 #pragma clang fp contract(off)
                             float fx, fy, fz;
                             pair_force<kMode>(e, Ka, Kr, fx, fy, fz);
@@ -461,7 +609,7 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
                 } else {
                     for (int t = 0; t < nb; t += J) {
                         float4 e;
-                        const bool ok = test(t, e);
+                        const bool ok = test(t + jslot, e);
                         const uint64_t m = __ballot(ok);
                         if (m) {
                             if (ok) {
@@ -492,7 +640,14 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
         }
         if (qtail > qhead) flush<kMode>(L, lane, qhead, qtail - qhead, Ka, Kr);
         if (lane == 0) pa += (unsigned long long)qtail;
-        if (vi && jslot == 0) {
+        if (kNewton && vi && jslot == 0) {
+            pc -= 1;  // the particle itself
+            // the i's own sums join its reactions; k_pp_newton_terms forms
+            // the velocity term once every chunk has added to it
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                fixed_add(A.react + 3 * (size_t)ia + c, (float)L.acc[islot][c], A.react_scale);
+        } else if (vi && jslot == 0) {
 #pragma clang fp contract(off)
             pc -= 1;  // the particle itself
             const int a = ia;
@@ -538,29 +693,41 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
 }
 
 template <int kJB>
-static hipError_t pp_occupancy(int mode, int* per) {
+static hipError_t pp_occupancy(int mode, bool newton, int* per) {
+    if (newton && mode == CUBE2_PP_REF)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeRef, kJB, true>,
+                                                            256, 0);
+    if (newton && mode == CUBE2_PP_FAST)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeFast, kJB, true>,
+                                                            256, 0);
     if (mode == CUBE2_PP_REF)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeRef, kJB>, 256, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeRef, kJB, false>,
+                                                            256, 0);
     if (mode == CUBE2_PP_FAST)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeFast, kJB>, 256, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeFast, kJB, false>,
+                                                            256, 0);
     if (mode == CUBE2_PP_PLUMMER)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModePlummer, kJB>, 256,
-                                                            0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModePrecise, kJB>, 256, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            per, k_pp_tiled<kModePlummer, kJB, false>, 256, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModePrecise, kJB, false>,
+                                                        256, 0);
 }
+
+bool pp_newton_mode(int mode) { return mode == CUBE2_PP_REF || mode == CUBE2_PP_FAST; }
 
 // Blocks of the persistent pair kernel: the resident maximum over all CUs
 // (occupancy query, cached per mode).
 // overlap: the kernel runs beside PM3 (CUBE2_TUNE_PP_OVERLAP_BLOCKS per CU,
 // at most the occupancy maximum).
-int pp_tiled_grid(int mode, bool overlap) {
-    static int cache[16] = {};
+int pp_tiled_grid(int mode, bool overlap, bool newton) {
+    static int cache[32] = {};
     const int jb = pp_jbatch();
-    const int m = (mode & 7) + (jb == 128 ? 8 : 0);
+    newton = newton && pp_newton_mode(mode);
+    const int m = (mode & 7) + (jb == 128 ? 8 : 0) + (newton ? 16 : 0);
     const int forced = ctx().tune[CUBE2_TUNE_PP_BLOCKS_PER_CU];
     int dev = 0, ncu = 0, per = 0;
     if (overlap) {
-        const int full = pp_tiled_grid(mode, false);
+        const int full = pp_tiled_grid(mode, false, newton);
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             ncu = 256;
@@ -580,7 +747,8 @@ int pp_tiled_grid(int mode, bool overlap) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         ncu = 256;
-    const hipError_t e = jb == 128 ? pp_occupancy<128>(mode, &per) : pp_occupancy<64>(mode, &per);
+    const hipError_t e =
+        jb == 128 ? pp_occupancy<128>(mode, newton, &per) : pp_occupancy<64>(mode, newton, &per);
     if (e != hipSuccess || per < 1) per = 2;
     cache[m] = std::max(1, ncu) * per;
     return cache[m];
@@ -590,16 +758,19 @@ template <int kJB>
 static void launch_pp_tiled_jb(const PPArgs& a, const int* chunks, const int* nchunks,
                                unsigned blocks, int mode, int direct_lg, int dense_lg,
                                hipStream_t s) {
-    if (mode == CUBE2_PP_REF)
-        k_pp_tiled<kModeRef, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
+    const int dl = direct_lg, gl = dense_lg;
+    if (a.react && mode == CUBE2_PP_REF)
+        k_pp_tiled<kModeRef, kJB, true><<<blocks, 256, 0, s>>>(a, chunks, nchunks, dl, gl);
+    else if (a.react && mode == CUBE2_PP_FAST)
+        k_pp_tiled<kModeFast, kJB, true><<<blocks, 256, 0, s>>>(a, chunks, nchunks, dl, gl);
+    else if (mode == CUBE2_PP_REF)
+        k_pp_tiled<kModeRef, kJB, false><<<blocks, 256, 0, s>>>(a, chunks, nchunks, dl, gl);
     else if (mode == CUBE2_PP_FAST)
-        k_pp_tiled<kModeFast, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg, dense_lg);
+        k_pp_tiled<kModeFast, kJB, false><<<blocks, 256, 0, s>>>(a, chunks, nchunks, dl, gl);
     else if (mode == CUBE2_PP_PLUMMER)
-        k_pp_tiled<kModePlummer, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg,
-                                                             dense_lg);
+        k_pp_tiled<kModePlummer, kJB, false><<<blocks, 256, 0, s>>>(a, chunks, nchunks, dl, gl);
     else
-        k_pp_tiled<kModePrecise, kJB><<<blocks, 256, 0, s>>>(a, chunks, nchunks, direct_lg,
-                                                             dense_lg);
+        k_pp_tiled<kModePrecise, kJB, false><<<blocks, 256, 0, s>>>(a, chunks, nchunks, dl, gl);
 }
 
 int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int grid, int mode,
@@ -611,7 +782,7 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
     Stage st(name);
     const unsigned blocks = (unsigned)std::max(1, grid);
     // chunks of more than 2 particles sum directly unless tuned otherwise
-    const int direct_lg = ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] ? ctx().tune[CUBE2_TUNE_PP_DIRECT_LG] : 2;
+    const int direct_lg = pp_direct_lg();
     const int dense_lg = pp_dense_lg();
     if (pp_jbatch() == 128)
         launch_pp_tiled_jb<128>(a, chunks, nchunks, blocks, mode, direct_lg, dense_lg, s);

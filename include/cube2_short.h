@@ -137,7 +137,12 @@ int cube2_set_pp_mode(int mode);
  *                     candidate pairs or more first, heaviest first, then
  *                     the rest in cell order;
  *   PP_JBATCH         1: the pair kernel fetches and tests j candidates 64
- *                     at a time (built-in); 2: 128 at a time.
+ *                     at a time (built-in); 2: 128 at a time;
+ *   PP_NEWTON         1: the split pair pass of the kick (REF, FAST)
+ *                     evaluates each pair of two active particles of one
+ *                     range once and adds its force to both, summing in
+ *                     64-bit fixed point (built-in; needs app > 0); 2: every
+ *                     ordered pair is evaluated by its i.
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
@@ -149,7 +154,8 @@ enum {
     CUBE2_TUNE_PP_OVERLAP_BLOCKS = 6,
     CUBE2_TUNE_PP_ORDER = 7,
     CUBE2_TUNE_PP_JBATCH = 8,
-    CUBE2_TUNE_COUNT = 9
+    CUBE2_TUNE_PP_NEWTON = 9,
+    CUBE2_TUNE_COUNT = 10
 };
 int cube2_set_tuning(int key, int value);
 int cube2_get_tuning(int key, int* value); /* the value in use (built-in choice resolved) */

@@ -168,4 +168,6 @@ def test_tuning_api_validates_and_round_trips():
     assert L.cube2_set_tuning(_lib.TUNE_PP_ORDER, 31) == _lib.ERR_ARG
     assert _lib.get_tuning(_lib.TUNE_PP_JBATCH) == 1
     assert L.cube2_set_tuning(_lib.TUNE_PP_JBATCH, 3) == _lib.ERR_ARG
+    assert _lib.get_tuning(_lib.TUNE_PP_NEWTON) == 1
+    assert L.cube2_set_tuning(_lib.TUNE_PP_NEWTON, 3) == _lib.ERR_ARG
     assert L.cube2_set_tuning(_lib.TUNE_PP_DENSE_LG, 2) == _lib.ERR_ARG

@@ -206,7 +206,7 @@ def test_tunables_keep_results_and_rerecord(lib, mixed_layout):
     base = k.vp_host()
     settings = [(lib.TUNE_PP_BLOCKS_PER_CU, 3), (lib.TUNE_PP_DIRECT_LG, 6),
                 (lib.TUNE_FFT_MIN_LGC, 4), (lib.TUNE_INTERP_SEG, 2), (lib.TUNE_PP_DENSE_LG, 6),
-                (lib.TUNE_PP_ORDER, 18), (lib.TUNE_PP_JBATCH, 2)]
+                (lib.TUNE_PP_ORDER, 18), (lib.TUNE_PP_JBATCH, 2), (lib.TUNE_PP_NEWTON, 2)]
     try:
         for key, v in settings:
             lib.set_tuning(key, v)
