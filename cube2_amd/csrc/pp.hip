@@ -41,7 +41,9 @@ int pp_tiled_grid(int mode, bool overlap, bool newton);
 struct PPState {
     DevBuf nzero, tiles, ires_loc, apm3, counts, scan_tmp, pos4, slot, cellid, red, tile_first;
     DevBuf chunk_flags, chunks, big, dense;
-    DevBuf sub_pos4, sub_slot;  // staging of the dense cells' sub-cell reorder
+    // the scatter's output when the dense cells are then reordered (all
+    // modes but EXACT): k_chunk_flags and k_pp_subsort copy it to pos4 / slot
+    DevBuf raw_pos4, raw_slot;
     DevBuf order, order_bin, order_hist;  // chunks heaviest first (k_chunk_cost/order)
     DevBuf term;  // split pass: per slot velocity term [n_used][3] (active slots)
     DevBuf rows_len, rows_base;  // split pass: physical rows' slot counts (scanned) and first slots
@@ -388,27 +390,39 @@ __global__ void k_pp_ptotal(const int* __restrict__ rhoc, int nt, int ncb, int n
 // where a chunk starts.
 // The sorted count n comes from the scan on the device; flags[a] = 0 for
 // n <= a < cap, and for every a if the call has failed.
-// Pairs-once pass (small > 0): the particles of active cells of at most
-// `small` particles, whose chunks evaluate their pairs one-sidedly, are
+// Reads the scatter's output (rpos, rslot) and copies every particle not in
+// an active cell of >= 64 particles to pos4 / slot (k_pp_subsort places the
+// others).  Pairs-once pass (small > 0): the particles of active cells of at
+// most `small` particles, whose chunks evaluate their pairs one-sidedly, are
 // marked by a negative range (w = -range); every reader of w then takes |w|
 // as the range and w != 0 as active.
-__global__ void k_chunk_flags(float4* __restrict__ pos4, const uint32_t* __restrict__ cellid,
+__global__ void k_chunk_flags(const float4* __restrict__ rpos, const uint32_t* __restrict__ rslot,
+                              const uint32_t* __restrict__ cellid,
                               const int* __restrict__ cell_start, const int* __restrict__ n_sorted,
                               const int* __restrict__ err, int cap, int dense_lg, int small,
                               int* __restrict__ flags, int* __restrict__ dense_list,
-                              int* __restrict__ ndense) {
+                              int* __restrict__ ndense, float4* __restrict__ pos4,
+                              uint32_t* __restrict__ slot) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= cap) return;
     const int n = (*err & 4) ? 0 : *n_sorted;
     int f = 0;
-    if (a < n && pos4[a].w > 0.0f) {
-        const uint32_t c = cellid[a];
-        const int c0 = cell_start[c], cn = cell_start[c + 1] - c0;
-        const bool dense = cn >= 64;
-        const int mask = dense ? (1 << dense_lg) - 1 : 63;
-        f = ((a - c0) & mask) == 0;
-        if (dense && a == c0) dense_list[atomicAdd(ndense, 1)] = (int)c;
-        if (cn <= small) pos4[a].w = -pos4[a].w;
+    if (a < n) {
+        float4 p = rpos[a];
+        bool dense = false;
+        if (p.w > 0.0f) {
+            const uint32_t c = cellid[a];
+            const int c0 = cell_start[c], cn = cell_start[c + 1] - c0;
+            dense = cn >= 64;
+            const int mask = dense ? (1 << dense_lg) - 1 : 63;
+            f = ((a - c0) & mask) == 0;
+            if (dense && a == c0) dense_list[atomicAdd(ndense, 1)] = (int)c;
+            if (cn <= small) p.w = -p.w;
+        }
+        if (!dense) {
+            pos4[a] = p;
+            slot[a] = rslot[a];
+        }
     }
     flags[a] = f;
 }
@@ -531,15 +545,16 @@ __global__ __launch_bounds__(kOrderBins) void k_chunk_order(const int* __restric
 __device__ inline int spread3(int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4); }
 
 // One wavefront per PP cell of at least 64 particles (listed by
-// k_chunk_flags): the cell's particles (pos4, slot) are reordered in place by
-// an 8x8x8 Morton sub-cell key (counting sort in LDS, staged through tpos /
-// tslot), so that consecutive chunks of a dense cell are spatially compact
-// and a chunk is a range of sorted indices.  Cells keep their ranges.
+// k_chunk_flags): the cell's particles are copied from the scatter's output
+// (rpos, rslot) to pos4 / slot ordered by an 8x8x8 Morton sub-cell key
+// (counting sort in LDS; within a sub-cell, the cell order), so that
+// consecutive chunks of a dense cell are spatially compact and a chunk is a
+// range of sorted indices.  Cells keep their ranges.
 __global__ __launch_bounds__(256) void k_pp_subsort(
-    float4* __restrict__ pos4, uint32_t* __restrict__ slot, const int* __restrict__ cell_start,
-    const PPTile* __restrict__ tiles, int ntile, const int* __restrict__ dense_list,
-    const int* __restrict__ ndense_dev, int ratio_cs, float4* __restrict__ tpos,
-    uint32_t* __restrict__ tslot) {
+    const float4* __restrict__ rpos, const uint32_t* __restrict__ rslot,
+    const int* __restrict__ cell_start, const PPTile* __restrict__ tiles, int ntile,
+    const int* __restrict__ dense_list, const int* __restrict__ ndense_dev, int ratio_cs,
+    float4* __restrict__ pos4, uint32_t* __restrict__ slot) {
     __shared__ int bins[kWaves][512];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ndense = *ndense_dev;
@@ -565,7 +580,7 @@ __global__ __launch_bounds__(256) void k_pp_subsort(
     for (int q = lane; q < 512; q += 64) b[q] = 0;
     __builtin_amdgcn_wave_barrier();
     auto subkey = [&](int a) {
-        const float4 v = pos4[a];
+        const float4 v = rpos[a];
         const int ix = min(max((int)((v.x - sx) * inv), 0), 7);
         const int iy = min(max((int)((v.y - sy) * inv), 0), 7);
         const int iz = min(max((int)((v.z - sz) * inv), 0), 7);
@@ -595,13 +610,8 @@ __global__ __launch_bounds__(256) void k_pp_subsort(
     __builtin_amdgcn_wave_barrier();
     for (int a = c0 + lane; a < c1; a += 64) {
         const int pos = atomicAdd(&b[subkey(a)], 1);
-        tpos[c0 + pos] = pos4[a];
-        tslot[c0 + pos] = slot[a];
-    }
-    __threadfence();  // the staged cell is complete and visible to every lane
-    for (int a = c0 + lane; a < c1; a += 64) {
-        pos4[a] = tpos[a];
-        slot[a] = tslot[a];
+        pos4[c0 + pos] = rpos[a];
+        slot[c0 + pos] = rslot[a];
     }
     __builtin_amdgcn_wave_barrier();  // bins are cleared for the next cell
     }
@@ -769,8 +779,8 @@ int pp_prepare(const int* ires, const int* ixyz3, const float* apm3, const int* 
         C2_TRY(g_pp.order_hist.reserve(((size_t)kOrderBins * kOrderBlocks + 1) * sizeof(int)));
     }
     if (P.mode != CUBE2_PP_EXACT) {
-        C2_TRY(g_pp.sub_pos4.reserve(P.cap * sizeof(float4)));
-        C2_TRY(g_pp.sub_slot.reserve(P.cap * sizeof(uint32_t)));
+        C2_TRY(g_pp.raw_pos4.reserve(P.cap * sizeof(float4)));
+        C2_TRY(g_pp.raw_slot.reserve(P.cap * sizeof(uint32_t)));
     }
     C2_TRY(g_pp.big.reserve(std::max<int64_t>(P.ncells_ext, 1) * sizeof(int)));
     C2_TRY(g_pp.dense.reserve((P.cap / 64 + 1) * sizeof(int)));
@@ -958,19 +968,23 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
         k_pp_check_total<<<1, 64, 0, s>>>(n_sorted, (int64_t)P.cap, red);
         C2_HIP(hipGetLastError());
     }
+    // the scatter writes pos4 / slot directly in the EXACT mode; otherwise
+    // k_chunk_flags and k_pp_subsort copy its output there
+    const bool reorder = P.mode != CUBE2_PP_EXACT;
+    float4* spos = reorder ? g_pp.raw_pos4.as<float4>() : g_pp.pos4.as<float4>();
+    uint32_t* sslot = reorder ? g_pp.raw_slot.as<uint32_t>() : g_pp.slot.as<uint32_t>();
     if (P.ncells_ext > 0) {
         Stage st("pp_scatter", (double)P.n_used * (6 + 24));
         k_pp_scatter<<<ceil_div(P.ncells_ext, kWaves), 256, 0, s>>>(
             d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g_pp.ires_loc.as<int>(),
             g_pp.apm3.as<float>(), g.nt, g.ncb, g.nte, g.p.nnt, g.p.nns, g.ntt, g.p.ratio_cs,
-            P.ncells_ext, P.n_used, counts, &red->err, g_pp.pos4.as<float4>(),
-            g_pp.slot.as<uint32_t>(), g_pp.cellid.as<uint32_t>());
+            P.ncells_ext, P.n_used, counts, &red->err, spos, sslot, g_pp.cellid.as<uint32_t>());
         C2_HIP(hipGetLastError());
         k_pp_scatter_big<<<kBigBlocks, 64 * kBigWaves, 0, s>>>(
             d_rhoc, nzp, d_xp, g_pp.tiles.as<PPTile>(), g_pp.ires_loc.as<int>(),
             g_pp.apm3.as<float>(), g.nt, g.ncb, g.nte, g.p.nnt, g.p.nns, g.ntt, g.p.ratio_cs,
-            g_pp.big.as<int>(), &red->nbig, counts, &red->err, g_pp.pos4.as<float4>(),
-            g_pp.slot.as<uint32_t>(), g_pp.cellid.as<uint32_t>());
+            g_pp.big.as<int>(), &red->nbig, counts, &red->err, spos, sslot,
+            g_pp.cellid.as<uint32_t>());
         C2_HIP(hipGetLastError());
     }
     {
@@ -1016,9 +1030,10 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
         {
             Stage st("pp_chunks", (double)P.n_used * 24);
             k_chunk_flags<<<ceil_div(P.cap, 256), 256, 0, s>>>(
-                g_pp.pos4.as<float4>(), A.cellid, A.cell_start, n_sorted, &red->err, (int)P.cap,
-                pp_dense_lg(), P.newton ? (1 << pp_direct_lg()) / 2 : 0, flags,
-                g_pp.dense.as<int>(), &red->ndense);
+                g_pp.raw_pos4.as<float4>(), g_pp.raw_slot.as<uint32_t>(), A.cellid, A.cell_start,
+                n_sorted, &red->err, (int)P.cap, pp_dense_lg(),
+                P.newton ? (1 << pp_direct_lg()) / 2 : 0, flags, g_pp.dense.as<int>(),
+                &red->ndense, g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>());
             C2_HIP(hipGetLastError());
             C2_TRY(exclusive_scan(flags, (int64_t)P.cap, g_pp.scan_tmp, s));
             k_chunk_emit<<<ceil_div(P.cap, 256), 256, 0, s>>>(flags, (int)P.cap, g_pp.chunks.as<int>());
@@ -1028,9 +1043,9 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             Stage st("pp_subsort", (double)P.n_used * 24);
             // one wavefront per chunk (only first chunks of dense cells work)
             k_pp_subsort<<<(unsigned)std::max(1, P.grid_pairs), 256, 0, s>>>(
-                g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>(), A.cell_start, A.tiles, P.ntile,
-                g_pp.dense.as<int>(), &red->ndense, g.p.ratio_cs, g_pp.sub_pos4.as<float4>(),
-                g_pp.sub_slot.as<uint32_t>());
+                g_pp.raw_pos4.as<float4>(), g_pp.raw_slot.as<uint32_t>(), A.cell_start, A.tiles,
+                P.ntile, g_pp.dense.as<int>(), &red->ndense, g.p.ratio_cs, g_pp.pos4.as<float4>(),
+                g_pp.slot.as<uint32_t>());
             C2_HIP(hipGetLastError());
         }
         const int* chunks = g_pp.chunks.as<int>();
@@ -1303,7 +1318,7 @@ void pp_release() {
     PPState& p = g_pp;
     for (DevBuf* b : {&p.nzero, &p.tiles, &p.ires_loc, &p.apm3, &p.counts, &p.scan_tmp, &p.pos4,
                       &p.slot, &p.cellid, &p.red, &p.tile_first, &p.chunk_flags, &p.chunks,
-                      &p.sub_pos4, &p.sub_slot, &p.big, &p.dense, &p.term, &p.rows_len,
+                      &p.raw_pos4, &p.raw_slot, &p.big, &p.dense, &p.term, &p.rows_len,
                       &p.rows_base, &p.order, &p.order_bin, &p.order_hist, &p.react})
         b->release();
     p.valid = false;
