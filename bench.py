@@ -213,12 +213,12 @@ def bench_c2(args) -> int:
     for L in levels:
         k.green(L)
     mass, amid, dt = 1.0, 0.02, 1e-3
+    tuning = apply_tune_args(args)
     t1 = time.perf_counter()
     res = k.kick(mass, amid, dt)
     cold_ms = 1e3 * (time.perf_counter() - t1)
     for _ in range(args.warmup):
         k.kick(mass, amid, dt)
-    tuning = apply_tune_args(args)
     if args.autotune:
         from cube2_amd.autotune import autotune
         tuning = autotune(k, mass, amid, dt, log=log)
@@ -494,6 +494,7 @@ def main(argv=None) -> int:
     log(f"levels {dict(zip(*np.unique(k.ires, return_counts=True)))}; Green tables built")
     mass, amid, dt = 1.0, 0.02, 1e-3
 
+    tuning = apply_tune_args(args)  # before the first kick, so that warmup runs the same code
     t1 = time.perf_counter()
     res = k.kick(mass, amid, dt)
     cold_ms = 1e3 * (time.perf_counter() - t1)
@@ -503,7 +504,6 @@ def main(argv=None) -> int:
         t1 = time.perf_counter()
         res = k.kick(mass, amid, dt)
         log(f"warmup {w}: {1e3 * (time.perf_counter() - t1):.1f} ms")
-    tuning = apply_tune_args(args)
     if args.autotune:
         from cube2_amd.autotune import autotune
         tuning = autotune(k, mass, amid, dt, log=log)

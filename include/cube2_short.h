@@ -100,7 +100,10 @@ int cube2_get_params(cube2_params_t* p);
 /* PP pair kernel:
  *   CUBE2_PP_REF (default): the reference C kernel's pair arithmetic
  *     operation for operation (bit-identical pair forces), work-compacted;
- *     per-particle sums in another order (relative RMS ~4e-7 vs reference);
+ *     per-particle sums in another order (relative RMS ~4e-7 vs reference;
+ *     in the kick, with PP_NEWTON built in, each pair of one range is
+ *     evaluated once for both particles and the sums are 64-bit fixed
+ *     point);
  *   CUBE2_PP_EXACT: one lane per particle, the reference's summation order
  *     too: Delta-v bit-identical to pp_force_kernel.c (slow, for checking);
  *   CUBE2_PP_FAST: FMA/rsqrt arithmetic, work-compacted (its own rounding;
