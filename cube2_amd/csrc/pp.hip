@@ -1031,7 +1031,7 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
             Stage st("pp_chunks", (double)P.n_used * 24);
             k_chunk_flags<<<ceil_div(P.cap, 256), 256, 0, s>>>(
                 g_pp.raw_pos4.as<float4>(), g_pp.raw_slot.as<uint32_t>(), A.cellid, A.cell_start,
-                n_sorted, &red->err, (int)P.cap, pp_dense_lg(),
+                n_sorted, &red->err, (int)P.cap, pp_dense_lg(P.newton),
                 P.newton ? (1 << pp_direct_lg()) / 2 : 0, flags, g_pp.dense.as<int>(),
                 &red->ndense, g_pp.pos4.as<float4>(), g_pp.slot.as<uint32_t>());
             C2_HIP(hipGetLastError());
@@ -1052,7 +1052,8 @@ int pp_enqueue(const PPPlan& P, const int* d_rhoc, const int64_t* d_idx, const i
         if (pp_heavy_lg()) {
             Stage st("pp_order", (double)P.cap * 9);
             int* hist = g_pp.order_hist.as<int>();
-            k_chunk_cost<<<kOrderBlocks, kOrderBins, 0, s>>>(A, chunks, flags + P.cap, pp_dense_lg(),
+            k_chunk_cost<<<kOrderBlocks, kOrderBins, 0, s>>>(A, chunks, flags + P.cap,
+                                                             pp_dense_lg(P.newton),
                                                              pp_heavy_lg(),
                                                              g_pp.order_bin.as<unsigned char>(), hist);
             C2_HIP(hipGetLastError());

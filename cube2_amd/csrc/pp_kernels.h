@@ -85,10 +85,13 @@ bool pp_newton_mode(int mode);
 // Whether PP calls count pairs per parent tile (off by default).
 bool& pp_count_tiles();
 
-// log2 of the pair kernel's chunk size in PP cells of >= 64 particles
-inline int pp_dense_lg() {
+// log2 of the pair kernel's chunk size in PP cells of >= 64 particles: 16
+// built in, 32 with pairs once (half the reaction atomics, which share HBM
+// with the mesh kernels in the overlapped kick: at 256^3 the kick 112.3 ->
+// 111.4 ms, the pair kernel alone 64.5 -> 67.2 ms)
+inline int pp_dense_lg(bool newton) {
     const int v = ctx().tune[CUBE2_TUNE_PP_DENSE_LG];
-    return v ? v : 4;
+    return v ? v : (newton ? 5 : 4);
 }
 
 // j candidates per batch of the pair kernel: 64 (built in) or 128.

@@ -783,7 +783,7 @@ int launch_pp_tiled(const PPArgs& a, const int* chunks, const int* nchunks, int 
     const unsigned blocks = (unsigned)std::max(1, grid);
     // chunks of more than 2 particles sum directly unless tuned otherwise
     const int direct_lg = pp_direct_lg();
-    const int dense_lg = pp_dense_lg();
+    const int dense_lg = pp_dense_lg(a.react != nullptr);
     if (pp_jbatch() == 128)
         launch_pp_tiled_jb<128>(a, chunks, nchunks, blocks, mode, direct_lg, dense_lg, s);
     else

@@ -128,7 +128,8 @@ int cube2_set_pp_mode(int mode);
  *   INTERP_SEG        1..32, particles per wavefront work item of the mesh
  *                     interpolation, in units of 64 (4);
  *   PP_DENSE_LG       3..6, log2 of the pair kernel's chunk size in PP
- *                     cells of >= 64 particles (4);
+ *                     cells of >= 64 particles (4; 5 when PP_NEWTON
+ *                     evaluates pairs once);
  *   KICK_OVERLAP      1: the kick driver runs PP's cell lists and pair sums
  *                     on a second stream beside PM3, the velocity update
  *                     after both (built-in); 2: PM3, then PP;
