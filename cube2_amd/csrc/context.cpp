@@ -305,7 +305,7 @@ int cube2_set_params(const cube2_params_t* p) {
 
 int cube2_set_tuning(int key, int value) {
     static const int lo[CUBE2_TUNE_COUNT] = {1, 1, 1, 1, 3, 1, 1, 1, 1, 1},
-                     hi[CUBE2_TUNE_COUNT] = {16, 7, 6, 32, 6, 2, 64, 30, 2, 2};
+                     hi[CUBE2_TUNE_COUNT] = {256, 7, 6, 32, 6, 2, 256, 30, 2, 2};
     if (key < 0 || key >= CUBE2_TUNE_COUNT || (value != 0 && (value < lo[key] || value > hi[key]))) {
         set_error("tuning key %d value %d out of range", key, value);
         return CUBE2_ERR_ARG;

@@ -692,66 +692,30 @@ __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __rest
     block_sum_to<2>(&A.red->p_cand, sm);
 }
 
-template <int kJB>
-static hipError_t pp_occupancy(int mode, bool newton, int* per) {
-    if (newton && mode == CUBE2_PP_REF)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeRef, kJB, true>,
-                                                            256, 0);
-    if (newton && mode == CUBE2_PP_FAST)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeFast, kJB, true>,
-                                                            256, 0);
-    if (mode == CUBE2_PP_REF)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeRef, kJB, false>,
-                                                            256, 0);
-    if (mode == CUBE2_PP_FAST)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModeFast, kJB, false>,
-                                                            256, 0);
-    if (mode == CUBE2_PP_PLUMMER)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            per, k_pp_tiled<kModePlummer, kJB, false>, 256, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_pp_tiled<kModePrecise, kJB, false>,
-                                                        256, 0);
-}
-
 bool pp_newton_mode(int mode) { return mode == CUBE2_PP_REF || mode == CUBE2_PP_FAST; }
 
-// Blocks of the persistent pair kernel: the resident maximum over all CUs
-// (occupancy query, cached per mode).
-// overlap: the kernel runs beside PM3 (CUBE2_TUNE_PP_OVERLAP_BLOCKS per CU,
-// at most the occupancy maximum).
+// Blocks of the pair kernel: 64 per CU, ten times the resident maximum (6),
+// unless tuned.  Its wavefronts take chunks by a static stride over the
+// list; with a grid far beyond what is resident, the hardware hands out
+// blocks as others retire, which evens out the uneven chunk costs (at 256^3
+// pairs once, alone: 67.3 ms at 6 blocks per CU, 63.3 at 24, 56.1 at 64,
+// 57.4 at 128) and lets the PM3 mesh kernels on the other stream take the
+// slots of retiring blocks throughout (overlapped kick: 111.5 ms at 6, 105.3
+// at 24, 103.3 at 64).  overlap: the kernel runs beside PM3
+// (CUBE2_TUNE_PP_OVERLAP_BLOCKS per CU if set).
 int pp_tiled_grid(int mode, bool overlap, bool newton) {
-    static int cache[32] = {};
-    const int jb = pp_jbatch();
-    newton = newton && pp_newton_mode(mode);
-    const int m = (mode & 7) + (jb == 128 ? 8 : 0) + (newton ? 16 : 0);
-    const int forced = ctx().tune[CUBE2_TUNE_PP_BLOCKS_PER_CU];
-    int dev = 0, ncu = 0, per = 0;
-    if (overlap) {
-        const int full = pp_tiled_grid(mode, false, newton);
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            ncu = 256;
-        // explicit choice: may exceed the resident maximum, so that blocks
-        // retire throughout the launch and the mesh kernels' workgroups take
-        // their slots
-        const int t = ctx().tune[CUBE2_TUNE_PP_OVERLAP_BLOCKS];
-        return t ? std::max(1, ncu) * t : full;
-    }
-    if (forced) {
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            ncu = 256;
-        return std::max(1, ncu) * forced;
-    }
-    if (cache[m]) return cache[m];
+    (void)mode;
+    (void)newton;
+    constexpr int kBlocksPerCU = 64;
+    int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         ncu = 256;
-    const hipError_t e =
-        jb == 128 ? pp_occupancy<128>(mode, newton, &per) : pp_occupancy<64>(mode, newton, &per);
-    if (e != hipSuccess || per < 1) per = 2;
-    cache[m] = std::max(1, ncu) * per;
-    return cache[m];
+    ncu = std::max(1, ncu);
+    const int forced = ctx().tune[CUBE2_TUNE_PP_BLOCKS_PER_CU];
+    const int over = ctx().tune[CUBE2_TUNE_PP_OVERLAP_BLOCKS];
+    if (overlap && over) return ncu * over;
+    return ncu * (forced ? forced : kBlocksPerCU);
 }
 
 template <int kJB>

@@ -119,8 +119,9 @@ int cube2_set_pp_mode(int mode);
 /* Launch-shape tunables (no counterpart in the reference; set by the
  * autotuner in cube2_amd/autotune.py).  value 0 restores the built-in
  * choice.  Valid values:
- *   PP_BLOCKS_PER_CU  1..16, blocks per CU of the persistent pair kernel
- *                     (built-in: the occupancy maximum);
+ *   PP_BLOCKS_PER_CU  1..256, blocks per CU of the pair kernel (built-in:
+ *                     64, beyond the resident maximum, so that the hardware
+ *                     balances the chunks' uneven costs);
  *   PP_DIRECT_LG      1..7, chunks of > 2^(v-1) particles (lanes for >= 2^v
  *                     i slots) sum directly, smaller ones queue pairs (2);
  *   FFT_MIN_LGC       1..6, minimum log2 column-tile width of the y/z FFT
@@ -133,9 +134,8 @@ int cube2_set_pp_mode(int mode);
  *   KICK_OVERLAP      1: the kick driver runs PP's cell lists and pair sums
  *                     on a second stream beside PM3, the velocity update
  *                     after both (built-in); 2: PM3, then PP;
- *   PP_OVERLAP_BLOCKS 1..64, blocks per CU of the pair kernel while PM3
- *                     runs beside it, beyond the resident maximum if asked
- *                     (built-in: the resident maximum);
+ *   PP_OVERLAP_BLOCKS 1..256, blocks per CU of the pair kernel while PM3
+ *                     runs beside it (built-in: as PP_BLOCKS_PER_CU);
  *   PP_ORDER          1: the pair kernel takes its chunks in cell order
  *                     (built-in); 2..30: chunks of an estimated 2^v
  *                     candidate pairs or more first, heaviest first, then
