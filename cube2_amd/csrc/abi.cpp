@@ -1,9 +1,12 @@
 // C-ABI entry points: the reference's four bind(C) symbols (host pointers,
 // fatal errors) and the cube2_* device-resident API (status codes).
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 
 #include "common.h"
 #include "engine.h"
@@ -17,6 +20,85 @@ static std::recursive_mutex& g_mx = api_mutex();  // shared with context.cpp
 static int64_t n_used_of(const Geom& g, const int64_t* idx_b_r) {
     return idx_b_r[(size_t)g.ntile * g.nte * g.nte - 1];
 }
+
+// The drop-in entry points take pageable host arrays.  They cross PCIe
+// through two pinned staging buffers: host threads copy chunk k into one
+// while the DMA engine moves chunk k-1 from the other (a copy from pageable
+// memory otherwise stages through one host thread).
+namespace {
+constexpr size_t kStageBytes = (size_t)32 << 20;
+struct Staging {
+    HostBuf buf[2];
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+Staging g_stage;
+
+int staging_ready() {
+    for (int b = 0; b < 2; b++) {
+        if (g_stage.buf[b].cap < kStageBytes) C2_TRY(g_stage.buf[b].reserve(kStageBytes));
+        if (!g_stage.ev[b]) C2_HIP(hipEventCreateWithFlags(&g_stage.ev[b], hipEventDisableTiming));
+    }
+    return CUBE2_OK;
+}
+
+void par_memcpy(void* dst, const void* src, size_t n) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned T = std::min(8u, std::max(1u, hw / 2));
+    if (T == 1 || n < ((size_t)4 << 20)) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = ((n + T - 1) / T + 63) & ~(size_t)63;
+    std::thread th[8];
+    for (unsigned t = 1; t < T; t++) {
+        const size_t o = std::min(n, per * t), e = std::min(n, o + per);
+        th[t] = std::thread(
+            [=] { std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, e - o); });
+    }
+    std::memcpy(dst, src, std::min(n, per));
+    for (unsigned t = 1; t < T; t++) th[t].join();
+}
+
+// Host to device, ordered on stream s; the host array may change on return.
+int copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (!bytes) return CUBE2_OK;
+    C2_TRY(staging_ready());
+    for (size_t off = 0, k = 0; off < bytes; off += kStageBytes, k++) {
+        const int b = (int)(k & 1);
+        const size_t len = std::min(kStageBytes, bytes - off);
+        C2_HIP(hipEventSynchronize(g_stage.ev[b]));  // the buffer's last DMA is done
+        par_memcpy(g_stage.buf[b].ptr, static_cast<const char*>(src) + off, len);
+        C2_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, g_stage.buf[b].ptr, len,
+                              hipMemcpyHostToDevice, s));
+        C2_HIP(hipEventRecord(g_stage.ev[b], s));
+    }
+    return CUBE2_OK;
+}
+
+// Device to host after the work queued on stream s; complete on return.
+int copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (!bytes) return CUBE2_OK;
+    C2_TRY(staging_ready());
+    const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+    auto issue = [&](size_t k) -> int {
+        const int b = (int)(k & 1);
+        const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        C2_HIP(hipMemcpyAsync(g_stage.buf[b].ptr, static_cast<const char*>(src) + off, len,
+                              hipMemcpyDeviceToHost, s));
+        C2_HIP(hipEventRecord(g_stage.ev[b], s));
+        return CUBE2_OK;
+    };
+    C2_TRY(issue(0));
+    for (size_t k = 0; k < nch; k++) {
+        if (k + 1 < nch) C2_TRY(issue(k + 1));  // into the other buffer, already drained
+        const int b = (int)(k & 1);
+        const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        C2_HIP(hipEventSynchronize(g_stage.ev[b]));
+        par_memcpy(static_cast<char*>(dst) + off, g_stage.buf[b].ptr, len);
+    }
+    return CUBE2_OK;
+}
+}  // namespace
 
 // Stage host arrays for the drop-in entry points.
 static int stage_inputs(const int* rhoc, const int64_t* idx_b_r, const int16_t* xp, bool reuse_xp,
@@ -32,11 +114,11 @@ static int stage_inputs(const int* rhoc, const int64_t* idx_b_r, const int16_t* 
     const size_t nr = (size_t)g.ntile * g.nte * g.nte * g.nte, ni = (size_t)g.ntile * g.nte * g.nte;
     C2_TRY(C.d_rhoc.reserve(nr * sizeof(int)));
     C2_TRY(C.d_idx.reserve(ni * sizeof(int64_t)));
-    C2_HIP(hipMemcpyAsync(C.d_rhoc.ptr, rhoc, nr * sizeof(int), hipMemcpyHostToDevice, s));
-    C2_HIP(hipMemcpyAsync(C.d_idx.ptr, idx_b_r, ni * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    C2_TRY(copy_h2d(C.d_rhoc.ptr, rhoc, nr * sizeof(int), s));
+    C2_TRY(copy_h2d(C.d_idx.ptr, idx_b_r, ni * sizeof(int64_t), s));
     if (!(reuse_xp && C.staged_xp_host == xp && C.staged_xp_n == n)) {
         C2_TRY(C.d_xp.reserve((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(int16_t)));
-        if (n) C2_HIP(hipMemcpyAsync(C.d_xp.ptr, xp, (size_t)n * 3 * sizeof(int16_t), hipMemcpyHostToDevice, s));
+        C2_TRY(copy_h2d(C.d_xp.ptr, xp, (size_t)n * 3 * sizeof(int16_t), s));
         C.staged_xp_host = xp;
         C.staged_xp_n = n;
     }
@@ -78,19 +160,14 @@ void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* rati
     int64_t n = 0;
     if (stage_inputs(rhoc, idx_b_r, xp, C.dv_state == 1, &n) != CUBE2_OK) die("[PP]", false);
     if (C.d_vp.reserve((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)) != CUBE2_OK) die("[PP]", false);
-    if (n && hipMemcpyAsync(C.d_vp.ptr, vp, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice,
-                            C.stream) != hipSuccess) {
-        set_error("H2D of vp failed");
+    if (copy_h2d(C.d_vp.ptr, vp, (size_t)n * 3 * sizeof(float), C.stream) != CUBE2_OK)
         die("[PP]", false);
-    }
     if (pp_force(ires, ixyz3, apm3, ratio_sf, C.d_rhoc.as<int>(), C.d_idx.as<int64_t>(),
                  C.d_xp.as<int16_t>(), C.d_vp.as<float>(), n, mass_p_cdm, a_mid, dt, f2max, vmax,
                  ptotal, ttotal, nullptr) != CUBE2_OK)
         die("[PP]", false);
-    if (n && hipMemcpy(vp, C.d_vp.ptr, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
-        set_error("D2H of vp failed");
+    if (copy_d2h(vp, C.d_vp.ptr, (size_t)n * 3 * sizeof(float), C.stream) != CUBE2_OK)
         die("[PP]", false);
-    }
     C.staged_xp_host = nullptr;  // a new kick re-uploads positions
     if (prof) {
         const Geom& g = C.g;
@@ -197,6 +274,8 @@ void c_gpu_velocity_finalize(void) {
     C.d_vp.release();
     C.d_rhoc.release();
     C.d_idx.release();
+    g_stage.buf[0].release();
+    g_stage.buf[1].release();
     C.dv_state = 0;
     C.dv_n_used = 0;
 }
