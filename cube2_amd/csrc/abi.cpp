@@ -160,11 +160,14 @@ void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* rati
     int64_t n = 0;
     if (stage_inputs(rhoc, idx_b_r, xp, C.dv_state == 1, &n) != CUBE2_OK) die("[PP]", false);
     if (C.d_vp.reserve((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)) != CUBE2_OK) die("[PP]", false);
-    if (copy_h2d(C.d_vp.ptr, vp, (size_t)n * 3 * sizeof(float), C.stream) != CUBE2_OK)
-        die("[PP]", false);
+    // vp crosses PCIe while the pair kernel runs (it is read only by the
+    // velocity update after it)
+    auto fill_vp = [&](hipStream_t s) {
+        return copy_h2d(C.d_vp.ptr, vp, (size_t)n * 3 * sizeof(float), s);
+    };
     if (pp_force(ires, ixyz3, apm3, ratio_sf, C.d_rhoc.as<int>(), C.d_idx.as<int64_t>(),
                  C.d_xp.as<int16_t>(), C.d_vp.as<float>(), n, mass_p_cdm, a_mid, dt, f2max, vmax,
-                 ptotal, ttotal, nullptr) != CUBE2_OK)
+                 ptotal, ttotal, nullptr, false, fill_vp) != CUBE2_OK)
         die("[PP]", false);
     if (copy_d2h(vp, C.d_vp.ptr, (size_t)n * 3 * sizeof(float), C.stream) != CUBE2_OK)
         die("[PP]", false);

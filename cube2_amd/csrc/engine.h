@@ -7,6 +7,7 @@
 //   *_finish:  after the stream has completed, host-side results and errors.
 #pragma once
 #include <array>
+#include <functional>
 
 #include "common.h"
 #include "pp_kernels.h"
@@ -52,7 +53,8 @@ int pp_finish(const PPPlan& P, float* f2max, float* vmax, double* ptotal, double
 int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
              int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
-             double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned = false);
+             double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned = false,
+             const std::function<int(hipStream_t)>& fill_vp = nullptr);
 // Blocks of the persistent pair kernel (overlap: beside PM3).
 int pp_tiled_grid(int mode, bool overlap, bool newton = false);
 // The last PP call's sorted particles and cell lists (for the learned

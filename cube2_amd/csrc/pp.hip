@@ -1245,7 +1245,8 @@ int pp_debug_tile_pairs(int ntile, uint64_t* counts) {
 int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ratio_sf,
              const int* d_rhoc, const int64_t* d_idx, const int16_t* d_xp, float* d_vp,
              int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
-             double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned) {
+             double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned,
+             const std::function<int(hipStream_t)>& fill_vp) {
     ensure_params();
     Context& C = ctx();
     auto t0 = std::chrono::steady_clock::now();
@@ -1257,13 +1258,26 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
         set_error("PM3/PP particle count changed within a kick");
         return CUBE2_ERR_STATE;
     }
+    // fill_vp (the drop-in call: vp comes from host memory): the pair sums
+    // run split from the velocity update, and vp is filled on the second
+    // stream beside them (not in the exact-order mode, which has no split
+    // pass: vp is filled first)
+    const bool late = fill_vp && C.pp_mode != CUBE2_PP_EXACT;
+    if (fill_vp && !late) C2_TRY(fill_vp(C.stream));
     PPPlan P;
-    C2_TRY(pp_prepare(ires, ixyz3, apm3, ratio_sf, n_used, P, learned, learned));
+    C2_TRY(pp_prepare(ires, ixyz3, apm3, ratio_sf, n_used, P, learned || late, learned));
     P.grid_pairs = pp_tiled_grid(P.mode, false, P.newton);  // nothing runs beside it here
     const double* dv = (C.dv_state == 1) ? C.dv64.as<double>() : nullptr;
     const float* d_scal = nullptr;
     C2_TRY(scalars_upload(mass, a_mid, dt, C.stream, &d_scal));
     C2_TRY(pp_enqueue(P, d_rhoc, d_idx, d_xp, d_vp, d_scal, dv, C.stream));
+    if (late) {
+        static hipEvent_t filled = nullptr;
+        if (!filled) C2_HIP(hipEventCreateWithFlags(&filled, hipEventDisableTiming));
+        C2_TRY(fill_vp(C.stream2));
+        C2_HIP(hipEventRecord(filled, C.stream2));
+        C2_HIP(hipStreamWaitEvent(C.stream, filled, 0));
+    }
     if (P.split) C2_TRY(pp_enqueue_apply(P, d_vp, dv, C.stream));
     C2_TRY(sync_and_flush());
     C.dv_state = 0;
