@@ -14,9 +14,10 @@ import time
 from . import _lib
 
 CANDIDATES = {
-    "pp_blocks_per_cu": [0, 4, 5, 8],
-    "pp_dense_lg": [0, 5, 6],
-    "pp_direct_lg": [0, 7, 5, 4],
+    "pp_blocks_per_cu": [0, 24, 48, 128],
+    "pp_overlap_blocks": [0, 32, 96],
+    "pp_dense_lg": [0, 4, 6],
+    "pp_direct_lg": [0, 3, 1],
     "fft_min_lgc": [0, 3, 4],
     "interp_seg": [0, 2, 8],
     "pp_order": [0, 20],
