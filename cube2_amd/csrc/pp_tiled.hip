@@ -1,8 +1,10 @@
 // PP pair kernel.
 //
-// Persistent wavefronts take chunks of physical particles of one PP cell:
-// the whole cell if it holds fewer than 64, else 2^dense_lg at a time (16
-// built in).  The lanes are split into I i-slots x J j-slots (I the chunk
+// Wavefronts take chunks of physical particles of one PP cell by a static
+// stride (a grid of 64 blocks per CU, far more than are resident, so that
+// the hardware balances the chunks' uneven costs): the whole cell if it
+// holds fewer than 64, else 2^dense_lg at a time (16 built in, 32 with pairs
+// once).  The lanes are split into I i-slots x J j-slots (I the chunk
 // size rounded up to a power of two, I*J = 64).  The j of the (2R+1)^2
 // neighbour rows are contiguous runs of the sorted particle stream; a
 // wavefront concatenates them into one candidate list and fetches it 64 at
@@ -11,9 +13,8 @@
 // Chunks of I >= 2^direct_lg (I >= 4 built in) sum directly: lane (i, s)
 // evaluates the force law on the pairs of i with the j of its j-slot s as
 // they are accepted, partial sums in registers in the reference's j order,
-// combined over the J slots at the end.  The i of cells of >= 64 particles
-// are taken in 8x8x8 Morton sub-cell order (so a chunk is spatially
-// compact), and each fetched batch of j is first reduced to those within
+// combined over the J slots at the end.  Cells of >= 64 particles are
+// stored in 8x8x8 Morton sub-cell order (so a chunk is spatially compact), and each fetched batch of j is first reduced to those within
 // the range of the chunk's bounding box: the smaller the chunk, the fewer
 // lanes idle on a j accepted by only some of its i (REF at 256^3: 112 ms
 // with 64-particle chunks, 92 ms with 16).  Smaller chunks append accepted
@@ -27,7 +28,9 @@
 //   representable or at least 2^-49 (relative) away from every rounding
 //   midpoint, so this equals IEEE a/b.  The pair forces are bit-identical to
 //   the reference; only the order of the per-i sums differs (<= ~4e-7
-//   relative RMS).
+//   relative RMS), and with pairs once (kNewton, below) a pair of one range
+//   is evaluated once and its force summed for both particles in 64-bit
+//   fixed point.
 // FAST: FMA/rsqrt arithmetic with F_ra in Horner form.
 // PLUMMER: Plummer softening r/(r^2 + app^2)^(3/2) in place of F_ra(r, app)
 //   (no counterpart in the reference), in PRECISE's arithmetic.
@@ -304,13 +307,13 @@ __device__ inline int find_tile2(const PPTile* __restrict__ T, int ntile, uint32
     return lo;
 }
 
-// Persistent: wavefront w of the grid takes chunks w, w + W, w + 2W, ...
-// (W = waves in the grid); the chunk count is read on the device.  Chunk-
-// level values are kept wavefront-uniform (scalar registers), and the
-// register budget is capped for 6 wavefronts per SIMD (REF at 256^3: 80.7
-// ms; budgets for 5 and 7 wavefronts 89.2 and 86.0 ms; 4 blocks per CU
-// instead of 6: 105 ms).  The REF variant spills a few registers outside its
-// inner loops.
+// Wavefront w of the grid takes chunks w, w + W, w + 2W, ... (W = waves in
+// the grid); the chunk count is read on the device.  Chunk-level values are
+// kept wavefront-uniform (scalar registers), and the register budget is
+// capped for 6 wavefronts per SIMD (REF at 256^3, ordered pairs: 80.7 ms;
+// budgets for 5 and 7 wavefronts 89.2 and 86.0 ms; 4 blocks per CU instead
+// of 6: 105 ms).  The REF variant spills a few registers outside its inner
+// loops.
 template <int kMode, int kJB, bool kNewton>
 __global__ __launch_bounds__(256, 6) void k_pp_tiled(PPArgs A, const int* __restrict__ chunks,
                                                   const int* __restrict__ nchunks_dev,
