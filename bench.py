@@ -463,7 +463,7 @@ def main(argv=None) -> int:
         return copy_probe(args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path runs on a single MI355X; "
-              "multi-GPU sharding is by parent tile and is not part of this build.",
+              "sharding across GPUs (by parent tile) is not implemented in this library.",
               file=sys.stderr)
         return 2
 
