@@ -715,7 +715,7 @@ __global__ __launch_bounds__(256) void k_pm3_interp(MeshArgs A, const float* __r
     }
 }
 
-// Blocks of the persistent interpolation: what fits the CUs at once.
+// Blocks of the interpolation, whose wavefronts take items by a static stride.
 static int interp_grid() {
     static int cached = 0;
     if (cached) return cached;
@@ -726,7 +726,9 @@ static int interp_grid() {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pm3_interp, 256, 0) != hipSuccess ||
         per < 1)
         per = 2;
-    cached = std::max(1, ncu) * per;
+    // eight times what is resident: the hardware hands out blocks as others
+    // retire, which evens out the items' uneven costs (256^3: 2.35 -> 2.12 ms)
+    cached = std::max(1, ncu) * per * 8;
     return cached;
 }
 
