@@ -496,7 +496,9 @@ static int deposit_grid(size_t lds) {
             hipSuccess ||
         per < 1)
         per = 2;
-    cache[lds] = std::max(1, ncu) * per;
+    // eight times what is resident: blocks retire and are handed out again as
+    // the items' uneven costs come in
+    cache[lds] = std::max(1, ncu) * per * 8;
     return cache[lds];
 }
 
