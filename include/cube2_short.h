@@ -101,9 +101,8 @@ int cube2_get_params(cube2_params_t* p);
  *   CUBE2_PP_REF (default): the reference C kernel's pair arithmetic
  *     operation for operation (bit-identical pair forces), work-compacted;
  *     per-particle sums in another order (relative RMS ~4e-7 vs reference;
- *     in the kick, with PP_NEWTON built in, each pair of one range is
- *     evaluated once for both particles and the sums are 64-bit fixed
- *     point);
+ *     with PP_NEWTON built in, each pair of one range is evaluated once for
+ *     both particles and the sums are 64-bit fixed point);
  *   CUBE2_PP_EXACT: one lane per particle, the reference's summation order
  *     too: Delta-v bit-identical to pp_force_kernel.c (slow, for checking);
  *   CUBE2_PP_FAST: FMA/rsqrt arithmetic, work-compacted (its own rounding;
@@ -142,11 +141,13 @@ int cube2_set_pp_mode(int mode);
  *                     the rest in cell order;
  *   PP_JBATCH         1: the pair kernel fetches and tests j candidates 64
  *                     at a time (built-in); 2: 128 at a time;
- *   PP_NEWTON         1: the split pair pass of the kick (REF, FAST)
- *                     evaluates each pair of two active particles of one
- *                     range once and adds its force to both, summing in
- *                     64-bit fixed point (built-in; needs app > 0); 2: every
- *                     ordered pair is evaluated by its i.
+ *   PP_NEWTON         1: REF and FAST passes (not the learned kick)
+ *                     evaluate each pair of two active particles of one
+ *                     range once and add its force to both, summing in
+ *                     64-bit fixed point at 2^k, k = min(40, floor(62 -
+ *                     log2(10 cap / app^2))), cap the particle capacity
+ *                     (built-in; used when app > 0 and k >= 12, else as 2);
+ *                     2: every ordered pair is evaluated by its i.
  * A change makes the next recorded kick record again. */
 enum {
     CUBE2_TUNE_PP_BLOCKS_PER_CU = 0,
