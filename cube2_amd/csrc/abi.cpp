@@ -7,6 +7,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "common.h"
 #include "engine.h"
@@ -27,9 +28,19 @@ static int64_t n_used_of(const Geom& g, const int64_t* idx_b_r) {
 // memory otherwise stages through one host thread).
 namespace {
 constexpr size_t kStageBytes = (size_t)32 << 20;
+// Slot runs of the physical coarse cells' particles (see phys_runs).
+struct Runs {
+    std::vector<int64_t> dev;    // first slots, then the compact prefix (nruns + 1), in rows
+    std::vector<size_t> boff;    // compact prefix in bytes of 3 floats per row (nruns + 1)
+    std::vector<float*> host;    // each run's first row in the caller's vp
+    int nruns = 0;
+    int64_t rows = 0;
+};
 struct Staging {
     HostBuf buf[2];
     hipEvent_t ev[2] = {nullptr, nullptr};
+    DevBuf runs, compact;  // the run table and the compact rows on the device
+    Runs r;
 };
 Staging g_stage;
 
@@ -41,33 +52,58 @@ int staging_ready() {
     return CUBE2_OK;
 }
 
-void par_memcpy(void* dst, const void* src, size_t n) {
+// f(lo, hi) over [0, n), split among up to 8 host threads (one below 4 MB).
+template <class F>
+void par_bytes(size_t n, const F& f) {
     const unsigned hw = std::thread::hardware_concurrency();
     const unsigned T = std::min(8u, std::max(1u, hw / 2));
     if (T == 1 || n < ((size_t)4 << 20)) {
-        std::memcpy(dst, src, n);
+        f((size_t)0, n);
         return;
     }
     const size_t per = ((n + T - 1) / T + 63) & ~(size_t)63;
     std::thread th[8];
     for (unsigned t = 1; t < T; t++) {
         const size_t o = std::min(n, per * t), e = std::min(n, o + per);
-        th[t] = std::thread(
-            [=] { std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, e - o); });
+        th[t] = std::thread([&f, o, e] { f(o, e); });
     }
-    std::memcpy(dst, src, std::min(n, per));
+    f((size_t)0, std::min(n, per));
     for (unsigned t = 1; t < T; t++) th[t].join();
 }
 
-// Host to device, ordered on stream s; the host array may change on return.
-int copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+// Bytes [off, off + len) of the runs' compact stream to or from a staging
+// buffer that holds exactly that range.
+void runs_copy(const Runs& R, char* pinned, size_t off, size_t len, bool to_pinned) {
+    par_bytes(len, [&](size_t lo, size_t hi) {
+        size_t a = off + lo;
+        const size_t e = off + hi;
+        size_t r = (size_t)(std::upper_bound(R.boff.begin(), R.boff.end(), a) - R.boff.begin()) - 1;
+        while (a < e) {
+            const size_t re = std::min(e, R.boff[r + 1]);
+            char* h = reinterpret_cast<char*>(R.host[r]) + (a - R.boff[r]);
+            char* p = pinned + (a - off);
+            if (to_pinned)
+                std::memcpy(p, h, re - a);
+            else
+                std::memcpy(h, p, re - a);
+            a = re;
+            r++;
+        }
+    });
+}
+
+// Host to device, ordered on stream s; fill(pinned, off, len) writes bytes
+// [off, off + len) of the source into the staging buffer.  The source may
+// change on return.
+template <class Fill>
+int stage_up(void* dst, size_t bytes, hipStream_t s, const Fill& fill) {
     if (!bytes) return CUBE2_OK;
     C2_TRY(staging_ready());
     for (size_t off = 0, k = 0; off < bytes; off += kStageBytes, k++) {
         const int b = (int)(k & 1);
         const size_t len = std::min(kStageBytes, bytes - off);
         C2_HIP(hipEventSynchronize(g_stage.ev[b]));  // the buffer's last DMA is done
-        par_memcpy(g_stage.buf[b].ptr, static_cast<const char*>(src) + off, len);
+        fill(static_cast<char*>(g_stage.buf[b].ptr), off, len);
         C2_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, g_stage.buf[b].ptr, len,
                               hipMemcpyHostToDevice, s));
         C2_HIP(hipEventRecord(g_stage.ev[b], s));
@@ -75,8 +111,10 @@ int copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
     return CUBE2_OK;
 }
 
-// Device to host after the work queued on stream s; complete on return.
-int copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+// Device to host after the work queued on stream s; drain(pinned, off, len)
+// takes bytes [off, off + len).  Complete on return.
+template <class Drain>
+int stage_down(const void* src, size_t bytes, hipStream_t s, const Drain& drain) {
     if (!bytes) return CUBE2_OK;
     C2_TRY(staging_ready());
     const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
@@ -94,9 +132,99 @@ int copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
         const int b = (int)(k & 1);
         const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
         C2_HIP(hipEventSynchronize(g_stage.ev[b]));
-        par_memcpy(static_cast<char*>(dst) + off, g_stage.buf[b].ptr, len);
+        drain(static_cast<const char*>(g_stage.buf[b].ptr), off, len);
     }
     return CUBE2_OK;
+}
+
+int copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const char* h = static_cast<const char*>(src);
+    return stage_up(dst, bytes, s, [h](char* p, size_t off, size_t len) {
+        par_bytes(len, [=](size_t lo, size_t hi) { std::memcpy(p + lo, h + off + lo, hi - lo); });
+    });
+}
+
+int copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    char* h = static_cast<char*>(dst);
+    return stage_down(src, bytes, s, [h](const char* p, size_t off, size_t len) {
+        par_bytes(len, [=](size_t lo, size_t hi) { std::memcpy(h + off + lo, p + lo, hi - lo); });
+    });
+}
+
+// The slot runs of the physical coarse cells' particles, one per physical
+// row (t, k, j), as k_phys_rows finds them (the first slot of a cell is
+// idx_b_r of its row minus the particles of the row's cells from it on),
+// adjacent runs merged.  Only these particles' velocities change in a PP
+// call (halo copies are sources only), so the drop-in call moves just their
+// rows of vp across PCIe.  false when a run falls outside [0, n): the call
+// then moves the whole array, and the cell-list kernels report the layout.
+bool phys_runs(const Geom& g, const int* rhoc, const int64_t* idx_b_r, int64_t n, float* vp,
+               Runs& R) {
+    R.dev.clear();
+    R.host.clear();
+    std::vector<int64_t> base, len;
+    for (int t = 0; t < g.ntile; t++)
+        for (int k = g.ncb; k < g.ncb + g.nt; k++)
+            for (int j = g.ncb; j < g.ncb + g.nt; j++) {
+                const int64_t row = ((int64_t)t * g.nte + k) * g.nte + j;
+                const int* rc = rhoc + row * g.nte;
+                int64_t tail = 0, phys = 0;
+                for (int i = g.ncb + g.nt; i < g.nte; i++) tail += rc[i];
+                for (int i = g.ncb; i < g.ncb + g.nt; i++) phys += rc[i];
+                const int64_t e = idx_b_r[row] - tail, b = e - phys;
+                if (phys < 0 || tail < 0 || b < 0 || e > n) return false;
+                if (phys == 0) continue;
+                if (!base.empty() && base.back() + len.back() == b)
+                    len.back() += phys;
+                else {
+                    base.push_back(b);
+                    len.push_back(phys);
+                }
+            }
+    if (base.size() >= (size_t)INT32_MAX) return false;
+    R.nruns = (int)base.size();
+    R.dev = base;
+    R.boff.assign(1, 0);
+    int64_t acc = 0;
+    R.dev.push_back(0);
+    for (int r = 0; r < R.nruns; r++) {
+        R.host.push_back(vp + 3 * base[r]);
+        acc += len[r];
+        R.dev.push_back(acc);
+        R.boff.push_back((size_t)acc * 3 * sizeof(float));
+    }
+    R.rows = acc;
+    return true;
+}
+
+// vp's physical rows up (on stream s, into d_vp) and back (after the work on
+// C.stream): the run table and the compact rows on the device, the rows
+// gathered from and scattered to the caller's array by host threads.
+int runs_reserve() {
+    const Runs& R = g_stage.r;
+    C2_TRY(g_stage.runs.reserve(R.dev.size() * sizeof(int64_t)));
+    return g_stage.compact.reserve(std::max<size_t>((size_t)R.rows * 3 * sizeof(float), 4));
+}
+
+int runs_h2d(float* d_vp, hipStream_t s) {  // after runs_reserve
+    const Runs& R = g_stage.r;
+    const size_t bytes = (size_t)R.rows * 3 * sizeof(float);
+    C2_TRY(copy_h2d(g_stage.runs.ptr, R.dev.data(), R.dev.size() * sizeof(int64_t), s));
+    C2_TRY(stage_up(g_stage.compact.ptr, bytes, s, [&R](char* p, size_t off, size_t len) {
+        runs_copy(R, p, off, len, true);
+    }));
+    return vp_runs_move(d_vp, g_stage.compact.as<float>(), g_stage.runs.as<int64_t>(), R.nruns,
+                        false, s);
+}
+
+int runs_d2h(const float* d_vp, hipStream_t s) {
+    const Runs& R = g_stage.r;
+    C2_TRY(vp_runs_move(g_stage.compact.as<float>(), d_vp, g_stage.runs.as<int64_t>(), R.nruns,
+                        true, s));
+    return stage_down(g_stage.compact.ptr, (size_t)R.rows * 3 * sizeof(float), s,
+                      [&R](const char* p, size_t off, size_t len) {
+                          runs_copy(R, const_cast<char*>(p), off, len, false);
+                      });
 }
 }  // namespace
 
@@ -161,15 +289,22 @@ void c_pp_force_kernel(int* isort, int* ires, int* ixyz3, float* apm3, int* rati
     if (stage_inputs(rhoc, idx_b_r, xp, C.dv_state == 1, &n) != CUBE2_OK) die("[PP]", false);
     if (C.d_vp.reserve((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)) != CUBE2_OK) die("[PP]", false);
     // vp crosses PCIe while the pair kernel runs (it is read only by the
-    // velocity update after it)
+    // velocity update after it); with the split pass (not the exact-order
+    // mode, whose pair kernel updates vp itself) only the physical particles'
+    // rows cross, the halo copies' being neither read nor changed
+    const bool runs = C.pp_mode != CUBE2_PP_EXACT &&
+                      phys_runs(C.g, rhoc, idx_b_r, n, vp, g_stage.r) && g_stage.r.rows < n;
+    if (runs && runs_reserve() != CUBE2_OK) die("[PP]", false);
     auto fill_vp = [&](hipStream_t s) {
+        if (runs) return runs_h2d(C.d_vp.as<float>(), s);
         return copy_h2d(C.d_vp.ptr, vp, (size_t)n * 3 * sizeof(float), s);
     };
     if (pp_force(ires, ixyz3, apm3, ratio_sf, C.d_rhoc.as<int>(), C.d_idx.as<int64_t>(),
                  C.d_xp.as<int16_t>(), C.d_vp.as<float>(), n, mass_p_cdm, a_mid, dt, f2max, vmax,
                  ptotal, ttotal, nullptr, false, fill_vp) != CUBE2_OK)
         die("[PP]", false);
-    if (copy_d2h(vp, C.d_vp.ptr, (size_t)n * 3 * sizeof(float), C.stream) != CUBE2_OK)
+    if ((runs ? runs_d2h(C.d_vp.as<float>(), C.stream)
+              : copy_d2h(vp, C.d_vp.ptr, (size_t)n * 3 * sizeof(float), C.stream)) != CUBE2_OK)
         die("[PP]", false);
     C.staged_xp_host = nullptr;  // a new kick re-uploads positions
     if (prof) {
@@ -279,6 +414,8 @@ void c_gpu_velocity_finalize(void) {
     C.d_idx.release();
     g_stage.buf[0].release();
     g_stage.buf[1].release();
+    g_stage.runs.release();
+    g_stage.compact.release();
     C.dv_state = 0;
     C.dv_n_used = 0;
 }

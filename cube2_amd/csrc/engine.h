@@ -55,6 +55,12 @@ int pp_force(const int* ires, const int* ixyz3, const float* apm3, const int* ra
              int64_t n_used, float mass, float a_mid, float dt, float* f2max, float* vmax,
              double* ptotal, double* ttotal, uint64_t* pair_counts, bool learned = false,
              const std::function<int(hipStream_t)>& fill_vp = nullptr);
+// Copies the (x, y, z) float rows of nruns slot runs, on stream s, between a
+// full per-slot array and a compact one (runs: the runs' first slots, then
+// their compact offsets as an exclusive prefix of nruns + 1 entries, in
+// rows); to_compact: full src -> compact dst, else compact src -> full dst.
+int vp_runs_move(float* dst, const float* src, const int64_t* runs, int nruns, bool to_compact,
+                 hipStream_t s);
 // Blocks of the persistent pair kernel (overlap: beside PM3).
 int pp_tiled_grid(int mode, bool overlap, bool newton = false);
 // The last PP call's sorted particles and cell lists (for the learned

@@ -848,6 +848,30 @@ __global__ void k_phys_rows(const int64_t* __restrict__ nzero, int nt, int ncb, 
     len[r] = ok ? (int)(e - b) : 0;
 }
 
+// Moves the (x, y, z) rows of slot runs between a full per-slot array and a
+// compact one: run r is slots [base[r], base[r] + coff[r+1] - coff[r]) of the
+// full array and rows [coff[r], coff[r+1]) of the compact one (runs =
+// base[nruns] then coff[nruns + 1]).  One block per run at a time.
+__global__ __launch_bounds__(256) void k_runs_move(float* __restrict__ dst,
+                                                   const float* __restrict__ src,
+                                                   const int64_t* __restrict__ runs, int nruns,
+                                                   int to_compact) {
+    const int64_t* coff = runs + nruns;
+    for (int r = blockIdx.x; r < nruns; r += gridDim.x) {
+        const int64_t full = 3 * runs[r], comp = 3 * coff[r], n = 3 * (coff[r + 1] - coff[r]);
+        const int64_t so = to_compact ? full : comp, d0 = to_compact ? comp : full;
+        for (int64_t q = threadIdx.x; q < n; q += blockDim.x) dst[d0 + q] = src[so + q];
+    }
+}
+
+int vp_runs_move(float* dst, const float* src, const int64_t* runs, int nruns, bool to_compact,
+                 hipStream_t s) {
+    if (nruns <= 0) return CUBE2_OK;
+    k_runs_move<<<std::min(nruns, 16384), 256, 0, s>>>(dst, src, runs, nruns, to_compact ? 1 : 0);
+    C2_HIP(hipGetLastError());
+    return CUBE2_OK;
+}
+
 // Pairs-once pass: clears the fixed-point sums of the sorted particles.
 __global__ void k_react_zero(unsigned long long* __restrict__ react, const int* __restrict__ n_sorted,
                              int cap, const int* __restrict__ err) {

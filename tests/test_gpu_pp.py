@@ -326,3 +326,26 @@ def test_buffers_regrow_between_calls(lib, oracle_lib, mixed_layout, c0_layout):
         k, res = _dev_pp(lib, lay, ires=lv, mode=lib.PP_EXACT)
         o = _oracle_pp(oracle_lib, lay, k.ires)
         assert np.array_equal(k.vp_host().view(np.uint32), o["vp"].view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", ["ref", "fast"])
+def test_dropin_split_pass_equals_device_pass(lib, mixed_layout, mode):
+    """The drop-in call's split pass moves only the physical particles' vp
+    rows across PCIe: its whole vp (halo copies included, which keep their
+    input values) equals the device-resident PP pass bit for bit, and so do
+    f2max, vmax and the per-subtile counts."""
+    lay, ires = mixed_layout
+    m = {"ref": lib.PP_REF, "fast": lib.PP_FAST}[mode]
+    k, res = _dev_pp(lib, lay, ires=ires, mode=m)
+    want = k.vp_host()
+    phys = lay.physical_mask()
+    assert 0 < phys.sum() < phys.size
+    assert np.array_equal(want[~phys], lay.vp[~phys])
+    assert not np.array_equal(want[phys], lay.vp[phys])
+    lib.set_pp_mode(m)
+    lay_like = {"vp_in": lay.vp, "rhoc": lay.rhoc, "idx_b_r": lay.idx_b_r, "xp": lay.xp}
+    vp, f2, vm, pt = _dropin_pp(lib, lay.geom, lay_like, k.ires, 1.0, 0.02, 1e-3)
+    assert np.array_equal(vp.view(np.uint32), want.view(np.uint32))
+    assert np.float32(f2) == np.float32(res.f2max_pp)
+    assert np.array_equal(vm, res.vmax)
+    assert np.array_equal(pt, res.ptotal_pp)
